@@ -1,0 +1,151 @@
+"""Benchmark of the tt-fort local-solver hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+Prints one JSON line.  Workloads:
+
+  matvec  (default) -- the local projected matvec y = Phi_L A_k Phi_R x of the
+          scaled Heisenberg XXX eigenproblem (d=64, max TT rank 128, MPO rank
+          5, 5 eigenpairs): the largest local problem of that sweep,
+          (r_left, n, r_right, R) = (128, 2, 128, 5), applied to the block of
+          5 vectors the block eigensolver works on.  One step = one block
+          matvec on inputs already resident in HBM.
+
+The `roofline` object describes the dominant kernel (the MFMA GEMM that runs
+the two large contraction stages) measured with HIP events on the library
+stream; `cpu_baseline` times the oracle (numpy + host BLAS) on the same
+workload on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tt-fort_amd"))
+
+MI355X_PEAK_F64_TFLOPS = 78.6
+MI355X_PEAK_F32_TFLOPS = 157.3
+MI355X_HBM_GBS = 8000.0
+
+
+def bfun3_stage_flops(rx1, m, rx2, ry1, n, ry2, ra1, ra2):
+    return (2.0 * rx1 * m * rx2 * ra2 * ry2,
+            2.0 * ra1 * n * m * ra2 * ry2 * rx1,
+            2.0 * ry1 * n * ry2 * rx1 * ra1)
+
+
+def cpu_baseline_matvec(shape, nvec, budget_s=12.0):
+    """Time the oracle's bfun3 (the reference's GEMM chain on host BLAS)."""
+    from oracle import kernels as K
+    try:
+        from threadpoolctl import threadpool_info
+        cores = max([i.get("num_threads", 1) for i in threadpool_info()
+                     if i.get("user_api") == "blas"] or [1])
+    except Exception:
+        cores = os.cpu_count() or 1
+    rng = np.random.default_rng(1)
+    rx1, m, rx2, ry1, n, ry2, ra1, ra2 = shape
+    phi1 = rng.standard_normal(ry1 * rx1 * ra1)
+    A = rng.standard_normal(ra1 * n * m * ra2)
+    phi2 = rng.standard_normal(rx2 * ra2 * ry2)
+    X = rng.standard_normal((rx1 * m * rx2, nvec))
+    K.dbfun3(*shape, phi1, A, phi2, X[:, 0])   # warm
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        for v in range(nvec):
+            K.dbfun3(*shape, phi1, A, phi2, X[:, v])
+        reps += 1
+        el = time.perf_counter() - t0
+        if el > budget_s:
+            break
+    flops = sum(bfun3_stage_flops(*shape)) * nvec * reps
+    return dict(value=flops / el / 1e12, unit="TFLOP/s", cores=int(cores), kind="port",
+                sample="%d block matvecs (%d vectors each) of the oracle's dbfun3 GEMM chain "
+                       "in %.1f s" % (reps, nvec, el))
+
+
+def run_matvec(args):
+    import ttfort
+    lib = ttfort.load()
+    ttfort.check(lib.ttf_init(0), lib)
+    shape = (128, 2, 128, 128, 2, 128, 5, 5)
+    nvec = 5
+    rx1, m, rx2, ry1, n, ry2, ra1, ra2 = shape
+    N = rx1 * m * rx2
+    rng = np.random.default_rng(0)
+    d = dict(phi1=ttfort.DeviceArray.from_numpy(rng.standard_normal(ry1 * rx1 * ra1)),
+             A=ttfort.DeviceArray.from_numpy(rng.standard_normal(ra1 * n * m * ra2)),
+             phi2=ttfort.DeviceArray.from_numpy(rng.standard_normal(rx2 * ra2 * ry2)),
+             x=ttfort.DeviceArray.from_numpy(rng.standard_normal(N * nvec)),
+             y=ttfort.DeviceArray((N * nvec,)))
+
+    def step():
+        ttfort.check(lib.ttf_dbfun3(*shape, nvec, d["phi1"].ptr, d["A"].ptr, d["phi2"].ptr,
+                                    d["x"].ptr, N, d["y"].ptr, N), lib)
+
+    for _ in range(args.warmup):
+        step()
+    ttfort.sync(lib)
+    t0 = time.perf_counter()
+    with ttfort.Timer(lib) as tm:
+        for _ in range(args.steps):
+            step()
+    ttfort.sync(lib)
+    wall = time.perf_counter() - t0
+    ms_step = tm.ms / args.steps
+    fl = bfun3_stage_flops(*shape)
+    flops_step = sum(fl) * nvec
+    value = flops_step / (ms_step * 1e-3) / 1e12
+
+    # per-stage kernel durations (events around each launch on the library stream)
+    import ctypes
+    st = (ctypes.c_float * 3)()
+    ttfort.check(lib.ttf_dbfun3_profile(*shape, nvec, d["phi1"].ptr, d["A"].ptr, d["phi2"].ptr,
+                                        d["x"].ptr, N, d["y"].ptr, N, max(args.steps, 20), st), lib)
+    stage_ms = [float(v) for v in st]
+    # stages 1 and 3 run the same MFMA GEMM kernel and dominate the flops
+    dom_flops = (fl[0] + fl[2]) * nvec / 2.0
+    dom_ms = (stage_ms[0] + stage_ms[2]) / 2.0
+    achieved = dom_flops / (dom_ms * 1e-3) / 1e12
+    roofline = dict(bound="mfma", achieved=round(achieved, 3), peak=MI355X_PEAK_F64_TFLOPS,
+                    unit="TFLOP/s", frac=round(achieved / MI355X_PEAK_F64_TFLOPS, 4), traffic=None,
+                    kernel="gemm_mfma_kernel<double,32,32> (contraction stages 1 and 3)",
+                    stage_ms=[round(v, 5) for v in stage_ms])
+    out = dict(metric="local-matvec TFLOP/s (FP64, Heisenberg d=64 r=128 local problem)",
+               value=round(value, 4), unit="TFLOP/s", n_gpus=1, steps=args.steps,
+               warmup=args.warmup, ms_per_step=round(ms_step, 5), higher_is_better=True,
+               scaling="weak", vs_baseline=None, dtype="f64", data="synthetic",
+               config=dict(workload="eigb-local-matvec", r_left=rx1, n=m, r_right=rx2, R=ra2,
+                           block=nvec, wall_s=round(wall, 3)),
+               roofline=roofline)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_matvec(shape, nvec, budget_s=args.cpu_budget)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="matvec", choices=["matvec"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the local-solver path has no "
+              "multi-GPU mode); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    out = {"matvec": run_matvec}[args.workload](args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,153 @@
+/* ttfort_amd.h -- C ABI of the MI355X (gfx950) local-solver library for
+ * tensor-train sweeps, a drop-in for the local-operator hot path of
+ * oseledets/tt-fort.
+ *
+ * Two families of entry points:
+ *
+ *  1. Reference-compatible symbols.  Same names and by-reference argument
+ *     lists as the tt-fort `bind(c)` subroutines in tt_linalg.f90 (Fortran
+ *     default integers, host pointers, column-major arrays).  A Fortran
+ *     program that links this library instead of tt_linalg.o gets the GPU
+ *     path without source changes.  Workspace arguments (res1, res2, work)
+ *     are accepted and ignored: the library keeps its own device scratch.
+ *     These calls stage host arrays through PCIe on every call.
+ *
+ *  2. Native `ttf_*` entry points.  Plain int sizes, device pointers,
+ *     int status returns (0 = success), blocks of vectors, and the
+ *     device-resident sweep drivers.  These are the fast path.
+ *
+ * Layout conventions (the reference's): TT core k is (r_{k-1}, n_k, r_k)
+ * column-major; TT-matrix core k is (ra_{k-1}, n_k, m_k, ra_k) with n the
+ * row (output) index; left interface phi1 is (ry, rx, ra), right interface
+ * phi2 is (rx, ra, ry).
+ *
+ * Threading: one library state per process and device; calls from several
+ * host threads must be serialised by the caller.  Every kernel runs on the
+ * library's stream; native entry points are asynchronous unless stated.
+ */
+#ifndef TTFORT_AMD_H
+#define TTFORT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Fortran default INTEGER of the calling program: int32 for the reference's
+ * i4 presets, int64 for its i8 presets (Makefile.in:7-15).  The library is
+ * built once per kind: libttfort_amd.so (int32) and libttfort_amd_i8.so. */
+#ifdef TTF_FINT64
+typedef int64_t ttf_fint;
+#else
+typedef int32_t ttf_fint;
+#endif
+
+typedef struct { double re, im; } ttf_zdouble;
+
+/* ------------------------------------------------------------------ runtime */
+/* Select the device and create the library stream (idempotent).  Every other
+ * entry point initialises lazily on device 0 if this was not called. */
+int ttf_init(int device);
+/* Release scratch arenas and the stream. */
+int ttf_finalize(void);
+/* Last error message ("" if none); ttf_clear_error resets it. */
+const char* ttf_last_error(void);
+void ttf_clear_error(void);
+/* Library version string. */
+const char* ttf_version(void);
+
+/* Device memory helpers (used by tests and benchmarks; any hipMalloc'ed
+ * pointer is accepted by the native entry points). */
+int ttf_malloc(void** ptr, size_t bytes);
+int ttf_free(void* ptr);
+int ttf_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int ttf_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int ttf_memcpy_d2d(void* dst, const void* src, size_t bytes);
+int ttf_memset(void* dst, int value, size_t bytes);
+/* Wait for all work on the library stream. */
+int ttf_sync(void);
+/* Events on the library stream, for timing: ttf_timer_start records a start
+ * event, ttf_timer_stop records a stop event, waits for it and returns the
+ * elapsed milliseconds. */
+int ttf_timer_start(void);
+int ttf_timer_stop(float* ms);
+
+/* ------------------------------------------- local operator (bfun3 family) */
+/* y(c1,i,c2) = sum phi1(c1,b1,a1) A(a1,i,j,a2) phi2(b2,a2,c2) x(b1,j,b2)
+ * for nvec vectors; vector v of x starts at x + v*ldx, of y at y + v*ldy.
+ * y may alias x.  Replaces ttals.f90:9-32 / tt_linalg.f90:127-159 (dbfun3),
+ * one vector at a time in the reference (tt_eigb.f90:23-25). */
+int ttf_dbfun3(int rx1, int m, int rx2, int ry1, int n, int ry2, int ra1, int ra2,
+               int nvec, const double* phi1, const double* A, const double* phi2,
+               const double* x, int64_t ldx, double* y, int64_t ldy);
+/* Runs the three dbfun3 GEMM stages `iters` times with an event around each
+ * stage and returns the mean milliseconds per stage in stage_ms[3] (for the
+ * benchmark's per-kernel roofline).  Synchronous. */
+int ttf_dbfun3_profile(int rx1, int m, int rx2, int ry1, int n, int ry2, int ra1, int ra2,
+                       int nvec, const double* phi1, const double* A, const double* phi2,
+                       const double* x, int64_t ldx, double* y, int64_t ldy, int iters,
+                       float* stage_ms);
+/* Same contraction in FP32 on the FP32 matrix cores (mixed-precision path). */
+int ttf_sbfun3(int rx1, int m, int rx2, int ry1, int n, int ry2, int ra1, int ra2,
+               int nvec, const float* phi1, const float* A, const float* phi2,
+               const float* x, int64_t ldx, float* y, int64_t ldy);
+/* Complex128 version (ttals.f90:34-57, zbfun3). */
+int ttf_zbfun3(int rx1, int m, int rx2, int ry1, int n, int ry2, int ra1, int ra2,
+               int nvec, const ttf_zdouble* phi1, const ttf_zdouble* A,
+               const ttf_zdouble* phi2, const ttf_zdouble* x, int64_t ldx,
+               ttf_zdouble* y, int64_t ldy);
+/* y = B^H x for the zbfun3 operator B (ttals.f90:85-108, zbfun3_transp);
+ * x has shape (ry1,n,ry2), y has shape (rx1,m,rx2). */
+int ttf_zbfun3_transp(int rx1, int m, int rx2, int ry1, int n, int ry2, int ra1, int ra2,
+                      int nvec, const ttf_zdouble* phi1, const ttf_zdouble* A,
+                      const ttf_zdouble* phi2, const ttf_zdouble* x, int64_t ldx,
+                      ttf_zdouble* y, int64_t ldy);
+
+/* ------------------------------------------------------- interface updates */
+/* phi1new(c2,b2,a2) = sum phi1(c1,b1,a1) y(c1,i,c2) A(a1,i,j,a2) x(b1,j,b2),
+ * output (ry2, rx2, ra2).  Replaces tt_linalg.f90:287-312 / ttals.f90:203-227
+ * (dphi_left). */
+int ttf_dphi_left(int rx1, int m, int rx2, int ry1, int n, int ry2, int ra1, int ra2,
+                  const double* phi1, const double* A, const double* x, const double* y,
+                  double* phi1new);
+/* phi2new(b1,a1,c1) = sum phi2(b2,a2,c2) x(b1,j,b2) A(a1,i,j,a2) y(c1,i,c2),
+ * output (rx1, ra1, ry1).  Replaces tt_linalg.f90:259-284 / ttals.f90:150-173
+ * (dphi_right). */
+int ttf_dphi_right(int rx1, int m, int rx2, int ry1, int n, int ry2, int ra1, int ra2,
+                   const double* phi2, const double* A, const double* x, const double* y,
+                   double* phi2new);
+/* Complex versions with conj(y) (ttals.f90:229-256 zphi_left, :176-200
+ * zphi_right). */
+int ttf_zphi_left(int rx1, int m, int rx2, int ry1, int n, int ry2, int ra1, int ra2,
+                  const ttf_zdouble* phi1, const ttf_zdouble* A, const ttf_zdouble* x,
+                  const ttf_zdouble* y, ttf_zdouble* phi1new);
+int ttf_zphi_right(int rx1, int m, int rx2, int ry1, int n, int ry2, int ra1, int ra2,
+                   const ttf_zdouble* phi2, const ttf_zdouble* A, const ttf_zdouble* x,
+                   const ttf_zdouble* y, ttf_zdouble* phi2new);
+/* Interfaces without an operator (y'x projections), tt_linalg.f90:329-340
+ * (dphi2_left, output (ry2,rx2)) and :315-327 (dphi2_right, output (rx1,ry1)). */
+int ttf_dphi2_left(int rx1, int rx2, int ry1, int n, int ry2, const double* phi1,
+                   const double* x, const double* y, double* phi1new);
+int ttf_dphi2_right(int rx1, int rx2, int ry1, int n, int ry2, const double* phi2,
+                    const double* x, const double* y, double* phi2new);
+
+/* Explicit local matrix B((c1,i,c2),(b1,j,b2)), N x N column-major with
+ * N = rx1*m*rx2 (tt_eigb.f90:88-105, d2d_fullmat). */
+int ttf_d2d_fullmat(int rx1, int m, int rx2, int ra1, int ra2, const double* phi1,
+                    const double* A, const double* phi2, double* B);
+
+/* ------------------------------------------ reference-compatible symbols */
+/* tt_linalg.f90:127-159, dbfun3 bind(c).  Host pointers; res1/res2 ignored. */
+void dbfun3(const ttf_fint* rx1, const ttf_fint* m, const ttf_fint* rx2,
+            const ttf_fint* ry1, const ttf_fint* n, const ttf_fint* ry2,
+            const ttf_fint* ra1, const ttf_fint* ra2, const double* phi1,
+            const double* A, const double* phi2, const double* x, double* y,
+            double* res1, double* res2);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TTFORT_AMD_H */

@@ -1,0 +1,426 @@
+"""Oracle restatement of the tt-fort local-operator kernels (TEST INFRASTRUCTURE).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use
+this module.  Every routine follows the reference's own GEMM / transpose
+sequence on column-major (Fortran-order) flat arrays, so that intermediate
+layouts match the reference exactly.  Arrays are 1-D numpy arrays holding the
+column-major data of the reference's Fortran arrays.
+
+Pinning (see tests/test_oracle_kernels.py):
+  * bfun3 / phi / full-matrix kernels are checked against the index
+    definitions written in the reference comments (ttals.f90:18-29, :159-170,
+    :213-224) evaluated by an independent Kronecker-product construction of
+    the local matrix, against each other (phi_left of x with y == y' B x), and
+    against the exact Heisenberg spectrum from the reference's own MPO
+    fixture Heisen_20.sdv.
+  * chop rules are checked against hand-worked singular-value cases.
+  * QR/SVD against orthogonality and reconstruction invariants.
+  * GMRES against dense solves; block-Jacobi against block inverses.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+__all__ = [
+    "F", "dtransp", "dperm1324",
+    "dbfun3", "zbfun3", "zbfun3_transp", "dbfun3_right", "dbfun3_left",
+    "dphi_left", "dphi_right", "zphi_left", "zphi_right",
+    "dphi2_left", "dphi2_right", "d2d_fullmat", "local_matrix_kron",
+    "my_chop3", "chop", "dqr", "djac_gen", "djac_apply", "dgmresr_hh",
+    "bfun3_flops",
+]
+
+
+def F(a, *shape):
+    """View flat column-major data `a` as a Fortran array of `shape`."""
+    return np.reshape(a, shape, order="F")
+
+
+def _flat(a):
+    return np.reshape(a, -1, order="F")
+
+
+def dtransp(n, m, a):
+    """matrix_util.f90:152-170 / tt_linalg.f90:82-99: (n x m) -> (m x n)."""
+    return _flat(F(a, n, m).T)
+
+
+def dperm1324(n1, n2, n3, n4, a):
+    """tt_linalg.f90:101-119: B(n1,n3,n2,n4) = A(n1,n2,n3,n4)."""
+    return _flat(np.transpose(F(a, n1, n2, n3, n4), (0, 2, 1, 3)))
+
+
+def bfun3_flops(rx1, m, rx2, ry1, n, ry2, ra1, ra2, complex_=False):
+    """Algorithmic flops of one bfun3 application (the three reference GEMMs,
+    ttals.f90:20, :24, :29); x4 for complex arithmetic."""
+    f = (2.0 * rx1 * m * rx2 * ra2 * ry2
+         + 2.0 * ra1 * n * m * ra2 * ry2 * rx1
+         + 2.0 * ry1 * n * ry2 * rx1 * ra1)
+    return 4.0 * f if complex_ else f
+
+
+# ---------------------------------------------------------------- bfun3 family
+def dbfun3(rx1, m, rx2, ry1, n, ry2, ra1, ra2, phi1, A, phi2, x):
+    """ttals.f90:9-32 (== tt_linalg.f90:127-159 bind(c), :185-207 dbfun32).
+
+    y(c1,i,c2) = sum phi1(c1,b1,a1) A(a1,i,j,a2) phi2(b2,a2,c2) x(b1,j,b2)
+    """
+    res1 = F(x, rx1 * m, rx2) @ F(phi2, rx2, ra2 * ry2)            # :20
+    res1 = dtransp(rx1, m * ra2 * ry2, _flat(res1))                  # :22
+    res2 = F(A, ra1 * n, m * ra2) @ F(res1, m * ra2, ry2 * rx1)      # :24
+    res2 = dtransp(ra1 * n * ry2, rx1, _flat(res2))                  # :26
+    y = F(phi1, ry1, rx1 * ra1) @ F(res2, rx1 * ra1, n * ry2)        # :29
+    return _flat(y)
+
+
+def zbfun3(rx1, m, rx2, ry1, n, ry2, ra1, ra2, phi1, A, phi2, x):
+    """ttals.f90:34-57: complex bfun3, no conjugation anywhere."""
+    return dbfun3(rx1, m, rx2, ry1, n, ry2, ra1, ra2,
+                  np.asarray(phi1, complex), np.asarray(A, complex),
+                  np.asarray(phi2, complex), np.asarray(x, complex))
+
+
+def zbfun3_transp(rx1, m, rx2, ry1, n, ry2, ra1, ra2, phi1, A, phi2, x):
+    """ttals.f90:85-108: y = B^H x for the local matrix B of zbfun3.
+
+    x has the output shape (ry1,n,ry2); y has the input shape (rx1,m,rx2).
+    """
+    res1 = F(x, ry1, n * ry2).conj().T @ F(phi1, ry1, rx1 * ra1)       # :95
+    res1 = dtransp(n, ry2 * rx1 * ra1, _flat(res1))                   # :98
+    res2 = F(res1, ry2 * rx1, ra1 * n) @ F(A, ra1 * n, m * ra2)        # :100
+    res2 = dtransp(ry2, rx1 * m * ra2, _flat(res2))                   # :103
+    y = F(res2, rx1 * m, ra2 * ry2) @ F(phi2, rx2, ra2 * ry2).T        # :105
+    return _flat(y).conj()                                            # :107
+
+
+def dbfun3_right(rx1, m, rx2, ry1, n, ry2, ra1, ra2, A, phi2, x):
+    """tt_linalg.f90:210-240: the first two bfun3 stages only.
+
+    Output layout (rx1*ra1, n, ry2) = (b1,a1,i,c2).
+    """
+    res1 = F(x, rx1 * m, rx2) @ F(phi2, rx2, ra2 * ry2)                # :223
+    res1 = dtransp(rx1, m * ra2 * ry2, _flat(res1))                    # :228
+    res2 = F(A, ra1 * n, m * ra2) @ F(res1, m * ra2, ry2 * rx1)        # :232
+    return dtransp(ra1 * n * ry2, rx1, _flat(res2))                    # :237
+
+
+def dbfun3_left(rx1, m, rx2, ry1, n, ry2, ra1, ra2, phi1, A, x):
+    """tt_linalg.f90:242-256, restated as the reference executes it.
+
+    Output y of size (ry1, n, rx2, ra2).  NOTE: the reference computes the
+    transposed copy res1 = phi1^T (:249) but then passes the untransposed
+    `phi1` to the GEMM at :250 with leading dimension ra1*ry1.  For ra1 > 1
+    this reads phi1 in a different index order than the comment intends.
+    The restatement keeps the reference's behaviour; `dbfun3_left_intended`
+    gives the contraction the comments describe.
+    """
+    res2 = F(phi1, ra1 * ry1, rx1) @ F(x, rx1, m * rx2)                # :250
+    res1 = dperm1324(ra1, ry1, m, rx2, _flat(res2))                    # :251
+    res2 = dperm1324(ra1, n, m, ra2, A)                                # :252
+    y = F(res1, ra1 * m, ry1 * rx2).T @ F(res2, ra1 * m, n * ra2)      # :253
+    return dperm1324(ry1, rx2, n, ra2, _flat(y))                       # :254
+
+
+def dbfun3_left_intended(rx1, m, rx2, ry1, n, ry2, ra1, ra2, phi1, A, x):
+    """The contraction tt_linalg.f90:242-256 describes:
+    y(c1,i,b2,a2) = sum phi1(c1,b1,a1) A(a1,i,j,a2) x(b1,j,b2)."""
+    t = np.einsum("cba,aijk,bjd->cidk",
+                  F(phi1, ry1, rx1, ra1), F(A, ra1, n, m, ra2), F(x, rx1, m, rx2))
+    return _flat(t)
+
+
+# ---------------------------------------------------------------- phi updates
+def dphi_right(rx1, m, rx2, ry1, n, ry2, ra1, ra2, phi2_old, A, x, y):
+    """tt_linalg.f90:259-284 (== ttals.f90:150-173).
+
+    phi2(b1,a1,c1) = sum phi2_old(b2,a2,c2) x(b1,j,b2) A(a1,i,j,a2) y(c1,i,c2)
+    Output layout (rx1, ra1, ry1).
+    """
+    res1 = F(x, rx1 * m, rx2) @ F(phi2_old, rx2, ra2 * ry2)            # :270
+    res1 = dtransp(rx1, m * ra2 * ry2, _flat(res1))                    # :272
+    res2 = F(A, ra1 * n, m * ra2) @ F(res1, m * ra2, ry2 * rx1)        # :275
+    res1 = dtransp(ra1, n * ry2 * rx1, _flat(res2))                    # :277
+    phi2 = F(y, ry1, n * ry2) @ F(res1, n * ry2, rx1 * ra1)            # :279
+    return dtransp(ry1, rx1 * ra1, _flat(phi2))                        # :281
+
+
+def dphi_left(rx1, m, rx2, ry1, n, ry2, ra1, ra2, phi1_old, A, x, y):
+    """tt_linalg.f90:287-312 (== ttals.f90:203-227).
+
+    phi1(c2,b2,a2) = sum phi1_old(c1,b1,a1) y(c1,i,c2) A(a1,i,j,a2) x(b1,j,b2)
+    Output layout (ry2, rx2, ra2).
+    """
+    res1 = F(phi1_old, ry1, rx1 * ra1).T @ F(y, ry1, n * ry2)          # :298
+    res1 = dtransp(rx1, ra1 * n * ry2, _flat(res1))                    # :300
+    res2 = F(res1, ra1 * n, ry2 * rx1).T @ F(A, ra1 * n, m * ra2)      # :303
+    res1 = dtransp(ry2, rx1 * m * ra2, _flat(res2))                    # :305
+    phi1 = F(res1, rx1 * m, ra2 * ry2).T @ F(x, rx1 * m, rx2)          # :307
+    return dtransp(ra2, ry2 * rx2, _flat(phi1))                        # :309
+
+
+def zphi_right(rx1, m, rx2, ry1, n, ry2, ra1, ra2, phi2_old, A, x, y):
+    """ttals.f90:176-200: complex phi_right with conj(y) (:186)."""
+    return dphi_right(rx1, m, rx2, ry1, n, ry2, ra1, ra2,
+                      np.asarray(phi2_old, complex), np.asarray(A, complex),
+                      np.asarray(x, complex), np.conj(np.asarray(y, complex)))
+
+
+def zphi_left(rx1, m, rx2, ry1, n, ry2, ra1, ra2, phi1_old, A, x, y):
+    """ttals.f90:229-256: complex phi_left with conj(y) (:239)."""
+    return dphi_left(rx1, m, rx2, ry1, n, ry2, ra1, ra2,
+                     np.asarray(phi1_old, complex), np.asarray(A, complex),
+                     np.asarray(x, complex), np.conj(np.asarray(y, complex)))
+
+
+def dphi2_right(rx1, rx2, ry1, n, ry2, phi2_old, x, y):
+    """tt_linalg.f90:315-327: phi2(b1,c1) = sum x(b1,j,b2) phi2_old(b2,c2) y(c1,j,c2).
+
+    Output layout (rx1, ry1).
+    """
+    res1 = F(x, rx1 * n, rx2) @ F(phi2_old, rx2, ry2)                  # :324
+    phi2 = F(res1, rx1, n * ry2) @ F(y, ry1, n * ry2).T                # :326
+    return _flat(phi2)
+
+
+def dphi2_left(rx1, rx2, ry1, n, ry2, phi1_old, x, y):
+    """tt_linalg.f90:329-340: phi1(c2,b2) = sum phi1_old(c1,b1) y(c1,i,c2) x(b1,i,b2).
+
+    Output layout (ry2, rx2).
+    """
+    res1 = F(phi1_old, ry1, rx1).T @ F(y, ry1, n * ry2)                # :337
+    phi1 = F(res1, rx1 * n, ry2).T @ F(x, rx1 * n, rx2)                # :339
+    return _flat(phi1)
+
+
+# ---------------------------------------------------------------- full matrix
+def d2d_fullmat(rx1, m, rx2, ra1, ra2, phi1, A, phi2):
+    """tt_eigb.f90:88-105: explicit local matrix B((c1,i,c2),(b1,j,b2)).
+
+    Returned as a dense (N x N) Fortran-order matrix, N = rx1*m*rx2, with rows
+    indexing y and columns indexing x.
+    """
+    res1 = F(phi1, rx1 * rx1, ra1) @ F(A, ra1, m * m * ra2)            # :97
+    res2 = dperm1324(rx1, rx1, m, m * ra2, _flat(res1))                # :99
+    phi2t = dtransp(rx2, ra2 * rx2, phi2)                              # :101
+    res1 = F(res2, rx1 * m * rx1 * m, ra2) @ F(phi2t, ra2, rx2 * rx2)  # :102
+    B = dperm1324(rx1 * m, rx1 * m, rx2, rx2, _flat(res1))             # :103
+    N = rx1 * m * rx2
+    return F(B, N, N)
+
+
+def local_matrix_kron(ry1, n, ry2, rx1, m, rx2, ra1, ra2, phi1, A, phi2):
+    """Independent construction of the local matrix used to pin the kernels:
+    B = sum_{a1,a2} phi2[:, a2, :]^T (x) A[a1,:,:,a2] (x) phi1[:, :, a1]
+    (Kronecker order matches the column-major multi-index (c1,i,c2))."""
+    P1 = F(phi1, ry1, rx1, ra1)
+    AA = F(A, ra1, n, m, ra2)
+    P2 = F(phi2, rx2, ra2, ry2)
+    dt = np.result_type(P1, AA, P2)
+    B = np.zeros((ry1 * n * ry2, rx1 * m * rx2), dtype=dt)
+    for a1 in range(ra1):
+        for a2 in range(ra2):
+            B += np.kron(P2[:, a2, :].T, np.kron(AA[a1, :, :, a2], P1[:, :, a1]))
+    return B
+
+
+# ---------------------------------------------------------------- truncation
+def my_chop3(s, eps):
+    """tt_adapt_als.f90:40-61 (== matrix_util.f90:254-275).
+
+    Smallest r such that sum_{k>r} s_k^2 <= (eps*||s||)^2, r >= 1.
+    """
+    s = np.asarray(s, float)
+    n = s.size
+    nrm = np.sqrt(np.sum(s * s))
+    nrm = (nrm * eps) * (nrm * eps)
+    cursum = 0.0
+    i = n
+    while i > 0:
+        cursum += s[i - 1] * s[i - 1]
+        if cursum > nrm:
+            break
+        i -= 1
+    return min(i + 1, n)
+
+
+def chop(s, tol=None, rmax=None):
+    """svd.f90:86-111: Frobenius chop with optional rank cap.
+
+    Returns (r, err) with err = sqrt of the discarded sum of squares.
+    """
+    s = np.asarray(s, float)
+    r = s.size
+    er2 = 0.0
+    if rmax is not None and rmax < r:
+        er2 = float(np.dot(s[rmax:r], s[rmax:r]))
+        r = rmax
+    if tol is not None:
+        nrm = np.sqrt(np.sum(s * s))
+        bound = tol * tol * nrm * nrm
+        er = er2 + s[r - 1] * s[r - 1]
+        while er < bound:
+            er2 = er
+            r -= 1
+            er = er + s[r - 1] * s[r - 1]
+    return r, float(np.sqrt(er2))
+
+
+def dqr(n, m, A):
+    """tt_linalg.f90:30-51 / matrix_util.f90:37-61: thin Householder QR.
+
+    Returns (Q flat n x min(n,m), R flat min(n,m) x m).  LAPACK's sign
+    convention (dgeqrf) is kept by numpy.linalg.qr, which calls the same
+    routines; comparisons should still be made up to column signs.
+    """
+    Q, R = np.linalg.qr(F(A, n, m), mode="reduced")
+    return _flat(Q), _flat(R)
+
+
+# ---------------------------------------------------------------- block Jacobi
+def djac_gen(ptype, rx1, n, rx2, ra1, ra2, Phi1, A, Phi2):
+    """tt_linalg.f90:363-495: block-Jacobi preconditioner blocks.
+
+    'c': rx2*rx1 blocks of n x n, jacs laid out (n, n, rx2, rx1) (:397-429);
+    'l': rx2 blocks of (rx1*n)^2 (:434-462); 'r': rx1 blocks of (rx2*n)^2
+    (:464-491).  Each block is the inverse of the matching diagonal block of
+    the local matrix restricted as described by the reference.
+    """
+    P1 = F(Phi1, rx1, rx1, ra1)
+    AA = F(A, ra1, n, n, ra2)
+    P2 = F(Phi2, rx2, ra2, rx2)
+    p = ptype.lower()
+    if p == "c":
+        d1 = np.stack([np.diag(P1[:, :, a]) for a in range(ra1)], axis=1)   # (rx1, ra1)
+        d2 = np.stack([np.diag(P2[:, a, :]) for a in range(ra2)], axis=1)   # (rx2, ra2)
+        # blk(b1, i, j, b2) = sum d1(b1,a1) A(a1,i,j,a2) d2(b2,a2)   (:412-416)
+        blk = np.einsum("pa,aijk,qk->pijq", d1, AA, d2)
+        out = np.empty((n, n, rx2, rx1))
+        for b1 in range(rx1):
+            for b2 in range(rx2):
+                out[:, :, b2, b1] = np.linalg.inv(blk[b1, :, :, b2])
+        return _flat(out)
+    if p == "l":
+        d2 = np.stack([np.diag(P2[:, a, :]) for a in range(ra2)], axis=1)
+        # blk((c1,i),(b1,j), b2) = sum Phi1(c1,b1,a1) A(a1,i,j,a2) d2(b2,a2)
+        blk = np.einsum("cba,aijk,qk->cibjq", P1, AA, d2)
+        out = np.empty((rx1 * n, rx1 * n, rx2))
+        for b2 in range(rx2):
+            out[:, :, b2] = np.linalg.inv(np.reshape(blk[:, :, :, :, b2], (rx1 * n, rx1 * n), order="F"))
+        return _flat(out)
+    if p == "r":
+        d1 = np.stack([np.diag(P1[:, :, a]) for a in range(ra1)], axis=1)
+        # blk((i,c2),(j,b2), b1) = sum d1(b1,a1) A(a1,i,j,a2) Phi2(b2,a2,c2)
+        blk = np.einsum("pa,aijk,dke->iejdp", d1, AA, P2)
+        out = np.empty((n * rx2, n * rx2, rx1))
+        for b1 in range(rx1):
+            out[:, :, b1] = np.linalg.inv(np.reshape(blk[:, :, :, :, b1], (n * rx2, n * rx2), order="F"))
+        return _flat(out)
+    raise ValueError(ptype)
+
+
+def djac_apply(ptype, rx1, n, rx2, jacs, x):
+    """tt_linalg.f90:497-533: y = P x with the block-Jacobi blocks."""
+    p = ptype.lower()
+    X = F(x, rx1, n, rx2)
+    if p == "c":
+        J = F(jacs, n, n, rx2, rx1)
+        Y = np.einsum("ijqp,pjq->piq", J, X)
+        return _flat(Y)
+    if p == "l":
+        J = F(jacs, rx1 * n, rx1 * n, rx2)
+        Xm = F(x, rx1 * n, rx2)
+        return _flat(np.einsum("abq,bq->aq", J, Xm))
+    if p == "r":
+        J = F(jacs, n * rx2, n * rx2, rx1)
+        Xt = np.reshape(np.transpose(X, (1, 2, 0)), (n * rx2, rx1), order="F")
+        Y = np.einsum("abp,bp->ap", J, Xt)
+        return _flat(np.transpose(F(_flat(Y), n, rx2, rx1), (2, 0, 1)))
+    if p == "n":
+        return np.array(x, copy=True)
+    raise ValueError(ptype)
+
+
+# ---------------------------------------------------------------- GMRES
+def dgmresr_hh(matvec, rhs, nrestart, tol, niters, prec=None):
+    """tt_linalg.f90:651-845: right-preconditioned restarted GMRES with
+    Householder Arnoldi.  Solves B P z = rhs and returns z (the caller applies
+    P, as tt_adapt_als.f90:776-781 does).  `matvec(v)` applies the local
+    operator B; `prec(v)` applies P (None for ptype 'n').
+
+    Returns (sol, info) where info = dict(it, j, relres).
+    """
+    sz = rhs.size
+    nrestart = min(nrestart, sz)
+    U = np.zeros((sz, nrestart + 1))
+    tau = np.zeros(nrestart + 1)
+    JJ = np.zeros((2, nrestart))
+    R = np.zeros((nrestart, nrestart))
+    sol = np.zeros(sz)
+    P = prec if prec is not None else (lambda v: v)
+    curres = np.inf
+    nrmrhs = None
+    it = 0
+    j = 0
+    for it in range(niters):
+        w = matvec(P(sol)) - rhs                                         # :686-695
+        nrmr = np.linalg.norm(w)
+        if it == 0:
+            nrmrhs = nrmr
+        if nrmr == 0.0:
+            break
+        dbeta = nrmr if w[0] >= 0 else -nrmr                              # :709-712
+        w[0] += dbeta
+        tau[:] = 0.0
+        tau[0] = -dbeta
+        w /= np.linalg.norm(w)
+        U[:, 0] = w
+        jdone = nrestart
+        for j in range(1, nrestart + 1):                                  # :720
+            w = -2.0 * U[j - 1, j - 1] * U[:, j - 1]                      # :722-725
+            w[j - 1] += 1.0
+            for i in range(j - 1, 0, -1):                                 # :726-729
+                w += -2.0 * np.dot(U[:, i - 1], w) * U[:, i - 1]
+            w /= np.linalg.norm(w)                                        # :730-732
+            w = matvec(P(w))                                              # :735-743
+            for i in range(1, j + 1):                                     # :746-749
+                w += -2.0 * np.dot(U[:, i - 1], w) * U[:, i - 1]
+            if j < sz:                                                    # :752-773
+                U[:j, j] = 0.0
+                U[j:, j] = w[j:]
+                dalpha = np.linalg.norm(U[j:, j])
+                if dalpha != 0.0:
+                    if w[j] < 0.0:
+                        dalpha = -dalpha
+                    U[j, j] += dalpha
+                    U[j:, j] /= np.linalg.norm(U[j:, j])
+                    w[j] = -dalpha
+                    w[j + 1:] = 0.0
+            for i in range(1, j):                                         # :776-780
+                db = w[i - 1]
+                w[i - 1] = JJ[0, i - 1] * w[i - 1] + JJ[1, i - 1] * w[i]
+                w[i] = -JJ[1, i - 1] * db + JJ[0, i - 1] * w[i]
+            if j < sz:                                                    # :783-792
+                dalpha = np.hypot(w[j - 1], w[j])
+                JJ[0, j - 1] = w[j - 1] / dalpha
+                JJ[1, j - 1] = w[j] / dalpha
+                tau[j] = -JJ[1, j - 1] * tau[j - 1]
+                tau[j - 1] = JJ[0, j - 1] * tau[j - 1]
+                w[j - 1] = dalpha
+                w[j] = 0.0
+            R[:j, j - 1] = w[:j]                                          # :794
+            curres = abs(tau[j]) / nrmrhs                                 # :797
+            if curres < tol:
+                jdone = j
+                break
+        j = min(jdone, nrestart)                                          # :808-814
+        t = np.linalg.solve(np.triu(R[:j, :j]), tau[:j])                  # :816 dtrsv
+        tau[:j] = t
+        w = -2.0 * U[j - 1, j - 1] * tau[j - 1] * U[:, j - 1]             # :819-821
+        w[j - 1] += tau[j - 1]
+        for i in range(j - 1, 0, -1):                                     # :823-827
+            w[i - 1] += tau[i - 1]
+            w += -2.0 * np.dot(U[:, i - 1], w) * U[:, i - 1]
+        sol -= w                                                          # :828-829
+        if curres < tol:
+            break
+    return sol, dict(it=it, j=j, relres=curres)

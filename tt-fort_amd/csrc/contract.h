@@ -1,0 +1,51 @@
+// Contraction chains of the local operator and interface updates
+// (implemented in contract.hip) for use by the solvers and drivers.
+#pragma once
+
+#include "gemm_mfma.h"
+
+namespace ttf {
+
+// Sizes of one local problem: x (rx1,m,rx2), y (ry1,n,ry2), A (ra1,n,m,ra2).
+struct Shape {
+  int rx1, m, rx2, ry1, n, ry2, ra1, ra2;
+};
+
+template <typename T>
+void bfun3_chain(const Shape& s, int nvec, const T* phi1, const T* A, const T* phi2, const T* x,
+                 int64_t ldx, T* y, int64_t ldy, hipStream_t st);
+template <typename T>
+void bfun3_stage12(const Shape& s, int nvec, const T* A, const T* phi2, const T* x, int64_t ldx,
+                   T* T1, T* T2, hipStream_t st);
+
+void zbfun3_chain(const Shape& s, int nvec, const hipDoubleComplex* phi1, const hipDoubleComplex* A,
+                  const hipDoubleComplex* phi2, const hipDoubleComplex* x, int64_t ldx,
+                  hipDoubleComplex* y, int64_t ldy, hipStream_t st);
+void zbfun3_transp_chain(const Shape& s, int nvec, const hipDoubleComplex* phi1,
+                         const hipDoubleComplex* A, const hipDoubleComplex* phi2,
+                         const hipDoubleComplex* x, int64_t ldx, hipDoubleComplex* y, int64_t ldy,
+                         hipStream_t st);
+
+template <typename T>
+void phi_left_chain(const Shape& s, const T* phi1, const T* A, const T* x, const T* y, T* out,
+                    hipStream_t st);
+template <typename T>
+void phi_right_chain(const Shape& s, const T* phi2, const T* A, const T* x, const T* y, T* out,
+                     hipStream_t st);
+void zphi_left_chain(const Shape& s, const hipDoubleComplex* phi1, const hipDoubleComplex* A,
+                     const hipDoubleComplex* x, const hipDoubleComplex* y, hipDoubleComplex* out,
+                     hipStream_t st);
+void zphi_right_chain(const Shape& s, const hipDoubleComplex* phi2, const hipDoubleComplex* A,
+                      const hipDoubleComplex* x, const hipDoubleComplex* y, hipDoubleComplex* out,
+                      hipStream_t st);
+template <typename T>
+void phi2_left_chain(int rx1, int rx2, int ry1, int n, int ry2, const T* phi1, const T* x,
+                     const T* y, T* out, hipStream_t st);
+template <typename T>
+void phi2_right_chain(int rx1, int rx2, int ry1, int n, int ry2, const T* phi2, const T* x,
+                      const T* y, T* out, hipStream_t st);
+
+void fullmat_launch(int r1, int nn, int r2, int ra1, int ra2, const double* phi1, const double* A,
+                    const double* phi2, double* B, hipStream_t st);
+
+}  // namespace ttf

@@ -1,0 +1,420 @@
+// Strided, batched GEMM on the CDNA4 matrix cores (gfx950).
+//
+//   C[b](i,j) = alpha * sum_k A[b](i,k) * B[b](k,j) + beta * C[b](i,j)
+//
+// Every operand is addressed through explicit element strides, so the
+// transposes and index permutations of the reference's GEMM chains
+// (dgemm + dtransp, ttals.f90 / tt_linalg.f90) become stride choices and are
+// never materialised.  Batch index b is split in two levels,
+// b -> (b % nb0, b / nb0), each with its own stride, which covers "one GEMM
+// per vector of a block" and "one GEMM per bond index" at once.
+//
+// Tiling: 256 threads = 4 wave64s arranged 2x2; each wave owns a
+// (BM/2)x(BN/2) sub-tile made of 16x16 MFMA fragments
+// (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32).  K advances in chunks of
+// 16 staged through LDS; the next chunk is prefetched into registers while
+// the current one is consumed.  The global->LDS loader picks the coalesced
+// index order from the operand strides.  Work-group ids are remapped so that
+// neighbouring output tiles land on the same XCD (L2) of the 8-XCD chip.
+//
+// Split-K: with ksplit > 1 each work-group reduces a K range into a
+// workspace slice; gemm_splitk_reduce then sums the slices in a fixed order
+// (deterministic) and applies alpha/beta.
+#pragma once
+
+#include "ttf_common.h"
+
+namespace ttf {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Acc4;
+template <> struct Acc4<double> { typedef d4_t type; };
+template <> struct Acc4<float> { typedef f4_t type; };
+
+__device__ __forceinline__ d4_t mfma16(double a, double b, d4_t c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4_t mfma16(float a, float b, f4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+template <typename T>
+struct GemmOp {
+  int M, N, K;
+  const T* A; int64_t sAi, sAk, sAb0, sAb1;
+  const T* B; int64_t sBk, sBj, sBb0, sBb1;
+  T* C;       int64_t sCi, sCj, sCb0, sCb1;
+  int nb0;    // inner batch extent (>= 1)
+  int nbatch; // total batch count
+  T alpha, beta;
+  int ksplit;     // number of K ranges (>= 1)
+  T* work;        // ksplit * nbatch * M * N workspace when ksplit > 1
+};
+
+// Map the hardware work-group id so that consecutive logical tiles share an
+// XCD: the dispatcher hands work-groups to the 8 XCDs round-robin.
+__device__ __forceinline__ int xcd_remap(int wg, int total) {
+  const int nx = 8;
+  int full = (total / nx) * nx;
+  if (wg >= full) return wg;
+  int per = total / nx;
+  return (wg % nx) * per + (wg / nx);
+}
+
+template <typename T, int BM, int BN>
+__global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmOp<T> op) {
+  constexpr int BK = 16;
+  constexpr int PAD = 2;
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int FM = WM / 16, FN = WN / 16;
+  constexpr int LA = BM * BK / 256;   // A elements per thread
+  constexpr int LB = BN * BK / 256;   // B elements per thread
+  typedef typename Acc4<T>::type acc_t;
+
+  __shared__ T As[BK][BM + PAD];
+  __shared__ T Bs[BK][BN + PAD];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+
+  const int tilesM = (op.M + BM - 1) / BM;
+  const int tilesN = (op.N + BN - 1) / BN;
+  const int ntiles = tilesM * tilesN;
+  const int tile = xcd_remap(blockIdx.x, ntiles);
+  const int tm = tile / tilesN, tn = tile % tilesN;
+
+  const int b = blockIdx.y;
+  const int b0 = b % op.nb0, b1 = b / op.nb0;
+  const T* __restrict__ A = op.A + b0 * op.sAb0 + b1 * op.sAb1;
+  const T* __restrict__ B = op.B + b0 * op.sBb0 + b1 * op.sBb1;
+
+  // K range of this work-group
+  const int kchunk = ((op.K + op.ksplit - 1) / op.ksplit + BK - 1) / BK * BK;
+  const int kbeg = blockIdx.z * kchunk;
+  const int kend = min(op.K, kbeg + kchunk);
+
+  const bool a_ifast = (op.sAi == 1);
+  const bool b_jfast = (op.sBj == 1);
+
+  T ra[LA], rb[LB];
+  int ai[LA], ak[LA], bk[LB], bj[LB];
+#pragma unroll
+  for (int q = 0; q < LA; ++q) {
+    int idx = tid + q * 256;
+    if (a_ifast) { ai[q] = idx % BM; ak[q] = idx / BM; }
+    else         { ak[q] = idx % BK; ai[q] = idx / BK; }
+  }
+#pragma unroll
+  for (int q = 0; q < LB; ++q) {
+    int idx = tid + q * 256;
+    if (b_jfast) { bj[q] = idx % BN; bk[q] = idx / BN; }
+    else         { bk[q] = idx % BK; bj[q] = idx / BK; }
+  }
+
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < LA; ++q) {
+      int gi = tm * BM + ai[q], gk = k0 + ak[q];
+      ra[q] = (gi < op.M && gk < kend) ? A[gi * op.sAi + gk * op.sAk] : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < LB; ++q) {
+      int gj = tn * BN + bj[q], gk = k0 + bk[q];
+      rb[q] = (gj < op.N && gk < kend) ? B[gk * op.sBk + gj * op.sBj] : T(0);
+    }
+  };
+
+  acc_t acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+  if (kbeg < kend) load(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < LA; ++q) As[ak[q]][ai[q]] = ra[q];
+#pragma unroll
+    for (int q = 0; q < LB; ++q) Bs[bk[q]][bj[q]] = rb[q];
+    __syncthreads();
+    if (k0 + BK < kend) load(k0 + BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      T af[FM], bf[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) af[i] = As[kk + (lane >> 4)][wm * WM + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < FN; ++j) bf[j] = Bs[kk + (lane >> 4)][wn * WN + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = mfma16(af[i], bf[j], acc[i][j]);
+    }
+  }
+
+  // Epilogue.  D fragment layout: lane l, register r -> row 4*(l/16)+r,
+  // column l%16 of the 16x16 block.
+  if (op.ksplit == 1) {
+    T* __restrict__ C = op.C + b0 * op.sCb0 + b1 * op.sCb1;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int gi = tm * BM + wm * WM + i * 16 + 4 * (lane >> 4) + r;
+          int gj = tn * BN + wn * WN + j * 16 + (lane & 15);
+          if (gi < op.M && gj < op.N) {
+            T* c = C + gi * op.sCi + gj * op.sCj;
+            T v = op.alpha * acc[i][j][r];
+            if (op.beta != T(0)) v += op.beta * (*c);
+            *c = v;
+          }
+        }
+  } else {
+    T* __restrict__ W = op.work + ((int64_t)blockIdx.z * op.nbatch + b) * op.M * op.N;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int gi = tm * BM + wm * WM + i * 16 + 4 * (lane >> 4) + r;
+          int gj = tn * BN + wn * WN + j * 16 + (lane & 15);
+          if (gi < op.M && gj < op.N) W[gi + (int64_t)gj * op.M] = acc[i][j][r];
+        }
+  }
+}
+
+template <typename T>
+__global__ void gemm_splitk_reduce(GemmOp<T> op) {
+  int64_t mn = (int64_t)op.M * op.N;
+  int64_t total = mn * op.nbatch;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int b = (int)(t / mn);
+    int64_t e = t % mn;
+    int gi = (int)(e % op.M), gj = (int)(e / op.M);
+    T s = 0;
+    for (int z = 0; z < op.ksplit; ++z) s += op.work[((int64_t)z * op.nbatch + b) * mn + e];
+    int b0 = b % op.nb0, b1 = b / op.nb0;
+    T* c = op.C + b0 * op.sCb0 + b1 * op.sCb1 + gi * op.sCi + gj * op.sCj;
+    T v = op.alpha * s;
+    if (op.beta != T(0)) v += op.beta * (*c);
+    *c = v;
+  }
+}
+
+// ---------------------------------------------------------------- complex
+// Complex double GEMM: real and imaginary planes are staged separately in
+// LDS and each complex multiply-add is four real MFMAs.  conjA / conjB
+// conjugate the operand on load.
+struct ZGemmOp {
+  int M, N, K;
+  const hipDoubleComplex* A; int64_t sAi, sAk, sAb0, sAb1;
+  const hipDoubleComplex* B; int64_t sBk, sBj, sBb0, sBb1;
+  hipDoubleComplex* C;       int64_t sCi, sCj, sCb0, sCb1;
+  int nb0, nbatch;
+  int conjA, conjB;
+  hipDoubleComplex alpha, beta;
+};
+
+template <int BM, int BN>
+__global__ __launch_bounds__(256) void zgemm_mfma_kernel(ZGemmOp op) {
+  constexpr int BK = 16;
+  constexpr int PAD = 2;
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int FM = WM / 16, FN = WN / 16;
+  constexpr int LA = BM * BK / 256;
+  constexpr int LB = BN * BK / 256;
+
+  __shared__ double Ar[BK][BM + PAD], Ai[BK][BM + PAD];
+  __shared__ double Br[BK][BN + PAD], Bi[BK][BN + PAD];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int tilesM = (op.M + BM - 1) / BM, tilesN = (op.N + BN - 1) / BN;
+  const int tile = xcd_remap(blockIdx.x, tilesM * tilesN);
+  const int tm = tile / tilesN, tn = tile % tilesN;
+  const int b = blockIdx.y, b0 = b % op.nb0, b1 = b / op.nb0;
+  const hipDoubleComplex* __restrict__ A = op.A + b0 * op.sAb0 + b1 * op.sAb1;
+  const hipDoubleComplex* __restrict__ B = op.B + b0 * op.sBb0 + b1 * op.sBb1;
+  const bool a_ifast = (op.sAi == 1), b_jfast = (op.sBj == 1);
+  const double sa = op.conjA ? -1.0 : 1.0, sb = op.conjB ? -1.0 : 1.0;
+
+  double2 ra[LA], rb[LB];
+  int ai[LA], ak[LA], bk[LB], bj[LB];
+#pragma unroll
+  for (int q = 0; q < LA; ++q) {
+    int idx = tid + q * 256;
+    if (a_ifast) { ai[q] = idx % BM; ak[q] = idx / BM; } else { ak[q] = idx % BK; ai[q] = idx / BK; }
+  }
+#pragma unroll
+  for (int q = 0; q < LB; ++q) {
+    int idx = tid + q * 256;
+    if (b_jfast) { bj[q] = idx % BN; bk[q] = idx / BN; } else { bk[q] = idx % BK; bj[q] = idx / BK; }
+  }
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < LA; ++q) {
+      int gi = tm * BM + ai[q], gk = k0 + ak[q];
+      if (gi < op.M && gk < op.K) {
+        hipDoubleComplex v = A[gi * op.sAi + gk * op.sAk];
+        ra[q] = make_double2(v.x, sa * v.y);
+      } else ra[q] = make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < LB; ++q) {
+      int gj = tn * BN + bj[q], gk = k0 + bk[q];
+      if (gj < op.N && gk < op.K) {
+        hipDoubleComplex v = B[gk * op.sBk + gj * op.sBj];
+        rb[q] = make_double2(v.x, sb * v.y);
+      } else rb[q] = make_double2(0.0, 0.0);
+    }
+  };
+
+  d4_t cr[FM][FN], ci[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) { cr[i][j] = d4_t{0, 0, 0, 0}; ci[i][j] = d4_t{0, 0, 0, 0}; }
+
+  if (op.K > 0) load(0);
+  for (int k0 = 0; k0 < op.K; k0 += BK) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < LA; ++q) { Ar[ak[q]][ai[q]] = ra[q].x; Ai[ak[q]][ai[q]] = ra[q].y; }
+#pragma unroll
+    for (int q = 0; q < LB; ++q) { Br[bk[q]][bj[q]] = rb[q].x; Bi[bk[q]][bj[q]] = rb[q].y; }
+    __syncthreads();
+    if (k0 + BK < op.K) load(k0 + BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      double ar[FM], ai_[FM], nai[FM], br[FN], bi[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        int r = kk + (lane >> 4), c = wm * WM + i * 16 + (lane & 15);
+        ar[i] = Ar[r][c]; ai_[i] = Ai[r][c]; nai[i] = -ai_[i];
+      }
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        int r = kk + (lane >> 4), c = wn * WN + j * 16 + (lane & 15);
+        br[j] = Br[r][c]; bi[j] = Bi[r][c];
+      }
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          cr[i][j] = mfma16(ar[i], br[j], cr[i][j]);
+          cr[i][j] = mfma16(nai[i], bi[j], cr[i][j]);
+          ci[i][j] = mfma16(ar[i], bi[j], ci[i][j]);
+          ci[i][j] = mfma16(ai_[i], br[j], ci[i][j]);
+        }
+    }
+  }
+  hipDoubleComplex* __restrict__ C = op.C + b0 * op.sCb0 + b1 * op.sCb1;
+  const bool beta0 = (op.beta.x == 0.0 && op.beta.y == 0.0);
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int gi = tm * BM + wm * WM + i * 16 + 4 * (lane >> 4) + r;
+        int gj = tn * BN + wn * WN + j * 16 + (lane & 15);
+        if (gi < op.M && gj < op.N) {
+          hipDoubleComplex* c = C + gi * op.sCi + gj * op.sCj;
+          double xr = cr[i][j][r], xi = ci[i][j][r];
+          double vr = op.alpha.x * xr - op.alpha.y * xi;
+          double vi = op.alpha.x * xi + op.alpha.y * xr;
+          if (!beta0) {
+            hipDoubleComplex o = *c;
+            vr += op.beta.x * o.x - op.beta.y * o.y;
+            vi += op.beta.x * o.y + op.beta.y * o.x;
+          }
+          *c = make_hipDoubleComplex(vr, vi);
+        }
+      }
+}
+
+// ---------------------------------------------------------------- launchers
+// Tile choice: 64x64 when that already gives enough work-groups to cover the
+// chip twice, else 32x32; split-K when even 32x32 tiles leave most compute
+// units idle and K is long.
+struct GemmPlan {
+  int bm, bn, ksplit;
+};
+
+inline GemmPlan plan_gemm(int M, int N, int K, int nbatch, bool allow_split) {
+  const int64_t cus = 256;
+  int64_t t64 = ceil_div(M, 64) * ceil_div(N, 64) * (int64_t)nbatch;
+  if (t64 >= 2 * cus) return {64, 64, 1};
+  int64_t t32 = ceil_div(M, 32) * ceil_div(N, 32) * (int64_t)nbatch;
+  GemmPlan p{32, 32, 1};
+  if (allow_split && t32 < cus && K >= 512) {
+    int ks = (int)std::min<int64_t>(ceil_div(2 * cus, t32), ceil_div(K, 256));
+    p.ksplit = std::max(1, std::min(ks, 64));
+  }
+  return p;
+}
+
+template <typename T>
+void gemm_launch(GemmOp<T> op, hipStream_t st, const GemmPlan* forced = nullptr) {
+  if (op.M <= 0 || op.N <= 0 || op.nbatch <= 0) return;
+  if (op.nb0 <= 0) op.nb0 = 1;
+  GemmPlan p = forced ? *forced : plan_gemm(op.M, op.N, op.K, op.nbatch, op.work != nullptr);
+  op.ksplit = p.ksplit;
+  if (op.ksplit > 1 && op.work == nullptr) op.ksplit = 1;
+  TTF_REQUIRE(op.nbatch <= 65535, "gemm: batch count %d exceeds the grid limit", op.nbatch);
+  int64_t tiles = ceil_div(op.M, p.bm) * ceil_div(op.N, p.bn);
+  dim3 grid((unsigned)tiles, (unsigned)op.nbatch, (unsigned)op.ksplit);
+  if (p.bm == 64)
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, 64, 64>), grid, dim3(256), 0, st, op);
+  else
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, 32, 32>), grid, dim3(256), 0, st, op);
+  TTF_HIP(hipGetLastError());
+  if (op.ksplit > 1) {
+    int64_t total = (int64_t)op.M * op.N * op.nbatch;
+    int blocks = (int)std::min<int64_t>(ceil_div(total, 256), 4096);
+    hipLaunchKernelGGL((gemm_splitk_reduce<T>), dim3(blocks), dim3(256), 0, st, op);
+    TTF_HIP(hipGetLastError());
+  }
+}
+
+inline void zgemm_launch(ZGemmOp op, hipStream_t st) {
+  if (op.M <= 0 || op.N <= 0 || op.nbatch <= 0) return;
+  if (op.nb0 <= 0) op.nb0 = 1;
+  TTF_REQUIRE(op.nbatch <= 65535, "zgemm: batch count %d exceeds the grid limit", op.nbatch);
+  GemmPlan p = plan_gemm(op.M, op.N, op.K, op.nbatch, false);
+  int64_t tiles = ceil_div(op.M, p.bm) * ceil_div(op.N, p.bn);
+  dim3 grid((unsigned)tiles, (unsigned)op.nbatch, 1);
+  if (p.bm == 64)
+    hipLaunchKernelGGL((zgemm_mfma_kernel<64, 64>), grid, dim3(256), 0, st, op);
+  else
+    hipLaunchKernelGGL((zgemm_mfma_kernel<32, 32>), grid, dim3(256), 0, st, op);
+  TTF_HIP(hipGetLastError());
+}
+
+// Convenience constructor for a column-major GEMM with optional transposes:
+// C (M x N, leading dim ldc) = alpha * op(A) op(B) + beta * C.
+template <typename T>
+GemmOp<T> colmajor(char ta, char tb, int M, int N, int K, T alpha, const T* A, int64_t lda,
+                   const T* B, int64_t ldb, T beta, T* C, int64_t ldc) {
+  GemmOp<T> op{};
+  op.M = M; op.N = N; op.K = K;
+  op.A = A; op.B = B; op.C = C;
+  if (ta == 'N' || ta == 'n') { op.sAi = 1; op.sAk = lda; } else { op.sAi = lda; op.sAk = 1; }
+  if (tb == 'N' || tb == 'n') { op.sBk = 1; op.sBj = ldb; } else { op.sBk = ldb; op.sBj = 1; }
+  op.sCi = 1; op.sCj = ldc;
+  op.nb0 = 1; op.nbatch = 1;
+  op.alpha = alpha; op.beta = beta;
+  op.ksplit = 1; op.work = nullptr;
+  return op;
+}
+
+}  // namespace ttf

@@ -1,0 +1,176 @@
+// Library runtime: error state, device/stream set-up, scratch arenas,
+// memory helpers and stream timers.
+#include "ttf_common.h"
+
+#include <cstdarg>
+#include <cstring>
+#include <mutex>
+
+namespace ttf {
+
+namespace {
+thread_local char g_err[1024] = {0};
+std::mutex g_mu;
+bool g_init = false;
+int g_device = 0;
+hipStream_t g_stream = nullptr;
+hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
+struct ArenaSlot { void* p = nullptr; size_t bytes = 0; };
+ArenaSlot g_arena[kArenaCount];
+}  // namespace
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+bool has_error() { return g_err[0] != 0; }
+
+static void init_locked(int device) {
+  if (g_init) return;
+  int count = 0;
+  TTF_HIP(hipGetDeviceCount(&count));
+  TTF_REQUIRE(count > 0, "no HIP device available");
+  TTF_REQUIRE(device >= 0 && device < count, "device %d out of range (%d devices)", device, count);
+  TTF_HIP(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  TTF_HIP(hipGetDeviceProperties(&prop, device));
+  TTF_REQUIRE(std::strncmp(prop.gcnArchName, "gfx950", 6) == 0,
+              "this library is built for gfx950 (MI355X); device %d is %s", device,
+              prop.gcnArchName);
+  TTF_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  TTF_HIP(hipEventCreate(&g_ev0));
+  TTF_HIP(hipEventCreate(&g_ev1));
+  g_device = device;
+  g_init = true;
+}
+
+void ensure_init() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  init_locked(0);
+}
+
+hipStream_t stream() {
+  ensure_init();
+  return g_stream;
+}
+
+void* arena(Arena a, size_t bytes) {
+  ensure_init();
+  ArenaSlot& s = g_arena[a];
+  if (bytes == 0) bytes = 256;
+  if (s.bytes < bytes) {
+    if (s.p) {
+      TTF_HIP(hipStreamSynchronize(g_stream));
+      TTF_HIP(hipFree(s.p));
+      s.p = nullptr;
+      s.bytes = 0;
+    }
+    size_t nb = bytes + bytes / 4;   // grow with headroom
+    nb = (nb + 4095) & ~size_t(4095);
+    TTF_HIP(hipMalloc(&s.p, nb));
+    s.bytes = nb;
+  }
+  return s.p;
+}
+
+}  // namespace ttf
+
+using namespace ttf;
+
+extern "C" {
+
+int ttf_init(int device) {
+  return guarded([&] {
+    std::lock_guard<std::mutex> lk(g_mu);
+    init_locked(device);
+  });
+}
+
+int ttf_finalize(void) {
+  return guarded([&] {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_init) return;
+    TTF_HIP(hipStreamSynchronize(g_stream));
+    for (auto& s : g_arena) {
+      if (s.p) TTF_HIP(hipFree(s.p));
+      s.p = nullptr;
+      s.bytes = 0;
+    }
+    TTF_HIP(hipEventDestroy(g_ev0));
+    TTF_HIP(hipEventDestroy(g_ev1));
+    TTF_HIP(hipStreamDestroy(g_stream));
+    g_stream = nullptr;
+    g_init = false;
+  });
+}
+
+const char* ttf_last_error(void) { return g_err; }
+void ttf_clear_error(void) { g_err[0] = 0; }
+const char* ttf_version(void) { return "ttfort_amd 0.1 (gfx950)"; }
+
+int ttf_malloc(void** ptr, size_t bytes) {
+  return guarded([&] {
+    ensure_init();
+    TTF_REQUIRE(ptr != nullptr, "ttf_malloc: null output pointer");
+    TTF_HIP(hipMalloc(ptr, bytes ? bytes : 16));
+  });
+}
+
+int ttf_free(void* ptr) {
+  return guarded([&] {
+    if (!ptr) return;
+    TTF_HIP(hipStreamSynchronize(stream()));
+    TTF_HIP(hipFree(ptr));
+  });
+}
+
+int ttf_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  return guarded([&] {
+    TTF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream()));
+    TTF_HIP(hipStreamSynchronize(stream()));
+  });
+}
+
+int ttf_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  return guarded([&] {
+    TTF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream()));
+    TTF_HIP(hipStreamSynchronize(stream()));
+  });
+}
+
+int ttf_memcpy_d2d(void* dst, const void* src, size_t bytes) {
+  return guarded([&] {
+    TTF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream()));
+  });
+}
+
+int ttf_memset(void* dst, int value, size_t bytes) {
+  return guarded([&] { TTF_HIP(hipMemsetAsync(dst, value, bytes, stream())); });
+}
+
+int ttf_sync(void) {
+  return guarded([&] { TTF_HIP(hipStreamSynchronize(stream())); });
+}
+
+int ttf_timer_start(void) {
+  return guarded([&] {
+    hipStream_t s = stream();
+    TTF_HIP(hipEventRecord(g_ev0, s));
+  });
+}
+
+int ttf_timer_stop(float* ms) {
+  return guarded([&] {
+    hipStream_t s = stream();
+    TTF_HIP(hipEventRecord(g_ev1, s));
+    TTF_HIP(hipEventSynchronize(g_ev1));
+    float t = 0.f;
+    TTF_HIP(hipEventElapsedTime(&t, g_ev0, g_ev1));
+    if (ms) *ms = t;
+  });
+}
+
+}  // extern "C"

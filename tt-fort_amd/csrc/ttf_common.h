@@ -1,0 +1,79 @@
+// Common definitions for the tt-fort MI355X library (gfx950, HIP).
+//
+// Error model: every C-ABI entry point returns an int status (0 = success)
+// and records a human-readable message retrievable with ttf_last_error().
+// Entry points that mirror the reference's void Fortran subroutines cannot
+// return a status; they record a sticky error that ttf_last_error() reports
+// and ttf_clear_error() resets.  Nothing in the library calls exit().
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_complex.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "ttfort_amd.h"
+
+namespace ttf {
+
+void set_error(const char* fmt, ...);
+bool has_error();
+
+struct HipFail {};
+
+#define TTF_HIP(call)                                                          \
+  do {                                                                         \
+    hipError_t _e = (call);                                                    \
+    if (_e != hipSuccess) {                                                    \
+      ::ttf::set_error("%s:%d: %s failed: %s", __FILE__, __LINE__, #call,      \
+                       hipGetErrorString(_e));                                 \
+      throw ::ttf::HipFail{};                                                  \
+    }                                                                          \
+  } while (0)
+
+#define TTF_REQUIRE(cond, ...)                                                 \
+  do {                                                                         \
+    if (!(cond)) {                                                             \
+      ::ttf::set_error(__VA_ARGS__);                                           \
+      throw ::ttf::HipFail{};                                                  \
+    }                                                                          \
+  } while (0)
+
+// Runs `body` and converts library failures into an int status.
+template <class Fn>
+int guarded(Fn&& body) {
+  try {
+    body();
+    return 0;
+  } catch (const HipFail&) {
+    return -1;
+  } catch (const std::exception& e) {
+    set_error("exception: %s", e.what());
+    return -2;
+  }
+}
+
+// Library-wide device state: the stream every kernel is launched on and a
+// set of growing scratch arenas (device memory beyond the caller's inputs and
+// outputs: contraction intermediates, solver bases).  Arenas grow with
+// hipMalloc and are released by ttf_finalize().
+hipStream_t stream();
+void ensure_init();
+
+enum Arena {
+  kArenaBfun = 0,    // contraction intermediates (res1 / res2)
+  kArenaHostIO,      // staging for the host-pointer drop-in entry points
+  kArenaHostIO2,
+  kArenaSolver,      // local solver bases (LOBPCG / GMRES / Krylov)
+  kArenaSmall,       // small dense matrices and scalars
+  kArenaTrunc,       // SVD / QR workspaces
+  kArenaCount
+};
+void* arena(Arena a, size_t bytes);
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace ttf

@@ -1,0 +1,261 @@
+"""ctypes binding of the C ABI in include/ttfort_amd.h.
+
+This is the Python face of the library used by the tests and the benchmark:
+it loads lib/libttfort_amd.so (built by build.py) and raises if the library
+is missing -- there is no CPU fallback.  Device arrays are `DeviceArray`
+objects wrapping memory from ttf_malloc; helpers upload/download numpy
+arrays in column-major (Fortran) order, the layout of the reference.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(HERE, "lib")
+
+_c_int = C.c_int
+_c_i64 = C.c_int64
+_vp = C.c_void_p
+
+
+class TTFError(RuntimeError):
+    pass
+
+
+def lib_path(kind: str = "i4") -> str:
+    name = "libttfort_amd.so" if kind == "i4" else "libttfort_amd_i8.so"
+    return os.path.join(LIBDIR, name)
+
+
+_LIB = {}
+
+
+def load(kind: str = "i4"):
+    """Load (once) and return the ctypes CDLL of the HIP library."""
+    if kind in _LIB:
+        return _LIB[kind]
+    path = lib_path(kind)
+    if not os.path.exists(path):
+        raise TTFError("HIP library not built: %s (run python tt-fort_amd/build.py)" % path)
+    lib = C.CDLL(path)
+    _declare(lib)
+    _LIB[kind] = lib
+    return lib
+
+
+# (name, restype, argtypes)
+_SIGS = []
+
+
+def _sig(name, res, *args):
+    _SIGS.append((name, res, list(args)))
+
+
+_S8 = [_c_int] * 8
+_sig("ttf_init", _c_int, _c_int)
+_sig("ttf_finalize", _c_int)
+_sig("ttf_last_error", C.c_char_p)
+_sig("ttf_clear_error", None)
+_sig("ttf_version", C.c_char_p)
+_sig("ttf_malloc", _c_int, C.POINTER(_vp), C.c_size_t)
+_sig("ttf_free", _c_int, _vp)
+_sig("ttf_memcpy_h2d", _c_int, _vp, _vp, C.c_size_t)
+_sig("ttf_memcpy_d2h", _c_int, _vp, _vp, C.c_size_t)
+_sig("ttf_memcpy_d2d", _c_int, _vp, _vp, C.c_size_t)
+_sig("ttf_memset", _c_int, _vp, _c_int, C.c_size_t)
+_sig("ttf_sync", _c_int)
+_sig("ttf_timer_start", _c_int)
+_sig("ttf_timer_stop", _c_int, C.POINTER(C.c_float))
+for _nm in ("ttf_dbfun3", "ttf_sbfun3", "ttf_zbfun3", "ttf_zbfun3_transp"):
+    _sig(_nm, _c_int, *(_S8 + [_c_int, _vp, _vp, _vp, _vp, _c_i64, _vp, _c_i64]))
+_sig("ttf_dbfun3_profile", _c_int, *(_S8 + [_c_int, _vp, _vp, _vp, _vp, _c_i64, _vp, _c_i64, _c_int,
+                                           C.POINTER(C.c_float)]))
+for _nm in ("ttf_dphi_left", "ttf_dphi_right", "ttf_zphi_left", "ttf_zphi_right"):
+    _sig(_nm, _c_int, *(_S8 + [_vp, _vp, _vp, _vp, _vp]))
+for _nm in ("ttf_dphi2_left", "ttf_dphi2_right"):
+    _sig(_nm, _c_int, *([_c_int] * 5 + [_vp, _vp, _vp, _vp]))
+_sig("ttf_d2d_fullmat", _c_int, *([_c_int] * 5 + [_vp, _vp, _vp, _vp]))
+_sig("dbfun3", None, *([_vp] * 15))
+
+
+def _declare(lib):
+    for name, res, args in _SIGS:
+        f = getattr(lib, name, None)
+        if f is None:
+            continue
+        f.restype = res
+        f.argtypes = args
+
+
+def exported_symbols():
+    """Names declared in this binding (tests compare them with the header)."""
+    return [s[0] for s in _SIGS]
+
+
+def check(status: int, lib=None):
+    if status != 0:
+        lib = lib or load()
+        msg = lib.ttf_last_error().decode(errors="replace")
+        lib.ttf_clear_error()
+        raise TTFError(msg or "ttf call failed with status %d" % status)
+
+
+class DeviceArray:
+    """A device buffer holding a column-major array."""
+
+    def __init__(self, shape, dtype=np.float64, lib=None):
+        self.lib = lib or load()
+        self.shape = tuple(int(s) for s in np.atleast_1d(shape))
+        self.dtype = np.dtype(dtype)
+        self.size = int(np.prod(self.shape))
+        self.nbytes = self.size * self.dtype.itemsize
+        p = _vp()
+        check(self.lib.ttf_malloc(C.byref(p), max(self.nbytes, 16)), self.lib)
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, a, dtype=None, lib=None):
+        a = np.asarray(a, dtype=dtype) if dtype is not None else np.asarray(a)
+        d = cls(a.shape if a.ndim else (1,), a.dtype, lib)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.asfortranarray(np.asarray(a, dtype=self.dtype))
+        if a.size != self.size:
+            raise ValueError("size mismatch %d != %d" % (a.size, self.size))
+        check(self.lib.ttf_memcpy_h2d(self.ptr, a.ctypes.data, self.nbytes), self.lib)
+
+    def numpy(self):
+        out = np.empty(self.shape, dtype=self.dtype, order="F")
+        check(self.lib.ttf_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes), self.lib)
+        return out
+
+    def free(self):
+        if self.ptr:
+            check(self.lib.ttf_free(self.ptr), self.lib)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "ptr", None):
+                self.lib.ttf_free(self.ptr)
+        except Exception:
+            pass
+
+
+def sync(lib=None):
+    lib = lib or load()
+    check(lib.ttf_sync(), lib)
+
+
+class Timer:
+    """Stream-event timer around work on the library stream."""
+
+    def __init__(self, lib=None):
+        self.lib = lib or load()
+
+    def __enter__(self):
+        check(self.lib.ttf_timer_start(), self.lib)
+        return self
+
+    def __exit__(self, *exc):
+        ms = C.c_float()
+        check(self.lib.ttf_timer_stop(C.byref(ms)), self.lib)
+        self.ms = float(ms.value)
+        return False
+
+
+# ---------------------------------------------------------------- helpers
+def _dev(a, dtype):
+    return DeviceArray.from_numpy(np.ravel(np.asarray(a), order="F"), dtype=dtype)
+
+
+def bfun3(shape, phi1, A, phi2, x, nvec=1, dtype=np.float64, transp=False):
+    """Apply the local operator to nvec column-major vectors (host in/out).
+
+    shape = (rx1, m, rx2, ry1, n, ry2, ra1, ra2).  x has nvec columns of
+    rx1*m*rx2 entries (or ry1*n*ry2 when transp=True, complex only)."""
+    lib = load()
+    rx1, m, rx2, ry1, n, ry2, ra1, ra2 = shape
+    nx, ny = rx1 * m * rx2, ry1 * n * ry2
+    if transp:
+        nx, ny = ny, nx
+    dt = np.dtype(dtype)
+    dphi1, dA, dphi2 = _dev(phi1, dt), _dev(A, dt), _dev(phi2, dt)
+    dx = _dev(x, dt)
+    dy = DeviceArray((ny * nvec,), dt)
+    fn = {np.dtype(np.float64): lib.ttf_dbfun3, np.dtype(np.float32): lib.ttf_sbfun3,
+          np.dtype(np.complex128): lib.ttf_zbfun3}[dt]
+    if transp:
+        fn = lib.ttf_zbfun3_transp
+    check(fn(*shape, nvec, dphi1.ptr, dA.ptr, dphi2.ptr, dx.ptr, nx, dy.ptr, ny), lib)
+    return dy.numpy()
+
+
+def phi_left(shape, phi1, A, x, y, complex_=False):
+    lib = load()
+    rx1, m, rx2, ry1, n, ry2, ra1, ra2 = shape
+    dt = np.complex128 if complex_ else np.float64
+    d = [_dev(a, dt) for a in (phi1, A, x, y)]
+    out = DeviceArray((ry2 * rx2 * ra2,), dt)
+    fn = lib.ttf_zphi_left if complex_ else lib.ttf_dphi_left
+    check(fn(*shape, d[0].ptr, d[1].ptr, d[2].ptr, d[3].ptr, out.ptr), lib)
+    return out.numpy()
+
+
+def phi_right(shape, phi2, A, x, y, complex_=False):
+    lib = load()
+    rx1, m, rx2, ry1, n, ry2, ra1, ra2 = shape
+    dt = np.complex128 if complex_ else np.float64
+    d = [_dev(a, dt) for a in (phi2, A, x, y)]
+    out = DeviceArray((rx1 * ra1 * ry1,), dt)
+    fn = lib.ttf_zphi_right if complex_ else lib.ttf_dphi_right
+    check(fn(*shape, d[0].ptr, d[1].ptr, d[2].ptr, d[3].ptr, out.ptr), lib)
+    return out.numpy()
+
+
+def phi2_left(rx1, rx2, ry1, n, ry2, phi1, x, y):
+    lib = load()
+    d = [_dev(a, np.float64) for a in (phi1, x, y)]
+    out = DeviceArray((ry2 * rx2,), np.float64)
+    check(lib.ttf_dphi2_left(rx1, rx2, ry1, n, ry2, d[0].ptr, d[1].ptr, d[2].ptr, out.ptr), lib)
+    return out.numpy()
+
+
+def phi2_right(rx1, rx2, ry1, n, ry2, phi2, x, y):
+    lib = load()
+    d = [_dev(a, np.float64) for a in (phi2, x, y)]
+    out = DeviceArray((rx1 * ry1,), np.float64)
+    check(lib.ttf_dphi2_right(rx1, rx2, ry1, n, ry2, d[0].ptr, d[1].ptr, d[2].ptr, out.ptr), lib)
+    return out.numpy()
+
+
+def fullmat(rx1, m, rx2, ra1, ra2, phi1, A, phi2):
+    lib = load()
+    d = [_dev(a, np.float64) for a in (phi1, A, phi2)]
+    N = rx1 * m * rx2
+    out = DeviceArray((N, N), np.float64)
+    check(lib.ttf_d2d_fullmat(rx1, m, rx2, ra1, ra2, d[0].ptr, d[1].ptr, d[2].ptr, out.ptr), lib)
+    return out.numpy()
+
+
+def ref_dbfun3(shape, phi1, A, phi2, x, kind="i4"):
+    """Call the reference-compatible `dbfun3` symbol with host arrays."""
+    lib = load(kind)
+    it = C.c_int32 if kind == "i4" else C.c_int64
+    dims = [it(int(v)) for v in shape]
+    rx1, m, rx2, ry1, n, ry2, ra1, ra2 = shape
+    y = np.zeros(ry1 * n * ry2)
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (phi1, A, phi2, x)]
+    ws = np.zeros(1)
+    lib.dbfun3(*[C.cast(C.byref(v), _vp) for v in dims], *[a.ctypes.data for a in arrs],
+               y.ctypes.data, ws.ctypes.data, ws.ctypes.data)
+    msg = lib.ttf_last_error().decode()
+    if msg:
+        lib.ttf_clear_error()
+        raise TTFError(msg)
+    return y
