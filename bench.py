@@ -41,34 +41,52 @@ def bfun3_stage_flops(rx1, m, rx2, ry1, n, ry2, ra1, ra2):
             2.0 * ry1 * n * ry2 * rx1 * ra1)
 
 
+def _blas_threads_options():
+    n = os.cpu_count() or 1
+    lim = int(os.environ.get("OMP_NUM_THREADS", n) or n)
+    opts = [t for t in (1, 2, 4, 8, 16, 32) if t <= max(1, min(n, lim))]
+    return opts or [1]
+
+
 def cpu_baseline_matvec(shape, nvec, budget_s=12.0):
-    """Time the oracle's bfun3 (the reference's GEMM chain on host BLAS)."""
+    """Time the oracle's bfun3 (the reference's GEMM chain on host BLAS).
+
+    The host BLAS thread count is calibrated first (about 0.5 s per option)
+    and the sample then runs with the fastest count, which is reported."""
     from oracle import kernels as K
-    try:
-        from threadpoolctl import threadpool_info
-        cores = max([i.get("num_threads", 1) for i in threadpool_info()
-                     if i.get("user_api") == "blas"] or [1])
-    except Exception:
-        cores = os.cpu_count() or 1
+    from threadpoolctl import threadpool_limits
     rng = np.random.default_rng(1)
     rx1, m, rx2, ry1, n, ry2, ra1, ra2 = shape
     phi1 = rng.standard_normal(ry1 * rx1 * ra1)
     A = rng.standard_normal(ra1 * n * m * ra2)
     phi2 = rng.standard_normal(rx2 * ra2 * ry2)
     X = rng.standard_normal((rx1 * m * rx2, nvec))
-    K.dbfun3(*shape, phi1, A, phi2, X[:, 0])   # warm
-    reps, t0 = 0, time.perf_counter()
-    while True:
-        for v in range(nvec):
-            K.dbfun3(*shape, phi1, A, phi2, X[:, v])
-        reps += 1
-        el = time.perf_counter() - t0
-        if el > budget_s:
-            break
+
+    def rate(threads, budget):
+        with threadpool_limits(limits=threads, user_api="blas"):
+            K.dbfun3(*shape, phi1, A, phi2, X[:, 0])   # warm
+            reps, t0 = 0, time.perf_counter()
+            while True:
+                for v in range(nvec):
+                    K.dbfun3(*shape, phi1, A, phi2, X[:, v])
+                reps += 1
+                el = time.perf_counter() - t0
+                if el > budget:
+                    break
+        return reps, el
+
+    best = None
+    for t in _blas_threads_options():
+        reps, el = rate(t, 0.5)
+        if best is None or reps / el > best[1]:
+            best = (t, reps / el)
+    cores = best[0]
+    reps, el = rate(cores, budget_s)
     flops = sum(bfun3_stage_flops(*shape)) * nvec * reps
     return dict(value=flops / el / 1e12, unit="TFLOP/s", cores=int(cores), kind="port",
                 sample="%d block matvecs (%d vectors each) of the oracle's dbfun3 GEMM chain "
-                       "in %.1f s" % (reps, nvec, el))
+                       "in %.1f s, host BLAS on %d threads (fastest of %s)"
+                       % (reps, nvec, el, cores, _blas_threads_options()))
 
 
 def run_matvec(args):
@@ -110,14 +128,16 @@ def run_matvec(args):
     ttfort.check(lib.ttf_dbfun3_profile(*shape, nvec, d["phi1"].ptr, d["A"].ptr, d["phi2"].ptr,
                                         d["x"].ptr, N, d["y"].ptr, N, max(args.steps, 20), st), lib)
     stage_ms = [float(v) for v in st]
-    # stages 1 and 3 run the same MFMA GEMM kernel and dominate the flops
-    dom_flops = (fl[0] + fl[2]) * nvec / 2.0
-    dom_ms = (stage_ms[0] + stage_ms[2]) / 2.0
-    achieved = dom_flops / (dom_ms * 1e-3) / 1e12
+    # the dominant kernel is the MFMA GEMM launch of the slower of the two
+    # large stages (1: x*phi2, 3: phi1*T2); stage 2 is a small panel product
+    dom = 0 if stage_ms[0] >= stage_ms[2] else 2
+    achieved = fl[dom] * nvec / (stage_ms[dom] * 1e-3) / 1e12
     roofline = dict(bound="mfma", achieved=round(achieved, 3), peak=MI355X_PEAK_F64_TFLOPS,
                     unit="TFLOP/s", frac=round(achieved / MI355X_PEAK_F64_TFLOPS, 4), traffic=None,
-                    kernel="gemm_mfma_kernel<double,32,32> (contraction stages 1 and 3)",
-                    stage_ms=[round(v, 5) for v in stage_ms])
+                    kernel="gemm_mfma_kernel<double,...> of contraction stage %d" % (dom + 1),
+                    stage_ms=[round(v, 5) for v in stage_ms],
+                    stage_tflops=[round(fl[k] * nvec / (stage_ms[k] * 1e-3) / 1e12, 3)
+                                  for k in range(3)])
     out = dict(metric="local-matvec TFLOP/s (FP64, Heisenberg d=64 r=128 local problem)",
                value=round(value, 4), unit="TFLOP/s", n_gpus=1, steps=args.steps,
                warmup=args.warmup, ms_per_step=round(ms_step, 5), higher_is_better=True,

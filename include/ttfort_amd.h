@@ -73,6 +73,12 @@ int ttf_sync(void);
  * elapsed milliseconds. */
 int ttf_timer_start(void);
 int ttf_timer_stop(float* ms);
+/* GEMM tile autotuning (on by default; env TTF_AUTOTUNE=0 disables).  The
+ * first product of each new shape times the tile / split-K candidates on the
+ * library stream and caches the fastest; disabling clears the cache and uses
+ * a fixed heuristic.  ttf_autotune_count returns how many shapes were tuned. */
+int ttf_set_autotune(int enable);
+long ttf_autotune_count(void);
 
 /* ------------------------------------------- local operator (bfun3 family) */
 /* y(c1,i,c2) = sum phi1(c1,b1,a1) A(a1,i,j,a2) phi2(b2,a2,c2) x(b1,j,b2)

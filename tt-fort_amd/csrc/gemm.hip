@@ -1,0 +1,276 @@
+// GEMM launchers: runtime autotuning of tile shape and split-K, and the
+// small-panel kernel for products with a short K and a narrow shared B.
+#include "gemm_mfma.h"
+
+#include <chrono>
+#include <cstdlib>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+namespace ttf {
+
+// ---------------------------------------------------------------- small panel
+// C[b](i,j) = alpha sum_k A[b](i,k) B(k,j) + beta C[b](i,j) with K <= 64,
+// N <= 32, B shared by all batches and A, C row-contiguous (sAi = sCi = 1).
+// This is the shape of the middle bfun3 / phi stage for small mode sizes
+// (K = m*ra2, N = ra1*n): memory-bound, so one thread per row i of one batch
+// accumulates all N outputs while B sits in LDS.
+// NN is N rounded up to the kernel's column bucket; B is zero-padded in LDS
+// to (64 x NN) so the inner loops carry no guards.
+template <typename T, int NN>
+__global__ __launch_bounds__(256) void panel_kernel(GemmOp<T> op) {
+  __shared__ T Bs[64][NN];
+  const int K = op.K, N = op.N;
+  for (int t = threadIdx.x; t < 64 * NN; t += blockDim.x) {
+    int k = t / NN, j = t % NN;
+    Bs[k][j] = (k < K && j < N) ? op.B[k * op.sBk + j * op.sBj] : T(0);
+  }
+  __syncthreads();
+  const int64_t total = (int64_t)op.M * op.nbatch;
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(g % op.M);
+    const int b = (int)(g / op.M);
+    const int b0 = b % op.nb0, b1 = b / op.nb0;
+    const T* a = op.A + b0 * op.sAb0 + b1 * op.sAb1 + i;
+    T* c = op.C + b0 * op.sCb0 + b1 * op.sCb1 + i;
+    T acc[NN];
+#pragma unroll
+    for (int j = 0; j < NN; ++j) acc[j] = T(0);
+    // the kernel is bound by memory latency: each chunk issues its 8 loads
+    // together before the arithmetic that consumes them
+    for (int k0 = 0; k0 < K; k0 += 8) {
+      T av[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) av[u] = (k0 + u < K) ? a[(k0 + u) * op.sAk] : T(0);
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int j = 0; j < NN; ++j) acc[j] += av[u] * Bs[k0 + u][j];
+    }
+#pragma unroll
+    for (int j = 0; j < NN; ++j)
+      if (j < N) {
+        T* cj = c + j * op.sCj;
+        T v = op.alpha * acc[j];
+        if (op.beta != T(0)) v += op.beta * (*cj);
+        *cj = v;
+      }
+  }
+}
+
+template <typename T>
+static bool panel_eligible(const GemmOp<T>& op) {
+  return op.K <= 64 && op.N <= 32 && op.sAi == 1 && op.sCi == 1 && op.sBb0 == 0 &&
+         op.sBb1 == 0 && (int64_t)op.M * op.nbatch >= 256;
+}
+
+// ---------------------------------------------------------------- autotuner
+struct Cfg {
+  int bm, bk, ks;   // square tiles bm x bm, K chunk bk, split-K factor ks
+};
+
+struct Key {
+  int type, M, N, K, nbatch, flags;
+  bool operator==(const Key& o) const {
+    return type == o.type && M == o.M && N == o.N && K == o.K && nbatch == o.nbatch &&
+           flags == o.flags;
+  }
+};
+struct KeyHash {
+  size_t operator()(const Key& k) const {
+    size_t h = 1469598103934665603ull;
+    for (int v : {k.type, k.M, k.N, k.K, k.nbatch, k.flags}) h = (h ^ (size_t)v) * 1099511628211ull;
+    return h;
+  }
+};
+
+namespace {
+std::mutex g_tune_mu;
+std::unordered_map<Key, Cfg, KeyHash> g_tuned;
+int g_autotune = -1;   // -1: read TTF_AUTOTUNE on first use
+long g_tune_count = 0;
+}  // namespace
+
+static bool autotune_enabled() {
+  if (g_autotune < 0) {
+    const char* e = std::getenv("TTF_AUTOTUNE");
+    g_autotune = (e && e[0] == '0') ? 0 : 1;
+  }
+  return g_autotune == 1;
+}
+
+static Cfg heuristic(int M, int N, int K, int nbatch) {
+  const int64_t cus = 256;
+  int64_t t64 = ceil_div(M, 64) * ceil_div(N, 64) * (int64_t)nbatch;
+  if (t64 >= cus / 2) return {64, 16, 1};
+  int64_t t32 = ceil_div(M, 32) * ceil_div(N, 32) * (int64_t)nbatch;
+  Cfg c{32, 16, 1};
+  if (t32 < cus && K >= 256) {
+    int ks = (int)std::min<int64_t>(ceil_div(2 * cus, t32), K / 128);
+    c.ks = std::max(1, std::min(ks, 16));
+  }
+  return c;
+}
+
+template <typename T>
+static void launch_cfg(GemmOp<T> op, const Cfg& c, hipStream_t st) {
+  op.ksplit = c.ks;
+  int64_t tiles = ceil_div(op.M, c.bm) * ceil_div(op.N, c.bm);
+  dim3 grid((unsigned)tiles, (unsigned)op.nbatch, (unsigned)op.ksplit);
+  if (c.bm == 64 && c.bk == 16)
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, 64, 64, 16>), grid, dim3(256), 0, st, op);
+  else if (c.bm == 64 && c.bk == 32)
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, 64, 64, 32>), grid, dim3(256), 0, st, op);
+  else if (c.bm == 64)
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, 64, 64, 64>), grid, dim3(256), 0, st, op);
+  else if (c.bk == 16)
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, 32, 32, 16, 1>), grid, dim3(64), 0, st, op);
+  else if (c.bk == 32)
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, 32, 32, 32, 1>), grid, dim3(64), 0, st, op);
+  else
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, 32, 32, 64, 1>), grid, dim3(64), 0, st, op);
+  TTF_HIP(hipGetLastError());
+  if (op.ksplit > 1) {
+    int64_t total = (int64_t)op.M * op.N * op.nbatch;
+    int blocks = (int)std::min<int64_t>(ceil_div(total, 256), 4096);
+    hipLaunchKernelGGL((gemm_splitk_reduce<T>), dim3(blocks), dim3(256), 0, st, op);
+    TTF_HIP(hipGetLastError());
+  }
+}
+
+template <typename T>
+static T* splitk_work(const GemmOp<T>& op, int ks) {
+  size_t need = sizeof(T) * (size_t)ks * op.nbatch * (size_t)op.M * op.N;
+  return (T*)arena(kArenaSplitK, need);
+}
+
+// Times each candidate configuration twice with events and keeps the
+// fastest.  Only products that overwrite C (beta == 0) are tuned, since the
+// candidates run the product several times.
+template <typename T>
+static Cfg tune(const GemmOp<T>& op0, hipStream_t st) {
+  std::vector<Cfg> cands;
+  for (int bm : {64, 32})
+    for (int bk : {16, 32, 64})
+      for (int ks : {1, 2, 4, 8}) {
+        if (ks > 1 && op0.K / ks < 64) continue;
+        int64_t tiles = ceil_div(op0.M, bm) * ceil_div(op0.N, bm) * (int64_t)op0.nbatch;
+        if (ks > 1 && tiles * ks > 4096) continue;
+        cands.push_back({bm, bk, ks});
+      }
+  hipEvent_t e0, e1;
+  TTF_HIP(hipEventCreate(&e0));
+  TTF_HIP(hipEventCreate(&e1));
+  Cfg best = heuristic(op0.M, op0.N, op0.K, op0.nbatch);
+  float best_ms = 1e30f;
+  for (const Cfg& c : cands) {
+    GemmOp<T> op = op0;
+    if (c.ks > 1) op.work = splitk_work(op, c.ks);
+    launch_cfg(op, c, st);   // warm-up (code object load, caches)
+    TTF_HIP(hipEventRecord(e0, st));
+    launch_cfg(op, c, st);
+    launch_cfg(op, c, st);
+    TTF_HIP(hipEventRecord(e1, st));
+    TTF_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    TTF_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best_ms) { best_ms = ms; best = c; }
+  }
+  TTF_HIP(hipEventDestroy(e0));
+  TTF_HIP(hipEventDestroy(e1));
+  ++g_tune_count;
+  return best;
+}
+
+template <typename T> static int type_id();
+template <> int type_id<double>() { return 0; }
+template <> int type_id<float>() { return 1; }
+
+template <typename T>
+void gemm_launch(GemmOp<T> op, hipStream_t st) {
+  if (op.M <= 0 || op.N <= 0 || op.nbatch <= 0) return;
+  if (op.nb0 <= 0) op.nb0 = 1;
+  TTF_REQUIRE(op.nbatch <= 65535, "gemm: batch count %d exceeds the grid limit", op.nbatch);
+  if (op.K <= 0) {
+    // empty sum: C = beta*C (alpha*0)
+    op.K = 0;
+  }
+  if (panel_eligible(op)) {
+    int64_t total = (int64_t)op.M * op.nbatch;
+    int blocks = (int)std::min<int64_t>(ceil_div(total, 256), 65535);
+    if (op.N <= 4)
+      hipLaunchKernelGGL((panel_kernel<T, 4>), dim3(blocks), dim3(256), 0, st, op);
+    else if (op.N <= 8)
+      hipLaunchKernelGGL((panel_kernel<T, 8>), dim3(blocks), dim3(256), 0, st, op);
+    else if (op.N <= 16)
+      hipLaunchKernelGGL((panel_kernel<T, 16>), dim3(blocks), dim3(256), 0, st, op);
+    else
+      hipLaunchKernelGGL((panel_kernel<T, 32>), dim3(blocks), dim3(256), 0, st, op);
+    TTF_HIP(hipGetLastError());
+    return;
+  }
+  const double flops = 2.0 * op.M * op.N * (double)op.K * op.nbatch;
+  Key key{type_id<T>(), op.M, op.N, op.K, op.nbatch,
+          (op.sAi == 1 ? 1 : 0) | (op.sBj == 1 ? 2 : 0) | (op.sCi == 1 ? 4 : 0)};
+  Cfg c;
+  bool have = false;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end()) { c = it->second; have = true; }
+  }
+  if (!have) {
+    const bool alias = (const void*)op.C == (const void*)op.A || (const void*)op.C == (const void*)op.B;
+    if (autotune_enabled() && op.beta == T(0) && !alias && flops >= 4e6) {
+      c = tune(op, st);
+    } else {
+      c = heuristic(op.M, op.N, op.K, op.nbatch);
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[key] = c;
+  }
+  if (c.ks > 1) op.work = splitk_work(op, c.ks);
+  launch_cfg(op, c, st);
+}
+
+template void gemm_launch<double>(GemmOp<double>, hipStream_t);
+template void gemm_launch<float>(GemmOp<float>, hipStream_t);
+
+void zgemm_launch(ZGemmOp op, hipStream_t st) {
+  if (op.M <= 0 || op.N <= 0 || op.nbatch <= 0) return;
+  if (op.nb0 <= 0) op.nb0 = 1;
+  TTF_REQUIRE(op.nbatch <= 65535, "zgemm: batch count %d exceeds the grid limit", op.nbatch);
+  Cfg c = heuristic(op.M, op.N, op.K, op.nbatch);
+  int64_t tiles = ceil_div(op.M, c.bm) * ceil_div(op.N, c.bm);
+  dim3 grid((unsigned)tiles, (unsigned)op.nbatch, 1);
+  if (c.bm == 64)
+    hipLaunchKernelGGL((zgemm_mfma_kernel<64, 64>), grid, dim3(256), 0, st, op);
+  else
+    hipLaunchKernelGGL((zgemm_mfma_kernel<32, 32>), grid, dim3(256), 0, st, op);
+  TTF_HIP(hipGetLastError());
+}
+
+long gemm_tuned_shapes() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  return (long)g_tuned.size();
+}
+
+}  // namespace ttf
+
+extern "C" {
+
+int ttf_set_autotune(int enable) {
+  std::lock_guard<std::mutex> lk(ttf::g_tune_mu);
+  ttf::g_autotune = enable ? 1 : 0;
+  if (!enable) ttf::g_tuned.clear();
+  return 0;
+}
+
+long ttf_autotune_count(void) {
+  std::lock_guard<std::mutex> lk(ttf::g_tune_mu);
+  return ttf::g_tune_count;
+}
+
+}  // extern "C"

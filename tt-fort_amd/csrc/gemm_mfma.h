@@ -40,6 +40,18 @@ __device__ __forceinline__ f4_t mfma16(float a, float b, f4_t c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+// Row of the 16x16 result block held by (lane, accumulator register r).
+// Measured on gfx950: the f64 16x16x4 result interleaves the four registers
+// across row groups (row = lane/16 + 4r); the f32 16x16x4 result keeps them
+// contiguous (row = 4*(lane/16) + r).  The column is lane%16 in both.
+template <typename T> __device__ __forceinline__ int mfma_row(int lane, int r);
+template <> __device__ __forceinline__ int mfma_row<double>(int lane, int r) {
+  return (lane >> 4) + 4 * r;
+}
+template <> __device__ __forceinline__ int mfma_row<float>(int lane, int r) {
+  return 4 * (lane >> 4) + r;
+}
+
 template <typename T>
 struct GemmOp {
   int M, N, K;
@@ -63,14 +75,20 @@ __device__ __forceinline__ int xcd_remap(int wg, int total) {
   return (wg % nx) * per + (wg / nx);
 }
 
-template <typename T, int BM, int BN>
-__global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmOp<T> op) {
-  constexpr int BK = 16;
+// NW waves per work-group: NW = 4 arranges them 2x2 over the tile, NW = 1
+// lets a single wave own the whole tile.  Every wave keeps at least four
+// independent 16x16 accumulator chains (a 32x32 wave tile), since dependent
+// f64 MFMAs issue back to back only when several chains interleave.
+template <typename T, int BM, int BN, int BK, int NW = 4>
+__global__ __launch_bounds__(64 * NW) void gemm_mfma_kernel(GemmOp<T> op) {
+  constexpr int NT = 64 * NW;
   constexpr int PAD = 2;
-  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int WR = (NW == 4) ? 2 : 1;   // wave rows
+  constexpr int WC = NW / WR;             // wave columns
+  constexpr int WM = BM / WR, WN = BN / WC;
   constexpr int FM = WM / 16, FN = WN / 16;
-  constexpr int LA = BM * BK / 256;   // A elements per thread
-  constexpr int LB = BN * BK / 256;   // B elements per thread
+  constexpr int LA = BM * BK / NT;   // A elements per thread
+  constexpr int LB = BN * BK / NT;   // B elements per thread
   typedef typename Acc4<T>::type acc_t;
 
   __shared__ T As[BK][BM + PAD];
@@ -79,7 +97,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmOp<T> op) {
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int wm = wave / WC, wn = wave % WC;
 
   const int tilesM = (op.M + BM - 1) / BM;
   const int tilesN = (op.N + BN - 1) / BN;
@@ -104,13 +122,13 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmOp<T> op) {
   int ai[LA], ak[LA], bk[LB], bj[LB];
 #pragma unroll
   for (int q = 0; q < LA; ++q) {
-    int idx = tid + q * 256;
+    int idx = tid + q * NT;
     if (a_ifast) { ai[q] = idx % BM; ak[q] = idx / BM; }
     else         { ak[q] = idx % BK; ai[q] = idx / BK; }
   }
 #pragma unroll
   for (int q = 0; q < LB; ++q) {
-    int idx = tid + q * 256;
+    int idx = tid + q * NT;
     if (b_jfast) { bj[q] = idx % BN; bk[q] = idx / BN; }
     else         { bk[q] = idx % BK; bj[q] = idx / BK; }
   }
@@ -157,8 +175,8 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmOp<T> op) {
     }
   }
 
-  // Epilogue.  D fragment layout: lane l, register r -> row 4*(l/16)+r,
-  // column l%16 of the 16x16 block.
+  // Epilogue: lane l, register r holds row mfma_row<T>(l, r), column l%16
+  // of each 16x16 block.
   if (op.ksplit == 1) {
     T* __restrict__ C = op.C + b0 * op.sCb0 + b1 * op.sCb1;
 #pragma unroll
@@ -167,7 +185,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmOp<T> op) {
       for (int j = 0; j < FN; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          int gi = tm * BM + wm * WM + i * 16 + 4 * (lane >> 4) + r;
+          int gi = tm * BM + wm * WM + i * 16 + mfma_row<T>(lane, r);
           int gj = tn * BN + wn * WN + j * 16 + (lane & 15);
           if (gi < op.M && gj < op.N) {
             T* c = C + gi * op.sCi + gj * op.sCj;
@@ -184,7 +202,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmOp<T> op) {
       for (int j = 0; j < FN; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          int gi = tm * BM + wm * WM + i * 16 + 4 * (lane >> 4) + r;
+          int gi = tm * BM + wm * WM + i * 16 + mfma_row<T>(lane, r);
           int gj = tn * BN + wn * WN + j * 16 + (lane & 15);
           if (gi < op.M && gj < op.N) W[gi + (int64_t)gj * op.M] = acc[i][j][r];
         }
@@ -200,8 +218,14 @@ __global__ void gemm_splitk_reduce(GemmOp<T> op) {
     int b = (int)(t / mn);
     int64_t e = t % mn;
     int gi = (int)(e % op.M), gj = (int)(e / op.M);
+    // load all partial sums first (latency-bound), then add in fixed order
+    T part[16];
+#pragma unroll
+    for (int z = 0; z < 16; ++z)
+      part[z] = (z < op.ksplit) ? op.work[((int64_t)z * op.nbatch + b) * mn + e] : T(0);
     T s = 0;
-    for (int z = 0; z < op.ksplit; ++z) s += op.work[((int64_t)z * op.nbatch + b) * mn + e];
+#pragma unroll
+    for (int z = 0; z < 16; ++z) s += part[z];
     int b0 = b % op.nb0, b1 = b / op.nb0;
     T* c = op.C + b0 * op.sCb0 + b1 * op.sCb1 + gi * op.sCi + gj * op.sCj;
     T v = op.alpha * s;
@@ -325,7 +349,7 @@ __global__ __launch_bounds__(256) void zgemm_mfma_kernel(ZGemmOp op) {
     for (int j = 0; j < FN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        int gi = tm * BM + wm * WM + i * 16 + 4 * (lane >> 4) + r;
+        int gi = tm * BM + wm * WM + i * 16 + mfma_row<double>(lane, r);
         int gj = tn * BN + wn * WN + j * 16 + (lane & 15);
         if (gi < op.M && gj < op.N) {
           hipDoubleComplex* c = C + gi * op.sCi + gj * op.sCj;
@@ -343,62 +367,15 @@ __global__ __launch_bounds__(256) void zgemm_mfma_kernel(ZGemmOp op) {
 }
 
 // ---------------------------------------------------------------- launchers
-// Tile choice: 64x64 when that already gives enough work-groups to cover the
-// chip twice, else 32x32; split-K when even 32x32 tiles leave most compute
-// units idle and K is long.
-struct GemmPlan {
-  int bm, bn, ksplit;
-};
-
-inline GemmPlan plan_gemm(int M, int N, int K, int nbatch, bool allow_split) {
-  const int64_t cus = 256;
-  int64_t t64 = ceil_div(M, 64) * ceil_div(N, 64) * (int64_t)nbatch;
-  if (t64 >= 2 * cus) return {64, 64, 1};
-  int64_t t32 = ceil_div(M, 32) * ceil_div(N, 32) * (int64_t)nbatch;
-  GemmPlan p{32, 32, 1};
-  if (allow_split && t32 < cus && K >= 512) {
-    int ks = (int)std::min<int64_t>(ceil_div(2 * cus, t32), ceil_div(K, 256));
-    p.ksplit = std::max(1, std::min(ks, 64));
-  }
-  return p;
-}
-
+// Implemented in gemm.hip.  The tile shape (64x64 or 32x32, K chunk 16 or
+// 32) and the split-K factor of each GEMM are chosen by a runtime autotuner
+// keyed on (type, M, N, K, batch, operand layout): the first launch of a new
+// shape times the candidates with events on the stream and caches the
+// fastest.  Small-panel products (K <= 64, N <= 32, B shared by all batches)
+// go to a dedicated vector kernel instead of the matrix cores.
 template <typename T>
-void gemm_launch(GemmOp<T> op, hipStream_t st, const GemmPlan* forced = nullptr) {
-  if (op.M <= 0 || op.N <= 0 || op.nbatch <= 0) return;
-  if (op.nb0 <= 0) op.nb0 = 1;
-  GemmPlan p = forced ? *forced : plan_gemm(op.M, op.N, op.K, op.nbatch, op.work != nullptr);
-  op.ksplit = p.ksplit;
-  if (op.ksplit > 1 && op.work == nullptr) op.ksplit = 1;
-  TTF_REQUIRE(op.nbatch <= 65535, "gemm: batch count %d exceeds the grid limit", op.nbatch);
-  int64_t tiles = ceil_div(op.M, p.bm) * ceil_div(op.N, p.bn);
-  dim3 grid((unsigned)tiles, (unsigned)op.nbatch, (unsigned)op.ksplit);
-  if (p.bm == 64)
-    hipLaunchKernelGGL((gemm_mfma_kernel<T, 64, 64>), grid, dim3(256), 0, st, op);
-  else
-    hipLaunchKernelGGL((gemm_mfma_kernel<T, 32, 32>), grid, dim3(256), 0, st, op);
-  TTF_HIP(hipGetLastError());
-  if (op.ksplit > 1) {
-    int64_t total = (int64_t)op.M * op.N * op.nbatch;
-    int blocks = (int)std::min<int64_t>(ceil_div(total, 256), 4096);
-    hipLaunchKernelGGL((gemm_splitk_reduce<T>), dim3(blocks), dim3(256), 0, st, op);
-    TTF_HIP(hipGetLastError());
-  }
-}
-
-inline void zgemm_launch(ZGemmOp op, hipStream_t st) {
-  if (op.M <= 0 || op.N <= 0 || op.nbatch <= 0) return;
-  if (op.nb0 <= 0) op.nb0 = 1;
-  TTF_REQUIRE(op.nbatch <= 65535, "zgemm: batch count %d exceeds the grid limit", op.nbatch);
-  GemmPlan p = plan_gemm(op.M, op.N, op.K, op.nbatch, false);
-  int64_t tiles = ceil_div(op.M, p.bm) * ceil_div(op.N, p.bn);
-  dim3 grid((unsigned)tiles, (unsigned)op.nbatch, 1);
-  if (p.bm == 64)
-    hipLaunchKernelGGL((zgemm_mfma_kernel<64, 64>), grid, dim3(256), 0, st, op);
-  else
-    hipLaunchKernelGGL((zgemm_mfma_kernel<32, 32>), grid, dim3(256), 0, st, op);
-  TTF_HIP(hipGetLastError());
-}
+void gemm_launch(GemmOp<T> op, hipStream_t st);
+void zgemm_launch(ZGemmOp op, hipStream_t st);
 
 // Convenience constructor for a column-major GEMM with optional transposes:
 // C (M x N, leading dim ldc) = alpha * op(A) op(B) + beta * C.

@@ -70,6 +70,7 @@ enum Arena {
   kArenaSolver,      // local solver bases (LOBPCG / GMRES / Krylov)
   kArenaSmall,       // small dense matrices and scalars
   kArenaTrunc,       // SVD / QR workspaces
+  kArenaSplitK,      // split-K partial sums of the GEMM engine
   kArenaCount
 };
 void* arena(Arena a, size_t bytes);
