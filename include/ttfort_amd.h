@@ -144,6 +144,51 @@ int ttf_dphi2_right(int rx1, int rx2, int ry1, int n, int ry2, const double* phi
 int ttf_d2d_fullmat(int rx1, int m, int rx2, int ra1, int ra2, const double* phi1,
                     const double* A, const double* phi2, double* B);
 
+/* ------------------------------------------------ dense linear algebra */
+/* Thin SVD of an m x n column-major device matrix: A = U diag(S) VT, k =
+ * min(m,n) singular values in descending order, U (m x k, ldu), VT (k x n,
+ * ldvt).  Blocked one-sided Jacobi; replaces the dgesvd calls of the
+ * truncation steps (tt_eigb.f90:312, :361; tt_adapt_als.f90:812, :930;
+ * svd.f90:25).  *sweeps (optional) returns the Jacobi sweeps used.
+ * Synchronous. */
+int ttf_dgesvd(int m, int n, const double* A, int64_t lda, double* U, int64_t ldu, double* S,
+               double* VT, int64_t ldvt, int* sweeps);
+/* Eigen-decomposition of `count` symmetric n x n device matrices (n <= 64)
+ * stored consecutively (A is overwritten): eigenvalues to w (ascending, or
+ * descending if desc), eigenvectors to Q.  Replaces the small dsyev calls
+ * (tt_eigb.f90:238 for local sizes <= 64; tt_linalg.f90:630). */
+int ttf_dsyev_small(int n, int count, double* A, double* w, double* Q, int desc);
+/* G (p x q, ldg) = X^T Y for tall X (N x p), Y (N x q); p, q <= 64. */
+int ttf_dgram(int N, int p, int q, const double* X, int64_t ldx, const double* Y, int64_t ldy,
+              double* G, int ldg);
+
+/* ------------------------------------------------------- sweep drivers */
+typedef struct {
+  int sweeps;            /* sweeps performed */
+  long total_matvecs;    /* local matvecs (vectors) over the run */
+  double last_ermax;     /* max relative change of sum(lambda) in the last sweep */
+  int max_rank;          /* largest TT rank at exit */
+  double seconds;        /* wall time of the call */
+  double solve_seconds;  /* local eigensolves */
+  double trunc_seconds;  /* SVD truncations and core moves */
+  double phi_seconds;    /* interface updates */
+} ttf_eigb_stats;
+
+/* Block eigensolver in TT format: the B lowest eigenpairs of the symmetric
+ * TT-matrix A.  Argument layout of tt_eigb (tt_eigb.f90:110-120): d modes,
+ * n(d)=m(d) mode sizes, ra(d+1) operator ranks, crA the concatenated
+ * operator cores (ra_k, n_k, m_k, ra_k+1), crY0 the initial block TT with
+ * ranks ry(d+1) (ry(d+1) == B, in/out), eps the tolerance, rmax the rank
+ * cap, lambda(B) out.  kickrank is accepted and ignored, as in the
+ * reference; nswp <= 0 means 20, max_full_size <= 0 means 2000.  The
+ * result cores (what the reference leaves in tt_block_eig::result_core)
+ * are written to `result` (capacity result_cap doubles; pass NULL to query
+ * the size, returned in *result_size).  Host pointers; device-resident run. */
+int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, const double* crA,
+                const double* crY0, int* ry, double eps, int rmax, double* lambda, int B,
+                int kickrank, int nswp, int max_full_size, int verb, double* result,
+                int64_t result_cap, int64_t* result_size, ttf_eigb_stats* stats);
+
 /* ------------------------------------------ reference-compatible symbols */
 /* tt_linalg.f90:127-159, dbfun3 bind(c).  Host pointers; res1/res2 ignored. */
 void dbfun3(const ttf_fint* rx1, const ttf_fint* m, const ttf_fint* rx2,

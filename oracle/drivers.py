@@ -1,0 +1,140 @@
+"""Oracle restatements of the tt-fort sweep drivers (TEST INFRASTRUCTURE).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use
+this module.
+
+tt_eigb   -- tt_eigb.f90:110-436, block eigensolver sweep.  The local
+             eigensolver the reference calls is PRIMME (DYNAMIC method) with
+             the same matvec (dbfun3); PRIMME is vendored C that is not
+             available to numpy, so the restatement uses a dense symmetric
+             eigensolver for small local problems (as the reference does
+             below max_full_size, tt_eigb.f90:234-249) and ARPACK Lanczos
+             (scipy.sparse.linalg.eigsh) on the same matvec otherwise, with
+             the reference's tolerance eps2/resid_damp.  Eigenvalues are
+             pinned by exact diagonalisation; sweep trajectories are not
+             reproducible even between two runs of the reference (PRIMME's
+             DYNAMIC switching is timing dependent), so parity is judged on
+             eigenvalues and residuals.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import kernels as K
+
+
+def _svd(M):
+    U, s, VT = np.linalg.svd(M, full_matrices=False)
+    return U, s, VT
+
+
+def local_eigs(shape, phi1, A, phi2, x0, B, tol, max_full_size):
+    """Lowest B eigenpairs of the local operator (tt_eigb.f90:234-292)."""
+    rx1, m, rx2 = shape[:3]
+    N = rx1 * m * rx2
+    if N < max_full_size or N <= 3 * B + 2:
+        Bm = K.d2d_fullmat(rx1, m, rx2, shape[6], shape[7], phi1, A, phi2)
+        w, V = np.linalg.eigh(0.5 * (Bm + Bm.T))
+        return w[:B], V[:, :B], 0
+    import scipy.sparse.linalg as sla
+    count = [0]
+
+    def mv(v):
+        count[0] += 1
+        return K.dbfun3(*shape, phi1, A, phi2, np.asarray(v, float).reshape(-1))
+
+    op = sla.LinearOperator((N, N), matvec=mv, dtype=float)
+    w, V = sla.eigsh(op, k=B, which="SA", tol=tol, v0=x0[:, 0] if x0 is not None else None,
+                     ncv=min(N, max(2 * B + 1, 20)))
+    idx = np.argsort(w)
+    return w[idx], V[:, idx], count[0]
+
+
+def tt_eigb(d, n, ra, crA, crY0, ry, eps, rmax, B, nswp=20, max_full_size=2000, verb=0):
+    """tt_eigb.f90:110-436 restated.  Returns (lambda, cores, ry, stats)."""
+    n = [int(v) for v in n]
+    ra = [int(v) for v in ra]
+    ry = [int(v) for v in ry]
+    assert ry[d] == B
+    eps2 = eps / np.sqrt(d)
+    resid_damp = 10.0
+    pa = [0]
+    for k in range(d):
+        pa.append(pa[-1] + ra[k] * n[k] * n[k] * ra[k + 1])
+    Acore = [np.asarray(crA[pa[k]:pa[k + 1]], float) for k in range(d)]
+    cores, p = [], 0
+    for k in range(d):
+        sz = ry[k] * n[k] * ry[k + 1]
+        cores.append(np.array(crY0[p:p + sz], float))
+        p += sz
+    phi = [None] * (d + 1)
+    phi[0] = np.ones(1)
+    phi[d] = np.ones(1)
+    # initial QR + phi_left (tt_eigb.f90:201-217)
+    for k in range(d):
+        mrow, ncol = ry[k] * n[k], ry[k + 1]
+        Q, R = np.linalg.qr(K.F(cores[k], mrow, ncol))
+        rnew = Q.shape[1]
+        cores[k] = Q.reshape(-1, order="F")
+        if k < d - 1:
+            nxt = R @ K.F(cores[k + 1], ncol, n[k + 1] * ry[k + 2])
+            cores[k + 1] = nxt.reshape(-1, order="F")
+            ry[k + 1] = rnew
+            phi[k + 1] = K.dphi_left(ry[k], n[k], ry[k + 1], ry[k], n[k], ry[k + 1], ra[k], ra[k + 1],
+                                     phi[k], Acore[k], cores[k], cores[k])
+    ry[d] = 1
+    k, direction, swp = d - 1, -1, 1
+    ermax, fvold, total_mv = 0.0, 0.0, 0
+    lam = np.zeros(B)
+    last = 0
+    while swp <= nswp:
+        shape = (ry[k], n[k], ry[k + 1], ry[k], n[k], ry[k + 1], ra[k], ra[k + 1])
+        N = ry[k] * n[k] * ry[k + 1]
+        x0 = K.F(cores[k], N, B)
+        lam, X, mv = local_eigs(shape, phi[k], Acore[k], phi[k + 1], x0, B, eps2 / resid_damp,
+                                max_full_size)
+        total_mv += mv
+        fv = lam.sum()
+        ermax = max(ermax, (fvold - fv) / abs(fv))
+        fvold = fv
+        if verb:
+            print("swp %d i [%d/%d] N %d ermax %.3e" % (swp, k + 1, d, N, ermax))
+        if direction < 0 and k > 0:
+            M = np.reshape(X.T.reshape(-1, order="F"), (B * ry[k], n[k] * ry[k + 1]), order="F")
+            U, s, VT = _svd(M)
+            nn = min(K.my_chop3(s, eps2), rmax)
+            cores[k] = VT[:nn, :].reshape(-1, order="F")
+            US = U[:, :nn] * s[:nn]                                   # (b, r_k) x j
+            US = np.transpose(US.reshape(B, ry[k], nn, order="F"), (1, 2, 0))   # r_k, j, b
+            left = K.F(cores[k - 1], ry[k - 1] * n[k - 1], ry[k])
+            cores[k - 1] = (left @ US.reshape(ry[k], nn * B, order="F")).reshape(-1, order="F")
+            ry[k] = nn
+            phi[k] = K.dphi_right(ry[k], n[k], ry[k + 1], ry[k], n[k], ry[k + 1], ra[k], ra[k + 1],
+                                  phi[k + 1], Acore[k], cores[k], cores[k])
+        elif direction > 0 and k < d - 1:
+            M = np.reshape(X.reshape(-1, order="F"), (ry[k] * n[k], ry[k + 1] * B), order="F")
+            U, s, VT = _svd(M)
+            nn = min(K.my_chop3(s, eps2), rmax)
+            cores[k] = U[:, :nn].reshape(-1, order="F")
+            SVT = (VT[:nn, :] * s[:nn, None]).reshape(nn, ry[k + 1], B, order="F")
+            right = K.F(cores[k + 1], ry[k + 1], n[k + 1] * ry[k + 2])
+            out = np.stack([SVT[:, :, b] @ right for b in range(B)], axis=2)   # j, (n r), b
+            cores[k + 1] = out.reshape(-1, order="F")
+            ry[k + 1] = nn
+            phi[k + 1] = K.dphi_left(ry[k], n[k], ry[k + 1], ry[k], n[k], ry[k + 1], ra[k], ra[k + 1],
+                                     phi[k], Acore[k], cores[k], cores[k])
+        else:
+            cores[k] = X.reshape(-1, order="F")
+        if direction > 0 and k == d - 2:
+            direction, k = -1, d - 1
+            last = swp
+            swp += 1
+            if ermax < eps:
+                break
+            ermax = 0.0
+        elif direction < 0 and k == 1:
+            direction, k = 1, 0
+        else:
+            k += direction
+    ry[d] = B
+    return lam, cores, ry, dict(sweeps=last, total_matvecs=total_mv)

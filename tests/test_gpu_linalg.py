@@ -1,0 +1,90 @@
+"""GPU dense linear algebra (through the C ABI) against numpy/LAPACK.
+
+SVD: singular values to 1e-12 relative + 1e-14*smax absolute (LAPACK's own
+guarantee is absolute, so the comparison is two-sided), orthonormal factors
+to 1e-12, reconstruction to 1e-12 relative.  Small symmetric eigensolver:
+eigenvalues to 1e-12*||A||, eigen-residual and orthonormality to 1e-12.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _check_svd(A, U, s, VT, rel=1e-12):
+    m, n = A.shape
+    k = min(m, n)
+    assert U.shape == (m, k) and VT.shape == (k, n) and s.shape == (k,)
+    assert np.all(np.diff(s) <= 1e-15 * max(s[0], 1e-300))
+    s_ref = np.linalg.svd(A, compute_uv=False)
+    smax = max(s_ref[0], 1e-300)
+    assert np.all(np.abs(s - s_ref) <= rel * s_ref + 1e-14 * smax), np.abs(s - s_ref).max()
+    assert np.allclose(U.T @ U, np.eye(k), atol=1e-12)
+    assert np.allclose(VT @ VT.T, np.eye(k), atol=1e-12)
+    assert np.linalg.norm(A - (U * s) @ VT) <= 1e-12 * max(np.linalg.norm(A), 1e-300)
+
+
+@pytest.mark.parametrize("m,n", [(12, 5), (5, 12), (33, 1), (1, 7), (200, 80), (80, 200),
+                                 (256, 640), (640, 256), (64, 64), (1000, 17)])
+def test_gesvd_random(ttlib, m, n):
+    import ttfort
+    rng = np.random.default_rng(m * 1000 + n)
+    A = rng.standard_normal((m, n))
+    U, s, VT, sw = ttfort.gesvd(A)
+    _check_svd(A, U, s, VT)
+
+
+def test_gesvd_rank_deficient_and_zero(ttlib):
+    import ttfort
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((100, 10)) @ rng.standard_normal((10, 50))
+    U, s, VT, _ = ttfort.gesvd(A)
+    _check_svd(A, U, s, VT)
+    assert s[10] < 1e-12 * s[0]
+    Z = np.zeros((20, 6))
+    U, s, VT, _ = ttfort.gesvd(Z)
+    assert np.all(s == 0)
+    assert np.allclose(U.T @ U, np.eye(6), atol=1e-12)
+
+
+def test_gesvd_graded_relative_accuracy(ttlib):
+    """Graded singular values 1 .. 1e-14: one-sided Jacobi keeps them to high
+    relative accuracy; the truncation tail sums depend on the small ones."""
+    import ttfort
+    rng = np.random.default_rng(6)
+    m, n = 120, 40
+    Qa, _ = np.linalg.qr(rng.standard_normal((m, n)))
+    Qb, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    s_true = np.logspace(0, -14, n)
+    A = (Qa * s_true) @ Qb.T
+    U, s, VT, _ = ttfort.gesvd(A)
+    assert np.all(np.abs(s - s_true) <= 1e-9 * s_true + 2e-16 * n), np.max(np.abs(s - s_true) / s_true)
+    assert np.allclose(U.T @ U, np.eye(n), atol=1e-12)
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 24, 48, 64])
+def test_syev_small(ttlib, n):
+    import ttfort
+    rng = np.random.default_rng(n)
+    As = []
+    for _ in range(3):
+        X = rng.standard_normal((n, n))
+        As.append(X + X.T)
+    w, Q = ttfort.syev_small(np.array(As))
+    for a, wi, qi in zip(As, w, Q):
+        nrm = np.linalg.norm(a, 2)
+        assert np.allclose(wi, np.linalg.eigvalsh(a), atol=1e-12 * nrm)
+        assert np.linalg.norm(a @ qi - qi * wi) <= 1e-12 * nrm * n
+        assert np.allclose(qi.T @ qi, np.eye(n), atol=1e-12)
+    wd, _ = ttfort.syev_small(np.array(As), desc=True)
+    assert np.allclose(wd, w[:, ::-1])
+
+
+def test_gram(ttlib):
+    import ttfort
+    rng = np.random.default_rng(7)
+    for N, p, q in ((10, 3, 4), (5000, 24, 24), (32768, 8, 16), (100, 64, 64)):
+        X = rng.standard_normal((N, p))
+        Y = rng.standard_normal((N, q))
+        G = ttfort.gram(X, Y)
+        assert np.allclose(G, X.T @ Y, rtol=1e-12, atol=1e-10)

@@ -30,7 +30,7 @@ CFLAGS = [
 ]
 
 # sources compiled once; refabi.hip is compiled per Fortran integer kind
-SHARED_SOURCES = ["runtime.hip", "gemm.hip", "contract.hip"]
+SHARED_SOURCES = ["runtime.hip", "gemm.hip", "contract.hip", "linalg.hip", "lobpcg.hip", "eigb.hip"]
 KIND_SOURCES = ["refabi.hip"]
 
 

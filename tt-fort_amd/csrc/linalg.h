@@ -1,0 +1,33 @@
+// Dense FP64 linear algebra on the device for the sweep drivers: small
+// symmetric eigenproblems (Rayleigh-Ritz, basis orthonormalisation), tall-
+// skinny Gram products, and the thin SVD behind the rank truncation.
+#pragma once
+
+#include "ttf_common.h"
+
+namespace ttf {
+
+// G (p x q, leading dim ldg) = X^T Y for X (N x p, ldx), Y (N x q, ldy);
+// p, q <= 64.  Deterministic two-pass reduction over row chunks.
+void tsgram(int N, int p, int q, const double* X, int64_t ldx, const double* Y, int64_t ldy,
+            double* G, int ldg, hipStream_t st);
+
+// Eigen-decomposition of `count` symmetric n x n matrices (n <= 64), stored
+// consecutively with leading dimension n at A (overwritten).  Eigenvalues go
+// to w (ascending, or descending with desc = 1), eigenvectors to Q (n x n
+// each).  Cyclic Jacobi in LDS, one work-group per matrix.
+void syev_small(int n, int count, double* A, double* w, double* Q, int desc, hipStream_t st);
+
+// Thin SVD of an m x n column-major matrix A (lda): A = U diag(S) VT with
+// k = min(m, n) values in descending order; U (m x k, ldu), VT (k x n, ldvt).
+// Blocked one-sided Jacobi (pair-Gram eigenvectors as block rotations),
+// singular values from the final column norms; columns belonging to zero
+// singular values are completed to an orthonormal set.
+// Returns the number of Jacobi sweeps used.
+int gesvd(int m, int n, const double* A, int64_t lda, double* U, int64_t ldu, double* S,
+          double* VT, int64_t ldvt, hipStream_t st);
+
+// B (n x m) = A (m x n)^T, both column-major with leading dims.
+void transpose(int m, int n, const double* A, int64_t lda, double* B, int64_t ldb, hipStream_t st);
+
+}  // namespace ttf

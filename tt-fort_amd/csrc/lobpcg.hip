@@ -1,0 +1,439 @@
+// Device-resident block eigensolver for the local problems of tt_eigb.
+//
+// The reference hands each local problem to PRIMME with a matvec callback
+// (tt_eigb.f90:251-283; method DYNAMIC, maxBasis 100, minRestart B+1,
+// maxMatvecs 500, eps = eps2/resid_damp).  Here the whole solve stays on the
+// device: LOBPCG with an orthonormal basis S = [X W P] (one of the methods
+// PRIMME itself offers; the reference has it wired in, commented, at
+// tt_eigb.f90:254), block size nev + guard vectors.
+//
+// Per iteration: residuals W = AX - X diag(theta); convergence test
+// ||r_i|| <= tol * anorm on the host (the only host synchronisation);
+// W orthogonalised twice against [X P] and orthonormalised by SVQB
+// (eigen-decomposition of W^T W, near-null directions dropped); one block
+// matvec AW; Rayleigh-Ritz on S^T A S in a single work-group (cyclic Jacobi
+// in LDS), which also builds the next search direction P in the small
+// space, orthogonal to the new Ritz vectors; then [X P] <- S [C Z].
+// Dropped (zero) columns carry a large diagonal penalty in the projected
+// matrix so they can never produce a spurious Ritz value.
+#include "solvers.h"
+
+#include <cmath>
+#include <vector>
+
+namespace ttf {
+
+namespace {
+
+constexpr int kMaxM = 64;   // largest projected basis (3 * block size)
+
+template <int LD>
+__device__ void jacobi_eig(double (*A)[LD], double (*V)[LD], int n, double* rc, double* rs, int* rp,
+                           int* rq, int* flag) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int nn = n + (n & 1), half = nn / 2;
+  for (int sweep = 0; sweep < 40; ++sweep) {
+    if (tid == 0) *flag = 0;
+    __syncthreads();
+    for (int r = 0; r < nn - 1; ++r) {
+      for (int k = tid; k < half; k += nt) {
+        int a, b;
+        if (k == 0) { a = r; b = nn - 1; }
+        else { a = (r + k) % (nn - 1); b = (r - k + (nn - 1)) % (nn - 1); }
+        int p = min(a, b), q = max(a, b);
+        double c = 1.0, s = 0.0;
+        if (q < n) {
+          double apq = A[p][q], app = A[p][p], aqq = A[q][q];
+          if (fabs(apq) > 1e-300 && fabs(apq) > 1e-17 * sqrt(fabs(app) * fabs(aqq))) {
+            double tau = (aqq - app) / (2.0 * apq);
+            double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+            c = 1.0 / sqrt(1.0 + t * t);
+            s = t * c;
+            *flag = 1;
+          }
+        }
+        rp[k] = p; rq[k] = q; rc[k] = c; rs[k] = s;
+      }
+      __syncthreads();
+      for (int e = tid; e < half * n; e += nt) {
+        int k = e / n, i = e % n;
+        int p = rp[k], q = rq[k];
+        if (q >= n || rs[k] == 0.0) continue;
+        double c = rc[k], s = rs[k];
+        double aip = A[i][p], aiq = A[i][q];
+        A[i][p] = c * aip - s * aiq;
+        A[i][q] = s * aip + c * aiq;
+        double vip = V[i][p], viq = V[i][q];
+        V[i][p] = c * vip - s * viq;
+        V[i][q] = s * vip + c * viq;
+      }
+      __syncthreads();
+      for (int e = tid; e < half * n; e += nt) {
+        int k = e / n, j = e % n;
+        int p = rp[k], q = rq[k];
+        if (q >= n || rs[k] == 0.0) continue;
+        double c = rc[k], s = rs[k];
+        double apj = A[p][j], aqj = A[q][j];
+        A[p][j] = c * apj - s * aqj;
+        A[q][j] = s * apj + c * aqj;
+      }
+      __syncthreads();
+    }
+    if (*flag == 0) break;
+    __syncthreads();
+  }
+}
+
+// SVQB: G (k x k) = W^T W -> T = U diag(lambda^-1/2) over directions with
+// lambda > 1e-14 lambda_max (others get a zero column and mask 0).
+__global__ __launch_bounds__(256) void svqb_kernel(int k, const double* __restrict__ G,
+                                                   double* __restrict__ T, int* __restrict__ mask) {
+  __shared__ double A[kMaxM][kMaxM + 1], V[kMaxM][kMaxM + 1];
+  __shared__ double rc[kMaxM / 2], rs[kMaxM / 2], dsc[kMaxM];
+  __shared__ int rp[kMaxM / 2], rq[kMaxM / 2], flag;
+  for (int e = threadIdx.x; e < k * k; e += blockDim.x) {
+    int i = e % k, j = e / k;
+    double gi = G[i + k * i], gj = G[j + k * j];
+    // scale to unit diagonal first (the "S" of SVQB) for a well-conditioned eig
+    double di = gi > 0 ? 1.0 / sqrt(gi) : 0.0, dj = gj > 0 ? 1.0 / sqrt(gj) : 0.0;
+    A[i][j] = 0.5 * (G[i + k * j] + G[j + k * i]) * di * dj;
+    V[i][j] = (i == j) ? 1.0 : 0.0;
+  }
+  for (int i = threadIdx.x; i < k; i += blockDim.x) {
+    double gi = G[i + k * i];
+    dsc[i] = gi > 0 ? 1.0 / sqrt(gi) : 0.0;
+  }
+  __syncthreads();
+  jacobi_eig<kMaxM + 1>(A, V, k, rc, rs, rp, rq, &flag);
+  __syncthreads();
+  double lmax = 0.0;
+  for (int i = 0; i < k; ++i) lmax = fmax(lmax, A[i][i]);
+  for (int e = threadIdx.x; e < k * k; e += blockDim.x) {
+    int i = e % k, j = e / k;
+    double l = A[j][j];
+    bool keep = l > 1e-14 * lmax && lmax > 0.0;
+    T[i + k * j] = keep ? dsc[i] * V[i][j] / sqrt(l) : 0.0;
+  }
+  for (int j = threadIdx.x; j < k; j += blockDim.x) {
+    double l = A[j][j];
+    mask[j] = (l > 1e-14 * lmax && lmax > 0.0) ? 1 : 0;
+  }
+}
+
+// Rayleigh-Ritz in the small space.  H (m x m) = S^T A S; mask (m) marks
+// valid basis columns; columns [0, nx) are the current Ritz vectors X.
+// Writes theta (bs), CZ (m x 2bs: Ritz coefficients C, then P-direction
+// coefficients Z), maskP (bs) and amax = max |valid Ritz value|.
+__global__ __launch_bounds__(256) void rr_kernel(int m, int bs, int nx, const double* __restrict__ H,
+                                                 const int* __restrict__ mask, double* __restrict__ theta,
+                                                 double* __restrict__ CZ, int* __restrict__ maskP,
+                                                 double* __restrict__ amax) {
+  __shared__ double A[kMaxM][kMaxM + 1], V[kMaxM][kMaxM + 1], Y[kMaxM][kMaxM + 1];
+  __shared__ double Z[kMaxM][kMaxM / 2 + 1];
+  __shared__ double rc[kMaxM / 2], rs[kMaxM / 2], w[kMaxM], red[256];
+  __shared__ int rp[kMaxM / 2], rq[kMaxM / 2], flag;
+  __shared__ double big;
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    double mx = 0.0;
+    for (int e = 0; e < m * m; ++e) mx = fmax(mx, fabs(H[e]));
+    big = 1e6 * (1.0 + mx);
+  }
+  __syncthreads();
+  for (int e = tid; e < m * m; e += blockDim.x) {
+    int i = e % m, j = e / m;
+    bool ok = mask[i] && mask[j];
+    double h = 0.5 * (H[i + m * j] + H[j + m * i]);
+    A[i][j] = ok ? h : ((i == j) ? big : 0.0);
+    V[i][j] = (i == j) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  jacobi_eig<kMaxM + 1>(A, V, m, rc, rs, rp, rq, &flag);
+  __syncthreads();
+  // ascending order
+  for (int i = tid; i < m; i += blockDim.x) {
+    double wi = A[i][i];
+    int pos = 0;
+    for (int j = 0; j < m; ++j) {
+      double wj = A[j][j];
+      pos += (wj < wi || (wj == wi && j < i)) ? 1 : 0;
+    }
+    w[pos] = wi;
+    for (int r = 0; r < m; ++r) Y[r][pos] = V[r][i];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double mx = 0.0;
+    for (int i = 0; i < m; ++i)
+      if (w[i] < 0.5 * big) mx = fmax(mx, fabs(w[i]));
+    *amax = mx;
+  }
+  for (int i = tid; i < bs; i += blockDim.x) theta[i] = w[i];
+  // Z0 = C with the X rows removed, then orthogonalised against C
+  for (int e = tid; e < m * bs; e += blockDim.x) {
+    int r = e % m, c = e / m;
+    Z[r][c] = (r < nx) ? 0.0 : Y[r][c];
+  }
+  __syncthreads();
+  for (int pass = 0; pass < 2; ++pass) {
+    // Z -= C (C^T Z): coefficient matrix computed per (i, c)
+    __shared__ double P[kMaxM / 2][kMaxM / 2 + 1];
+    for (int e = tid; e < bs * bs; e += blockDim.x) {
+      int i = e % bs, c = e / bs;
+      double s = 0.0;
+      for (int r = 0; r < m; ++r) s += Y[r][i] * Z[r][c];
+      P[i][c] = s;
+    }
+    __syncthreads();
+    for (int e = tid; e < m * bs; e += blockDim.x) {
+      int r = e % m, c = e / m;
+      double s = 0.0;
+      for (int i = 0; i < bs; ++i) s += Y[r][i] * P[i][c];
+      Z[r][c] -= s;
+    }
+    __syncthreads();
+  }
+  // modified Gram-Schmidt on the columns of Z (twice)
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int c = 0; c < bs; ++c) {
+      for (int j = 0; j < c; ++j) {
+        double s = 0.0;
+        for (int r = tid; r < m; r += blockDim.x) s += Z[r][j] * Z[r][c];
+        red[tid] = s;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+          if (tid < o) red[tid] += red[tid + o];
+          __syncthreads();
+        }
+        double d = red[0];
+        __syncthreads();
+        for (int r = tid; r < m; r += blockDim.x) Z[r][c] -= d * Z[r][j];
+        __syncthreads();
+      }
+      double s = 0.0;
+      for (int r = tid; r < m; r += blockDim.x) s += Z[r][c] * Z[r][c];
+      red[tid] = s;
+      __syncthreads();
+      for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) red[tid] += red[tid + o];
+        __syncthreads();
+      }
+      double nrm = sqrt(red[0]);
+      __syncthreads();
+      double inv = nrm > 1e-10 ? 1.0 / nrm : 0.0;
+      for (int r = tid; r < m; r += blockDim.x) Z[r][c] *= inv;
+      if (tid == 0 && pass == 1) maskP[c] = nrm > 1e-10 ? 1 : 0;
+      __syncthreads();
+    }
+  }
+  for (int e = tid; e < m * 2 * bs; e += blockDim.x) {
+    int r = e % m, c = e / m;
+    CZ[r + m * c] = (c < bs) ? Y[r][c] : Z[r][c - bs];
+  }
+}
+
+// W = AX - X diag(theta)
+__global__ void residual_kernel(int N, int bs, const double* __restrict__ X,
+                                const double* __restrict__ AX, int64_t ld,
+                                const double* __restrict__ theta, double* __restrict__ W) {
+  int64_t total = (int64_t)N * bs;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int c = (int)(e / N);
+    int64_t i = e % N;
+    W[i + ld * c] = AX[i + ld * c] - theta[c] * X[i + ld * c];
+  }
+}
+
+__global__ void colnorms_kernel(int N, const double* __restrict__ W, int64_t ld, double* out) {
+  __shared__ double red[256];
+  const double* w = W + ld * blockIdx.x;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) s += w[i] * w[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = sqrt(red[0]);
+}
+
+// deterministic pseudo-random fill of columns [c0, c1) (values in [-0.5, 0.5))
+__global__ void random_cols(int N, int c0, int c1, double* X, int64_t ld, unsigned seed) {
+  int64_t total = (int64_t)N * (c1 - c0);
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int c = c0 + (int)(e / N);
+    int64_t i = e % N;
+    unsigned h = (unsigned)(i * 2654435761u) ^ (seed + 0x9e3779b9u * (unsigned)(c + 1));
+    h ^= h >> 16; h *= 0x7feb352du; h ^= h >> 15; h *= 0x846ca68bu; h ^= h >> 16;
+    X[i + ld * c] = (double)(h & 0xffffff) / (double)0x1000000 - 0.5;
+  }
+}
+
+int grid_for(int64_t n) { return (int)std::min<int64_t>(std::max<int64_t>(ceil_div(n, 256), 1), 8192); }
+
+void copy_cols(const double* src, int64_t lds, double* dst, int64_t ldd, int rows, int ncols,
+               hipStream_t st) {
+  if (ncols <= 0) return;
+  TTF_HIP(hipMemcpy2DAsync(dst, sizeof(double) * ldd, src, sizeof(double) * lds, sizeof(double) * rows,
+                           ncols, hipMemcpyDeviceToDevice, st));
+}
+
+// out (N x q) = S (N x p) * C (p x q)   [overwrite]
+void mul_small(int N, int p, int q, const double* S, int64_t lds, const double* Cm, int ldc, double* out,
+               int64_t ldo, double beta, hipStream_t st) {
+  GemmOp<double> g = colmajor<double>('N', 'N', N, q, p, 1.0, S, lds, Cm, ldc, beta, out, ldo);
+  gemm_launch(g, st);
+}
+
+}  // namespace
+
+EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* lambda, double tol,
+                   int max_matvecs, int dense_max, hipStream_t st) {
+  EigStats stats;
+  const int N = op.N();
+  TTF_REQUIRE(nev >= 1 && nev <= N, "block_eig: nev=%d invalid for N=%d", nev, N);
+  // ---------------- dense path: explicit local matrix + Jacobi
+  if (N <= kMaxM && (N <= dense_max || N < 3 * (nev + guard))) {
+    double* B = (double*)arena(kArenaSolver, sizeof(double) * ((size_t)N * N * 2 + N));
+    double* Q = B + (size_t)N * N;
+    double* w = Q + (size_t)N * N;
+    fullmat_launch(op.s.ry1, op.s.n, op.s.ry2, op.s.ra1, op.s.ra2, op.phi1, op.A, op.phi2, B, st);
+    syev_small(N, 1, B, w, Q, 0, st);
+    copy_cols(Q, N, Xout, N, N, nev, st);
+    std::vector<double> hw(N);
+    TTF_HIP(hipMemcpyAsync(hw.data(), w, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+    TTF_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < nev; ++i) lambda[i] = hw[i];
+    stats.dense = 1;
+    stats.anorm = std::max(std::fabs(hw[0]), std::fabs(hw[N - 1]));
+    return stats;
+  }
+  // ---------------- LOBPCG
+  int bs = std::min(nev + guard, N / 3);
+  bs = std::max(bs, nev);
+  TTF_REQUIRE(3 * bs <= kMaxM && 3 * bs <= N, "block_eig: block %d too large for N=%d", bs, N);
+  const int64_t ld = N;
+  const size_t colsz = (size_t)N;
+  double* base = (double*)arena(kArenaSolver, sizeof(double) * colsz * (3 + 3 + 2 + 2 + 1) * bs);
+  double* S = base;                       // [X W P]   N x 3bs
+  double* AS = S + colsz * 3 * bs;        // [AX AW AP]
+  double* Sn = AS + colsz * 3 * bs;       // [Xn Pn]   N x 2bs
+  double* ASn = Sn + colsz * 2 * bs;
+  double* Wt = ASn + colsz * 2 * bs;      // N x bs
+  double* X = S, *W = S + colsz * bs, *P = S + colsz * 2 * bs;
+  double* AX = AS, *AW = AS + colsz * bs, *AP = AS + colsz * 2 * bs;
+  // small device buffers
+  const int M2 = kMaxM * kMaxM;
+  double* sm = (double*)arena(kArenaSmall, sizeof(double) * (4 * M2 + 4 * kMaxM) + sizeof(int) * 4 * kMaxM);
+  double* Gm = sm;
+  double* Tm = Gm + M2;
+  double* Hm = Tm + M2;
+  double* CZ = Hm + M2;
+  double* theta = CZ + M2;
+  double* nrm = theta + kMaxM;
+  double* amax = nrm + kMaxM;
+  int* imask = (int*)(amax + 2 * kMaxM);   // m entries: validity of S columns
+  int* maskW = imask + kMaxM;
+  int* maskP = maskW + kMaxM;
+
+  // initial block: guess + guard vectors, SVQB, A X, Rayleigh-Ritz on X
+  copy_cols(Xout, N, X, ld, N, nev, st);
+  hipLaunchKernelGGL(random_cols, dim3(grid_for((int64_t)N * (bs - nev))), dim3(256), 0, st, N, nev,
+                     bs, X, ld, 12345u + (unsigned)N);
+  TTF_HIP(hipGetLastError());
+  TTF_HIP(hipMemsetAsync(P, 0, sizeof(double) * colsz * bs, st));
+  TTF_HIP(hipMemsetAsync(AP, 0, sizeof(double) * colsz * bs, st));
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    tsgram(N, bs, bs, X, ld, X, ld, Gm, bs, st);
+    hipLaunchKernelGGL(svqb_kernel, dim3(1), dim3(256), 0, st, bs, Gm, Tm, maskW);
+    mul_small(N, bs, bs, X, ld, Tm, bs, Wt, ld, 0.0, st);
+    copy_cols(Wt, ld, X, ld, N, bs, st);
+    std::vector<int> hm(bs);
+    TTF_HIP(hipMemcpyAsync(hm.data(), maskW, sizeof(int) * bs, hipMemcpyDeviceToHost, st));
+    TTF_HIP(hipStreamSynchronize(st));
+    int bad = -1;
+    for (int i = 0; i < bs; ++i) if (!hm[i]) { bad = i; break; }
+    if (bad < 0) break;
+    // a rank-deficient start: replace the dropped directions by random ones
+    for (int i = 0; i < bs; ++i)
+      if (!hm[i]) {
+        hipLaunchKernelGGL(random_cols, dim3(grid_for(N)), dim3(256), 0, st, N, i, i + 1, X, ld,
+                           777u * (attempt + 1) + i);
+      }
+  }
+  op.apply(bs, X, ld, AX, ld, st);
+  int matvecs = bs;
+  {
+    std::vector<int> ones(kMaxM, 1);
+    TTF_HIP(hipMemcpyAsync(imask, ones.data(), sizeof(int) * kMaxM, hipMemcpyHostToDevice, st));
+    tsgram(N, bs, bs, X, ld, AX, ld, Hm, bs, st);
+    hipLaunchKernelGGL(rr_kernel, dim3(1), dim3(256), 0, st, bs, bs, bs, Hm, imask, theta, CZ, maskP,
+                       amax);
+    mul_small(N, bs, bs, X, ld, CZ, bs, Sn, ld, 0.0, st);
+    mul_small(N, bs, bs, AX, ld, CZ, bs, ASn, ld, 0.0, st);
+    copy_cols(Sn, ld, X, ld, N, bs, st);
+    copy_cols(ASn, ld, AX, ld, N, bs, st);
+  }
+  bool have_p = false;
+  double anorm = 0.0;
+  std::vector<double> hbuf(2 * kMaxM + 1);
+  for (int it = 0;; ++it) {
+    hipLaunchKernelGGL(residual_kernel, dim3(grid_for((int64_t)N * bs)), dim3(256), 0, st, N, bs, X,
+                       AX, ld, theta, W);
+    hipLaunchKernelGGL(colnorms_kernel, dim3(nev), dim3(256), 0, st, N, W, ld, nrm);
+    TTF_HIP(hipMemcpyAsync(hbuf.data(), theta, sizeof(double) * (2 * kMaxM + 1), hipMemcpyDeviceToHost, st));
+    TTF_HIP(hipStreamSynchronize(st));
+    anorm = std::max(anorm, hbuf[2 * kMaxM]);   // amax follows nrm in the buffer
+    double res2 = 0.0, rmax = 0.0;
+    for (int i = 0; i < nev; ++i) {
+      res2 += hbuf[kMaxM + i] * hbuf[kMaxM + i];
+      rmax = std::max(rmax, hbuf[kMaxM + i]);
+    }
+    stats.iters = it;
+    stats.res = std::sqrt(res2);
+    if (rmax <= tol * std::max(anorm, 1e-300) || matvecs + bs > max_matvecs) break;
+    // W <- W - [X P] ([X P]^T W), twice
+    const int nb = have_p ? 3 : 2;   // basis blocks after this step: S = [X W (P)]
+    for (int pass = 0; pass < 2; ++pass) {
+      tsgram(N, bs, bs, X, ld, W, ld, Gm, bs, st);
+      GemmOp<double> g = colmajor<double>('N', 'N', N, bs, bs, -1.0, X, ld, Gm, bs, 1.0, W, ld);
+      gemm_launch(g, st);
+      if (have_p) {
+        tsgram(N, bs, bs, P, ld, W, ld, Gm, bs, st);
+        GemmOp<double> h = colmajor<double>('N', 'N', N, bs, bs, -1.0, P, ld, Gm, bs, 1.0, W, ld);
+        gemm_launch(h, st);
+      }
+    }
+    // SVQB on W
+    tsgram(N, bs, bs, W, ld, W, ld, Gm, bs, st);
+    hipLaunchKernelGGL(svqb_kernel, dim3(1), dim3(256), 0, st, bs, Gm, Tm, maskW);
+    mul_small(N, bs, bs, W, ld, Tm, bs, Wt, ld, 0.0, st);
+    copy_cols(Wt, ld, W, ld, N, bs, st);
+    op.apply(bs, W, ld, AW, ld, st);
+    matvecs += bs;
+    // validity mask of S = [X W P]
+    TTF_HIP(hipMemcpyAsync(imask + bs, maskW, sizeof(int) * bs, hipMemcpyDeviceToDevice, st));
+    if (have_p) TTF_HIP(hipMemcpyAsync(imask + 2 * bs, maskP, sizeof(int) * bs, hipMemcpyDeviceToDevice, st));
+    const int m = nb * bs;
+    tsgram(N, m, m, S, ld, AS, ld, Hm, m, st);
+    hipLaunchKernelGGL(rr_kernel, dim3(1), dim3(256), 0, st, m, bs, bs, Hm, imask, theta, CZ, maskP,
+                       amax);
+    mul_small(N, m, 2 * bs, S, ld, CZ, m, Sn, ld, 0.0, st);
+    mul_small(N, m, 2 * bs, AS, ld, CZ, m, ASn, ld, 0.0, st);
+    copy_cols(Sn, ld, X, ld, N, bs, st);
+    copy_cols(Sn + colsz * bs, ld, P, ld, N, bs, st);
+    copy_cols(ASn, ld, AX, ld, N, bs, st);
+    copy_cols(ASn + colsz * bs, ld, AP, ld, N, bs, st);
+    have_p = true;
+  }
+  copy_cols(X, ld, Xout, N, N, nev, st);
+  for (int i = 0; i < nev; ++i) lambda[i] = hbuf[i];
+  stats.matvecs = matvecs;
+  stats.anorm = anorm;
+  return stats;
+}
+
+}  // namespace ttf

@@ -1,0 +1,38 @@
+// Device-resident local solvers used by the sweep drivers.
+#pragma once
+
+#include "contract.h"
+#include "linalg.h"
+
+namespace ttf {
+
+// The projected operator of one local step (device pointers).
+struct LocalOp {
+  Shape s;
+  const double* phi1;
+  const double* A;
+  const double* phi2;
+  int N() const { return s.ry1 * s.n * s.ry2; }
+  void apply(int nvec, const double* x, int64_t ldx, double* y, int64_t ldy, hipStream_t st) const {
+    bfun3_chain<double>(s, nvec, phi1, A, phi2, x, ldx, y, ldy, st);
+  }
+};
+
+struct EigStats {
+  int iters = 0;
+  int matvecs = 0;
+  double res = 0.0;      // 2-norm of the residual norms of the wanted pairs
+  double anorm = 0.0;    // operator norm estimate (largest |Ritz value|)
+  int dense = 0;         // 1 if the dense path was used
+};
+
+// Lowest `nev` eigenpairs of the symmetric local operator.  X (N x nev, ld N,
+// device) holds the initial guess on entry and the eigenvectors on exit;
+// lambda (host, nev) receives the eigenvalues in ascending order.
+// Stopping rule: all residual norms <= tol * anorm (the rule of the
+// reference's PRIMME call, eps = eps2/resid_damp), or max_matvecs reached.
+// `guard` extra block vectors accelerate convergence of the wanted pairs.
+EigStats block_eig(const LocalOp& op, int nev, int guard, double* X, double* lambda, double tol,
+                   int max_matvecs, int dense_max, hipStream_t st);
+
+}  // namespace ttf

@@ -70,7 +70,12 @@ enum Arena {
   kArenaSolver,      // local solver bases (LOBPCG / GMRES / Krylov)
   kArenaSmall,       // small dense matrices and scalars
   kArenaTrunc,       // SVD / QR workspaces
+  kArenaTrunc2,
+  kArenaTrunc3,
+  kArenaGram,        // partial sums of tall-skinny Gram products
   kArenaSplitK,      // split-K partial sums of the GEMM engine
+  kArenaSolver2,
+  kArenaDriver,      // sweep-driver staging (truncation factors)
   kArenaCount
 };
 void* arena(Arena a, size_t bytes);

@@ -81,6 +81,25 @@ for _nm in ("ttf_dphi2_left", "ttf_dphi2_right"):
     _sig(_nm, _c_int, *([_c_int] * 5 + [_vp, _vp, _vp, _vp]))
 _sig("ttf_d2d_fullmat", _c_int, *([_c_int] * 5 + [_vp, _vp, _vp, _vp]))
 _sig("dbfun3", None, *([_vp] * 15))
+_sig("ttf_dgesvd", _c_int, _c_int, _c_int, _vp, _c_i64, _vp, _c_i64, _vp, _vp, _c_i64,
+     C.POINTER(_c_int))
+_sig("ttf_dsyev_small", _c_int, _c_int, _c_int, _vp, _vp, _vp, _c_int)
+_sig("ttf_dgram", _c_int, _c_int, _c_int, _c_int, _vp, _c_i64, _vp, _c_i64, _vp, _c_int)
+
+
+class EigbStats(C.Structure):
+    _fields_ = [("sweeps", C.c_int), ("total_matvecs", C.c_long), ("last_ermax", C.c_double),
+                ("max_rank", C.c_int), ("seconds", C.c_double), ("solve_seconds", C.c_double),
+                ("trunc_seconds", C.c_double), ("phi_seconds", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_IP = C.POINTER(C.c_int)
+_DP = C.POINTER(C.c_double)
+_sig("ttf_tt_eigb", _c_int, _c_int, _IP, _IP, _IP, _DP, _DP, _IP, C.c_double, _c_int, _DP, _c_int,
+     _c_int, _c_int, _c_int, _c_int, _DP, _c_i64, C.POINTER(_c_i64), C.POINTER(EigbStats))
 
 
 def _declare(lib):
@@ -261,3 +280,71 @@ def ref_dbfun3(shape, phi1, A, phi2, x, kind="i4"):
         lib.ttf_clear_error()
         raise TTFError(msg)
     return y
+
+
+def _ip(a):
+    return a.ctypes.data_as(_IP)
+
+
+def _dp(a):
+    return a.ctypes.data_as(_DP)
+
+
+def gesvd(A):
+    """Thin SVD of a host matrix through ttf_dgesvd: returns (U, s, VT, sweeps)."""
+    lib = load()
+    A = np.asfortranarray(A, dtype=np.float64)
+    m, n = A.shape
+    k = min(m, n)
+    dA = DeviceArray.from_numpy(A)
+    dU, dS, dVT = DeviceArray((m, k)), DeviceArray((k,)), DeviceArray((k, n))
+    sw = C.c_int()
+    check(lib.ttf_dgesvd(m, n, dA.ptr, m, dU.ptr, m, dS.ptr, dVT.ptr, k, C.byref(sw)), lib)
+    return dU.numpy(), dS.numpy(), dVT.numpy(), sw.value
+
+
+def syev_small(As, desc=False):
+    """Eigen-decomposition of a stack of symmetric matrices (count, n, n)."""
+    lib = load()
+    As = np.asarray(As, dtype=np.float64)
+    cnt, n, _ = As.shape
+    flat = np.concatenate([np.asfortranarray(a).reshape(-1, order="F") for a in As])
+    dA = DeviceArray.from_numpy(flat)
+    dw, dQ = DeviceArray((cnt * n,)), DeviceArray((cnt * n * n,))
+    check(lib.ttf_dsyev_small(n, cnt, dA.ptr, dw.ptr, dQ.ptr, 1 if desc else 0), lib)
+    w = dw.numpy().reshape(cnt, n)
+    Q = dQ.numpy().reshape(cnt, n * n)
+    return w, np.stack([q.reshape(n, n, order="F") for q in Q])
+
+
+def gram(X, Y):
+    lib = load()
+    X = np.asfortranarray(X, dtype=np.float64)
+    Y = np.asfortranarray(Y, dtype=np.float64)
+    N, p = X.shape
+    q = Y.shape[1]
+    dX, dY, dG = DeviceArray.from_numpy(X), DeviceArray.from_numpy(Y), DeviceArray((p, q))
+    check(lib.ttf_dgram(N, p, q, dX.ptr, N, dY.ptr, N, dG.ptr, p), lib)
+    return dG.numpy()
+
+
+def tt_eigb(n, ra, crA, crY0, ry, eps, rmax, B, nswp=20, max_full_size=2000, verb=0, kickrank=5):
+    """Run ttf_tt_eigb with the reference's argument layout (host arrays).
+
+    Returns (lambda, result cores flat, ry out, stats dict)."""
+    lib = load()
+    d = len(n)
+    n_ = np.ascontiguousarray(n, dtype=np.int32)
+    ra_ = np.ascontiguousarray(ra, dtype=np.int32)
+    ry_ = np.ascontiguousarray(ry, dtype=np.int32).copy()
+    crA_ = np.ascontiguousarray(crA, dtype=np.float64)
+    crY_ = np.ascontiguousarray(crY0, dtype=np.float64)
+    lam = np.zeros(B)
+    cap = int(sum(max(rmax, max(ry)) ** 2 * int(nn) for nn in n) * B + 16)
+    res = np.zeros(cap)
+    size = C.c_int64()
+    st = EigbStats()
+    check(lib.ttf_tt_eigb(d, _ip(n_), _ip(n_), _ip(ra_), _dp(crA_), _dp(crY_), _ip(ry_), eps, rmax,
+                          _dp(lam), B, kickrank, nswp, max_full_size, verb, _dp(res), cap,
+                          C.byref(size), C.byref(st)), lib)
+    return lam, res[:size.value].copy(), ry_, st.as_dict()
