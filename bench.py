@@ -150,12 +150,88 @@ def run_matvec(args):
     return out
 
 
+def dominant_kernel_roofline(lib, shape, nvec, iters=50):
+    """Per-stage HIP-event timings of the bfun3 contraction on the library
+    stream; the roofline reports the slower of the two large MFMA stages."""
+    import ctypes
+    import ttfort
+    rx1, m, rx2, ry1, n, ry2, ra1, ra2 = shape
+    N = rx1 * m * rx2
+    rng = np.random.default_rng(3)
+    d = dict(phi1=ttfort.DeviceArray.from_numpy(rng.standard_normal(ry1 * rx1 * ra1)),
+             A=ttfort.DeviceArray.from_numpy(rng.standard_normal(ra1 * n * m * ra2)),
+             phi2=ttfort.DeviceArray.from_numpy(rng.standard_normal(rx2 * ra2 * ry2)),
+             x=ttfort.DeviceArray.from_numpy(rng.standard_normal(N * nvec)),
+             y=ttfort.DeviceArray((N * nvec,)))
+    st = (ctypes.c_float * 3)()
+    ttfort.check(lib.ttf_dbfun3_profile(*shape, nvec, d["phi1"].ptr, d["A"].ptr, d["phi2"].ptr,
+                                        d["x"].ptr, N, d["y"].ptr, N, 5, st), lib)
+    ttfort.check(lib.ttf_dbfun3_profile(*shape, nvec, d["phi1"].ptr, d["A"].ptr, d["phi2"].ptr,
+                                        d["x"].ptr, N, d["y"].ptr, N, iters, st), lib)
+    stage_ms = [float(v) for v in st]
+    fl = bfun3_stage_flops(*shape)
+    dom = 0 if stage_ms[0] >= stage_ms[2] else 2
+    achieved = fl[dom] * nvec / (stage_ms[dom] * 1e-3) / 1e12
+    return dict(bound="mfma", achieved=round(achieved, 3), peak=MI355X_PEAK_F64_TFLOPS,
+                unit="TFLOP/s", frac=round(achieved / MI355X_PEAK_F64_TFLOPS, 4), traffic=None,
+                kernel="gemm_mfma_kernel<double,...> of bfun3 stage %d at (r,n,r,R)=(%d,%d,%d,%d), "
+                       "block %d" % (dom + 1, rx1, m, rx2, ra2, nvec),
+                stage_ms=[round(v, 5) for v in stage_ms],
+                stage_tflops=[round(fl[k] * nvec / (stage_ms[k] * 1e-3) / 1e12, 3) for k in range(3)])
+
+
+def run_eigb(args):
+    """Heisenberg XXX d=64, 5 lowest eigenpairs, rmax 128, tol 1e-10, rank-8
+    N(0,1) start: W warm-up sweeps then K timed sweeps of one tt_eigb run."""
+    import ttfort
+    import mpo
+    lib = ttfort.load()
+    ttfort.check(lib.ttf_init(0), lib)
+    d, B, rmax, tol = args.eigb_d, 5, args.eigb_rmax, 1e-10
+    crA, ra, n = mpo.xxz(d)
+    crY, ry = mpo.random_block_tt(d, n, 8, B, seed=0)
+    nsw = args.warmup + args.steps
+    t0 = time.perf_counter()
+    lam, res, ry_out, st = ttfort.tt_eigb(n, ra, crA, crY, ry, tol, rmax, B, nswp=nsw, verb=args.verb)
+    wall = time.perf_counter() - t0
+    secs = st["sweep_seconds"]
+    flops = st["sweep_flops"]
+    timed = list(range(min(args.warmup, len(secs)), len(secs)))
+    if not timed:
+        timed = list(range(len(secs)))
+    t_timed = sum(secs[i] for i in timed)
+    f_timed = sum(flops[i] for i in timed)
+    value = f_timed / t_timed / 1e12
+    roof = dominant_kernel_roofline(lib, (rmax, 2, rmax, rmax, 2, rmax, 5, 5), 8)
+    out = dict(metric="sweep time + local-matvec TFLOP/s (FP64 tt_eigb sweep, Heisenberg XXX d=%d)" % d,
+               value=round(value, 4), unit="TFLOP/s", n_gpus=1, steps=len(timed), warmup=args.warmup,
+               ms_per_step=round(1e3 * t_timed / len(timed), 2), higher_is_better=True, scaling="weak",
+               vs_baseline=None, dtype="f64", data="synthetic",
+               config=dict(workload="tt_eigb Heisenberg XXX d=%d B=5 rmax=%d tol=1e-10 rank-8 start"
+                           % (d, rmax), d=d, B=B, rmax=rmax, tol=tol,
+                           sweep_seconds=[round(s, 3) for s in secs],
+                           sweep_tflops=[round(f / s / 1e12, 3) for f, s in zip(flops, secs)],
+                           matvecs=st["total_matvecs"], max_rank=st["max_rank"],
+                           lambda_=[float("%.12f" % v) for v in lam], last_ermax=st["last_ermax"],
+                           solve_s=round(st["solve_seconds"], 3), trunc_s=round(st["trunc_seconds"], 3),
+                           phi_s=round(st["phi_seconds"], 3), wall_s=round(wall, 2)),
+               roofline=roof)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_matvec((rmax, 2, rmax, rmax, 2, rmax, 5, 5), B,
+                                                  budget_s=args.cpu_budget)
+        out["cpu_baseline"]["sample"] += " (the oracle's local matvec at the sweep's largest local shape)"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="matvec", choices=["matvec"])
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--workload", default="eigb", choices=["eigb", "matvec"])
+    ap.add_argument("--eigb-d", type=int, default=64)
+    ap.add_argument("--eigb-rmax", type=int, default=128)
+    ap.add_argument("--verb", type=int, default=1, help="per-step progress lines on stderr")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     args = ap.parse_args()
@@ -163,7 +239,11 @@ def main():
         print("bench.py: this build measures one GPU only (the local-solver path has no "
               "multi-GPU mode); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    out = {"matvec": run_matvec}[args.workload](args)
+    if args.steps is None:
+        args.steps = 2 if args.workload == "eigb" else 200
+    if args.warmup is None:
+        args.warmup = 2 if args.workload == "eigb" else 20
+    out = {"matvec": run_matvec, "eigb": run_eigb}[args.workload](args)
     print(json.dumps(out), flush=True)
 
 

@@ -172,6 +172,10 @@ typedef struct {
   double solve_seconds;  /* local eigensolves */
   double trunc_seconds;  /* SVD truncations and core moves */
   double phi_seconds;    /* interface updates */
+  int nrec;              /* sweeps recorded below (at most 64) */
+  double sweep_seconds[64];   /* wall time of each sweep */
+  double sweep_flops[64];     /* algorithmic local-matvec flops of each sweep */
+  long sweep_matvecs[64];     /* local matvecs (vectors) of each sweep */
 } ttf_eigb_stats;
 
 /* Block eigensolver in TT format: the B lowest eigenpairs of the symmetric

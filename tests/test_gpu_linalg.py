@@ -32,15 +32,17 @@ def test_gesvd_random(ttlib, m, n):
     A = rng.standard_normal((m, n))
     U, s, VT, sw = ttfort.gesvd(A)
     _check_svd(A, U, s, VT)
+    assert sw <= 15, sw
 
 
 def test_gesvd_rank_deficient_and_zero(ttlib):
     import ttfort
     rng = np.random.default_rng(5)
     A = rng.standard_normal((100, 10)) @ rng.standard_normal((10, 50))
-    U, s, VT, _ = ttfort.gesvd(A)
+    U, s, VT, sw = ttfort.gesvd(A)
     _check_svd(A, U, s, VT)
     assert s[10] < 1e-12 * s[0]
+    assert sw <= 15, sw
     Z = np.zeros((20, 6))
     U, s, VT, _ = ttfort.gesvd(Z)
     assert np.all(s == 0)

@@ -167,6 +167,11 @@ extern "C" int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, con
     long total_mv = 0;
     std::vector<double> lam(B, 0.0), hs;
     int last_swp = 0;
+    std::vector<double> rec_sec, rec_flops;
+    std::vector<long> rec_mv;
+    Timer tsweep;
+    double sweep_flops = 0.0;
+    long sweep_mv = 0;
     while (swp <= nswp0) {
       const int N = ry[k] * n[k] * ry[k + 1];
       LocalOp op{Shape{ry[k], m[k], ry[k + 1], ry[k], n[k], ry[k + 1], ra[k], ra[k + 1]}, phi[k],
@@ -181,9 +186,17 @@ extern "C" int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, con
       const double erloc = (fvold - fv) / std::fabs(fv);
       ermax = std::max(ermax, erloc);
       if (verb > 0)
-        std::printf("swp: %d i: [%d/%d] loc_size: %d matvecs: %d res: %.3e ermax: %.3e dfv: %.3e\n", swp,
+        std::fprintf(stderr, "swp: %d i: [%d/%d] loc_size: %d matvecs: %d res: %.3e ermax: %.3e dfv: %.3e\n", swp,
                     k + 1, d, N, es.matvecs, es.res, ermax, fvold - fv);
       total_mv += es.matvecs;
+      sweep_mv += es.matvecs;
+      {
+        const Shape& s = op.s;
+        const double f = 2.0 * s.rx1 * s.m * s.rx2 * s.ra2 * s.ry2 +
+                         2.0 * s.ra1 * s.n * s.m * s.ra2 * s.ry2 * s.rx1 +
+                         2.0 * s.ry1 * s.n * s.ry2 * s.rx1 * s.ra1;
+        sweep_flops += f * es.matvecs;
+      }
       fvold = fv;
       if (dir < 0 && k > 0) {
         // (tt_eigb.f90:305-355) M = X^T viewed as (B*ry_k) x (n_k*ry_{k+1})
@@ -191,7 +204,7 @@ extern "C" int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, con
         const int mr = B * ry[k], nc = n[k] * ry[k + 1];
         const int kk = std::min(mr, nc);
         transpose(N, B, core[k], N, tmp, B, st);
-        gesvd(mr, nc, tmp, mr, U, mr, S, VT, kk, st);
+        gesvd(mr, nc, tmp, mr, U, mr, S, VT, kk, st, /*complete=*/0);
         hs.assign(kk, 0.0);
         TTF_HIP(hipMemcpyAsync(hs.data(), S, sizeof(double) * kk, hipMemcpyDeviceToHost, st));
         TTF_HIP(hipStreamSynchronize(st));
@@ -223,7 +236,7 @@ extern "C" int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, con
         Timer tv;
         const int mr = ry[k] * n[k], nc = ry[k + 1] * B;
         const int kk = std::min(mr, nc);
-        gesvd(mr, nc, core[k], mr, U, mr, S, VT, kk, st);
+        gesvd(mr, nc, core[k], mr, U, mr, S, VT, kk, st, /*complete=*/0);
         hs.assign(kk, 0.0);
         TTF_HIP(hipMemcpyAsync(hs.data(), S, sizeof(double) * kk, hipMemcpyDeviceToHost, st));
         TTF_HIP(hipStreamSynchronize(st));
@@ -255,10 +268,17 @@ extern "C" int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, con
         if (verb > 0) {
           int mx = 0;
           for (int q = 0; q < d; ++q) mx = std::max(mx, ry[q]);
-          std::printf("swp: %d er = %.3e rmax:%d\n", swp, ermax, mx);
+          std::fprintf(stderr, "swp: %d er = %.3e rmax:%d\n", swp, ermax, mx);
         }
         last_swp = swp;
         ++swp;
+        TTF_HIP(hipStreamSynchronize(st));
+        rec_sec.push_back(tsweep.ms() * 1e-3);
+        rec_flops.push_back(sweep_flops);
+        rec_mv.push_back(sweep_mv);
+        tsweep = Timer();
+        sweep_flops = 0.0;
+        sweep_mv = 0;
         if (ermax < eps) break;
         ermax = 0.0;
       } else if (dir < 0 && k == 1) {
@@ -302,6 +322,12 @@ extern "C" int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, con
       stats->solve_seconds = t_solve * 1e-3;
       stats->trunc_seconds = t_svd * 1e-3;
       stats->phi_seconds = t_phi * 1e-3;
+      stats->nrec = (int)std::min<size_t>(rec_sec.size(), 64);
+      for (int q = 0; q < stats->nrec; ++q) {
+        stats->sweep_seconds[q] = rec_sec[q];
+        stats->sweep_flops[q] = rec_flops[q];
+        stats->sweep_matvecs[q] = rec_mv[q];
+      }
     }
   });
 }

@@ -21,11 +21,13 @@ void syev_small(int n, int count, double* A, double* w, double* Q, int desc, hip
 // Thin SVD of an m x n column-major matrix A (lda): A = U diag(S) VT with
 // k = min(m, n) values in descending order; U (m x k, ldu), VT (k x n, ldvt).
 // Blocked one-sided Jacobi (pair-Gram eigenvectors as block rotations),
-// singular values from the final column norms; columns belonging to zero
-// singular values are completed to an orthonormal set.
+// singular values from the final column norms.  Singular vectors belonging
+// to zero singular values (norm at the rounding floor mw*eps*||A||_F) are
+// completed to an orthonormal set when `complete` is set, else left zero
+// (enough whenever they are only used multiplied by their singular value).
 // Returns the number of Jacobi sweeps used.
 int gesvd(int m, int n, const double* A, int64_t lda, double* U, int64_t ldu, double* S,
-          double* VT, int64_t ldvt, hipStream_t st);
+          double* VT, int64_t ldvt, hipStream_t st, int complete = 1);
 
 // B (n x m) = A (m x n)^T, both column-major with leading dims.
 void transpose(int m, int n, const double* A, int64_t lda, double* B, int64_t ldb, hipStream_t st);

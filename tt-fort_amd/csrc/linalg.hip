@@ -101,7 +101,7 @@ __device__ void jacobi_lds(double (*A)[LD], double (*V)[LD], int n, int max_swee
         double c = 1.0, s = 0.0;
         if (q < n) {
           double apq = A[p][q], app = A[p][p], aqq = A[q][q];
-          if (fabs(apq) > 1e-300 && fabs(apq) > 1e-17 * sqrt(fabs(app) * fabs(aqq))) {
+          if (fabs(apq) > 1e-300 && fabs(apq) > 2.2e-16 * sqrt(fabs(app) * fabs(aqq))) {
             double tau = (aqq - app) / (2.0 * apq);
             double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
             c = 1.0 / sqrt(1.0 + t * t);
@@ -208,7 +208,7 @@ __device__ inline void circle_pair(int r, int k, int nn, int* p, int* q) {
 
 __global__ __launch_bounds__(256) void svd_round_kernel(int mw, int nw, int bsz, int nb, int round,
                                                         double* __restrict__ W,
-                                                        double* __restrict__ V,
+                                                        double* __restrict__ V, double small2,
                                                         unsigned long long* offmax) {
   extern __shared__ double cols_lds[];   // [2b][mw]
   __shared__ double Qp[kSvdMaxP][kSvdMaxP + 1];
@@ -262,7 +262,8 @@ __global__ __launch_bounds__(256) void svd_round_kernel(int mw, int nw, int bsz,
             be += __shfl_xor(be, o, 16);
             ga += __shfl_xor(ga, o, 16);
           }
-          double den = sqrt(al * be);
+          // columns below the noise floor are numerically zero: never rotated
+          double den = (al > small2 && be > small2) ? sqrt(al * be) : 0.0;
           double rel = den > 0.0 ? fabs(ga) / den : 0.0;
           if (isweep == 0 && p < nf && q >= nf && l16 == 0)
             atomicMax(&local_off, (unsigned long long)__double_as_longlong(rel));
@@ -378,9 +379,12 @@ __global__ void gather_scale_cols(int m, int k, const double* __restrict__ W, in
 // Orthonormal completion of columns [k0, k) of U (m x k, ld): each new
 // column starts from a deterministic pseudo-random vector and is
 // orthogonalised twice against all previous columns.  One work-group.
+// Classical Gram-Schmidt applied twice, with all projection coefficients of
+// a pass computed in parallel (one thread per previous column).
 __global__ __launch_bounds__(256) void complete_basis(int m, int k0, int k, double* U, int64_t ldu) {
   __shared__ double red[256];
   __shared__ double coef;
+  __shared__ double cj[4096];
   for (int c = k0; c < k; ++c) {
     double* u = U + ldu * c;
     for (int i = threadIdx.x; i < m; i += blockDim.x) {
@@ -390,21 +394,19 @@ __global__ __launch_bounds__(256) void complete_basis(int m, int k0, int k, doub
     }
     __syncthreads();
     for (int pass = 0; pass < 2; ++pass) {
-      for (int j = 0; j < c; ++j) {
+      for (int j = threadIdx.x; j < c; j += blockDim.x) {
         const double* v = U + ldu * j;
         double s = 0.0;
-        for (int i = threadIdx.x; i < m; i += blockDim.x) s += v[i] * u[i];
-        red[threadIdx.x] = s;
-        __syncthreads();
-        for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-          if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-          __syncthreads();
-        }
-        if (threadIdx.x == 0) coef = red[0];
-        __syncthreads();
-        for (int i = threadIdx.x; i < m; i += blockDim.x) u[i] -= coef * v[i];
-        __syncthreads();
+        for (int i = 0; i < m; ++i) s += v[i] * u[i];
+        cj[j] = s;
       }
+      __syncthreads();
+      for (int i = threadIdx.x; i < m; i += blockDim.x) {
+        double s = 0.0;
+        for (int j = 0; j < c; ++j) s += cj[j] * U[i + ldu * j];
+        u[i] -= s;
+      }
+      __syncthreads();
     }
     double s = 0.0;
     for (int i = threadIdx.x; i < m; i += blockDim.x) s += u[i] * u[i];
@@ -422,7 +424,7 @@ __global__ __launch_bounds__(256) void complete_basis(int m, int k0, int k, doub
 }
 
 int gesvd(int m, int n, const double* A, int64_t lda, double* U, int64_t ldu, double* S,
-          double* VT, int64_t ldvt, hipStream_t st) {
+          double* VT, int64_t ldvt, hipStream_t st, int complete) {
   TTF_REQUIRE(m >= 1 && n >= 1, "gesvd: invalid sizes %d x %d", m, n);
   const bool tall = m >= n;
   const int mw = tall ? m : n, nw = tall ? n : m;   // work matrix mw x nw
@@ -455,9 +457,12 @@ int gesvd(int m, int n, const double* A, int64_t lda, double* U, int64_t ldu, do
   double fro2 = 0.0;
   for (double v : hn) fro2 += v * v;
   int sweeps = 0;
+  double small2 = 0.0;
   if (fro2 > 0.0) {
     const double eps = 2.220446049250313e-16;
     const double tol = 4.0 * eps * std::sqrt((double)mw);
+    // squared column norms below (sqrt(mw) eps ||A||_F)^2 are rounding noise
+    small2 = mw * eps * eps * fro2;
     // block size: the pair's 2b columns must fit the LDS budget
     int two_b = std::min<int>(kSvdMaxP, kSvdLdsDoubles / mw);
     two_b = std::max(2, two_b - (two_b & 1));
@@ -479,7 +484,7 @@ int gesvd(int m, int n, const double* A, int64_t lda, double* U, int64_t ldu, do
       TTF_HIP(hipMemsetAsync(offmax, 0, sizeof(unsigned long long), st));
       for (int r = 0; r < nb - 1; ++r) {
         hipLaunchKernelGGL(svd_round_kernel, dim3(nb / 2), dim3(256), lds, st, mw, nw, bsz, nb, r, W,
-                           V, offmax);
+                           V, small2, offmax);
         TTF_HIP(hipGetLastError());
       }
       TTF_HIP(hipMemcpyAsync(&hoff, offmax, sizeof(hoff), hipMemcpyDeviceToHost, st));
@@ -495,9 +500,8 @@ int gesvd(int m, int n, const double* A, int64_t lda, double* U, int64_t ldu, do
   std::vector<int> order(nw);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return hn[a] > hn[b]; });
-  const double smax = hn[order[0]];
-  // columns whose norm is pure rounding noise get an orthonormal completion
-  const double tiny = std::max(smax * 2.220446049250313e-16, 1e-300);
+  // columns at the rounding-noise floor count as zero singular values
+  const double tiny = std::max(std::sqrt(small2), 1e-300);
   std::vector<double> hs(nw), hscale(nw), hone(nw, 1.0);
   int kpos = 0;
   for (int c = 0; c < nw; ++c) {
@@ -516,7 +520,7 @@ int gesvd(int m, int n, const double* A, int64_t lda, double* U, int64_t ldu, do
   hipLaunchKernelGGL(gather_scale_cols, dim3(blocks), dim3(256), 0, st, mw, k, W, (int64_t)mw, perm,
                      scale, Uw, lduw);
   TTF_HIP(hipGetLastError());
-  if (kpos < k) {
+  if (kpos < k && complete) {
     hipLaunchKernelGGL(complete_basis, dim3(1), dim3(256), 0, st, mw, kpos, k, Uw, lduw);
     TTF_HIP(hipGetLastError());
   }

@@ -44,7 +44,7 @@ __device__ void jacobi_eig(double (*A)[LD], double (*V)[LD], int n, double* rc, 
         double c = 1.0, s = 0.0;
         if (q < n) {
           double apq = A[p][q], app = A[p][p], aqq = A[q][q];
-          if (fabs(apq) > 1e-300 && fabs(apq) > 1e-17 * sqrt(fabs(app) * fabs(aqq))) {
+          if (fabs(apq) > 1e-300 && fabs(apq) > 2.2e-16 * sqrt(fabs(app) * fabs(aqq))) {
             double tau = (aqq - app) / (2.0 * apq);
             double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
             c = 1.0 / sqrt(1.0 + t * t);
@@ -193,35 +193,25 @@ __global__ __launch_bounds__(256) void rr_kernel(int m, int bs, int nx, const do
     }
     __syncthreads();
   }
-  // modified Gram-Schmidt on the columns of Z (twice)
+  // modified Gram-Schmidt on the columns of Z (twice).  The kernel runs as a
+  // single wave with lane r owning row r (m <= 64): dot products are wave
+  // reductions through lane shuffles.
+  (void)red;
   for (int pass = 0; pass < 2; ++pass) {
     for (int c = 0; c < bs; ++c) {
       for (int j = 0; j < c; ++j) {
-        double s = 0.0;
-        for (int r = tid; r < m; r += blockDim.x) s += Z[r][j] * Z[r][c];
-        red[tid] = s;
-        __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
-          if (tid < o) red[tid] += red[tid + o];
-          __syncthreads();
-        }
-        double d = red[0];
-        __syncthreads();
-        for (int r = tid; r < m; r += blockDim.x) Z[r][c] -= d * Z[r][j];
+        double s = (tid < m) ? Z[tid][j] * Z[tid][c] : 0.0;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if (tid < m) Z[tid][c] -= s * Z[tid][j];
         __syncthreads();
       }
-      double s = 0.0;
-      for (int r = tid; r < m; r += blockDim.x) s += Z[r][c] * Z[r][c];
-      red[tid] = s;
-      __syncthreads();
-      for (int o = 128; o > 0; o >>= 1) {
-        if (tid < o) red[tid] += red[tid + o];
-        __syncthreads();
-      }
-      double nrm = sqrt(red[0]);
-      __syncthreads();
+      double s = (tid < m) ? Z[tid][c] * Z[tid][c] : 0.0;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+      double nrm = sqrt(s);
       double inv = nrm > 1e-10 ? 1.0 / nrm : 0.0;
-      for (int r = tid; r < m; r += blockDim.x) Z[r][c] *= inv;
+      if (tid < m) Z[tid][c] *= inv;
       if (tid == 0 && pass == 1) maskP[c] = nrm > 1e-10 ? 1 : 0;
       __syncthreads();
     }
@@ -348,7 +338,7 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
   TTF_HIP(hipMemsetAsync(AP, 0, sizeof(double) * colsz * bs, st));
   for (int attempt = 0; attempt < 3; ++attempt) {
     tsgram(N, bs, bs, X, ld, X, ld, Gm, bs, st);
-    hipLaunchKernelGGL(svqb_kernel, dim3(1), dim3(256), 0, st, bs, Gm, Tm, maskW);
+    hipLaunchKernelGGL(svqb_kernel, dim3(1), dim3(64), 0, st, bs, Gm, Tm, maskW);
     mul_small(N, bs, bs, X, ld, Tm, bs, Wt, ld, 0.0, st);
     copy_cols(Wt, ld, X, ld, N, bs, st);
     std::vector<int> hm(bs);
@@ -370,7 +360,7 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
     std::vector<int> ones(kMaxM, 1);
     TTF_HIP(hipMemcpyAsync(imask, ones.data(), sizeof(int) * kMaxM, hipMemcpyHostToDevice, st));
     tsgram(N, bs, bs, X, ld, AX, ld, Hm, bs, st);
-    hipLaunchKernelGGL(rr_kernel, dim3(1), dim3(256), 0, st, bs, bs, bs, Hm, imask, theta, CZ, maskP,
+    hipLaunchKernelGGL(rr_kernel, dim3(1), dim3(64), 0, st, bs, bs, bs, Hm, imask, theta, CZ, maskP,
                        amax);
     mul_small(N, bs, bs, X, ld, CZ, bs, Sn, ld, 0.0, st);
     mul_small(N, bs, bs, AX, ld, CZ, bs, ASn, ld, 0.0, st);
@@ -409,7 +399,7 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
     }
     // SVQB on W
     tsgram(N, bs, bs, W, ld, W, ld, Gm, bs, st);
-    hipLaunchKernelGGL(svqb_kernel, dim3(1), dim3(256), 0, st, bs, Gm, Tm, maskW);
+    hipLaunchKernelGGL(svqb_kernel, dim3(1), dim3(64), 0, st, bs, Gm, Tm, maskW);
     mul_small(N, bs, bs, W, ld, Tm, bs, Wt, ld, 0.0, st);
     copy_cols(Wt, ld, W, ld, N, bs, st);
     op.apply(bs, W, ld, AW, ld, st);
@@ -419,7 +409,7 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
     if (have_p) TTF_HIP(hipMemcpyAsync(imask + 2 * bs, maskP, sizeof(int) * bs, hipMemcpyDeviceToDevice, st));
     const int m = nb * bs;
     tsgram(N, m, m, S, ld, AS, ld, Hm, m, st);
-    hipLaunchKernelGGL(rr_kernel, dim3(1), dim3(256), 0, st, m, bs, bs, Hm, imask, theta, CZ, maskP,
+    hipLaunchKernelGGL(rr_kernel, dim3(1), dim3(64), 0, st, m, bs, bs, Hm, imask, theta, CZ, maskP,
                        amax);
     mul_small(N, m, 2 * bs, S, ld, CZ, m, Sn, ld, 0.0, st);
     mul_small(N, m, 2 * bs, AS, ld, CZ, m, ASn, ld, 0.0, st);

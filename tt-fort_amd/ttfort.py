@@ -90,10 +90,16 @@ _sig("ttf_dgram", _c_int, _c_int, _c_int, _c_int, _vp, _c_i64, _vp, _c_i64, _vp,
 class EigbStats(C.Structure):
     _fields_ = [("sweeps", C.c_int), ("total_matvecs", C.c_long), ("last_ermax", C.c_double),
                 ("max_rank", C.c_int), ("seconds", C.c_double), ("solve_seconds", C.c_double),
-                ("trunc_seconds", C.c_double), ("phi_seconds", C.c_double)]
+                ("trunc_seconds", C.c_double), ("phi_seconds", C.c_double), ("nrec", C.c_int),
+                ("sweep_seconds", C.c_double * 64), ("sweep_flops", C.c_double * 64),
+                ("sweep_matvecs", C.c_long * 64)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        out = {}
+        for k, _ in self._fields_:
+            v = getattr(self, k)
+            out[k] = list(v)[:self.nrec] if k.startswith("sweep_") else v
+        return out
 
 
 _IP = C.POINTER(C.c_int)
