@@ -153,6 +153,18 @@ int ttf_d2d_fullmat(int rx1, int m, int rx2, int ra1, int ra2, const double* phi
  * Synchronous. */
 int ttf_dgesvd(int m, int n, const double* A, int64_t lda, double* U, int64_t ldu, double* S,
                double* VT, int64_t ldvt, int* sweeps);
+/* Singular values (S, k = min(m,n), descending) of an m x n device matrix
+ * and the orthonormal singular vectors of one side only: side 'L' gives the
+ * left vectors (m x k), side 'R' the right vectors (n x k), as the columns
+ * of O (ldo).  The truncation steps need one side plus a product with A
+ * (tt_eigb.f90:312-333 keeps V^T and absorbs U S; :361-376 keeps U and
+ * absorbs S V^T).  When the wanted side is the shorter dimension the Jacobi
+ * iteration runs on R^T of a Householder QR of the tall orientation.
+ * Vectors of zero singular values are zero columns; *kpos (optional)
+ * returns how many are nonzero, *sweeps (optional) the Jacobi sweeps.
+ * Synchronous. */
+int ttf_dsvd_side(int m, int n, const double* A, int64_t lda, char side, double* S, double* O,
+                  int64_t ldo, int* kpos, int* sweeps);
 /* Eigen-decomposition of `count` symmetric n x n device matrices (n <= 64)
  * stored consecutively (A is overwritten): eigenvalues to w (ascending, or
  * descending if desc), eigenvectors to Q.  Replaces the small dsyev calls

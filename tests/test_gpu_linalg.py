@@ -90,3 +90,48 @@ def test_gram(ttlib):
         Y = rng.standard_normal((N, q))
         G = ttfort.gram(X, Y)
         assert np.allclose(G, X.T @ Y, rtol=1e-12, atol=1e-10)
+
+
+def _graded(rng, m, n, lo):
+    k = min(m, n)
+    Qa, _ = np.linalg.qr(rng.standard_normal((m, k)))
+    Qb, _ = np.linalg.qr(rng.standard_normal((n, k)))
+    return (Qa * np.logspace(0, lo, k)) @ Qb.T
+
+
+@pytest.mark.parametrize("m,n,kind", [(640, 256, "rand"), (256, 640, "rand"), (640, 256, "graded"),
+                                      (256, 640, "graded"), (200, 200, "graded"), (1000, 17, "graded"),
+                                      (17, 1000, "rand"), (33, 1, "rand"), (1, 7, "rand"),
+                                      (128, 320, "lowrank")])
+@pytest.mark.parametrize("side", ["L", "R"])
+def test_svd_side(ttlib, m, n, kind, side):
+    """One-sided SVD used by the truncations: singular values to LAPACK's
+    absolute accuracy (5e-13 * smax: thousands of rotations per column), orthonormal vectors for the nonzero
+    values, and the factorisation A = O (O^T A) (side L) or A = (A O) O^T
+    (side R) to 1e-12 relative; sweeps stay few for graded spectra (the
+    QR-preconditioned side)."""
+    import ttfort
+    rng = np.random.default_rng(m * 7 + n * 13 + (side == "R"))
+    if kind == "rand":
+        A = rng.standard_normal((m, n))
+    elif kind == "graded":
+        A = _graded(rng, m, n, -13)
+    else:
+        A = rng.standard_normal((m, 40)) @ rng.standard_normal((40, n))
+    s, O, kpos, sw = ttfort.svd_side(A, side)
+    k = min(m, n)
+    s_ref = np.linalg.svd(A, compute_uv=False)
+    assert np.all(np.diff(s) <= 0)
+    assert np.max(np.abs(s - s_ref)) <= 5e-13 * s_ref[0], np.max(np.abs(s - s_ref)) / s_ref[0]
+    if kind == "lowrank":
+        assert kpos <= 41 and kpos >= 40
+    else:
+        assert kpos == k
+    Ok = O[:, :kpos]
+    assert np.allclose(Ok.T @ Ok, np.eye(kpos), atol=1e-12)
+    assert np.all(O[:, kpos:] == 0)
+    rec = Ok @ (Ok.T @ A) if side == "L" else (A @ Ok) @ Ok.T
+    assert np.linalg.norm(A - rec) <= 1e-12 * np.linalg.norm(A)
+    short = (side == "L") == (m < n)
+    if short and kind == "graded":
+        assert sw <= 15, sw

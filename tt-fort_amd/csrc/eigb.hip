@@ -41,22 +41,6 @@ int my_chop3(const std::vector<double>& s, int n, double eps) {
 
 namespace {
 
-__global__ void scale_cols(int m, int k, double* U, int64_t ldu, const double* s) {
-  int64_t total = (int64_t)m * k;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * blockDim.x)
-    U[(e % m) + ldu * (e / m)] *= s[e / m];
-}
-
-__global__ void scale_rows(int k, int n, double* VT, int64_t ld, const double* s) {
-  int64_t total = (int64_t)k * n;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * blockDim.x)
-    VT[(e % k) + ld * (e / k)] *= s[e % k];
-}
-
-int blocks_for(int64_t n) { return (int)std::min<int64_t>(std::max<int64_t>(ceil_div(n, 256), 1), 8192); }
-
 struct Timer {
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   double ms() const {
@@ -67,15 +51,17 @@ struct Timer {
 }  // namespace
 
 // Orthogonal factorisation A (m x n, ld m) = Q R with Q (m x k) orthonormal,
-// R (k x n), k = min(m, n).  Computed through the thin SVD (Q = U,
-// R = diag(s) V^T); any orthonormal Q with Q R = A serves the sweep
-// (tt_eigb.f90:206-210 uses it only as such).
-void orth_factor(int m, int n, const double* A, double* Q, double* R, hipStream_t st) {
+// R = Q^T A (k x n), k = min(m, n).  Q is the left singular basis (completed
+// where A is rank deficient); any orthonormal Q with Q R = A serves the
+// sweep (tt_eigb.f90:206-210 uses it only as such).  S: k device doubles.
+void orth_factor(int m, int n, const double* A, double* Q, double* R, double* S, hipStream_t st) {
   const int k = std::min(m, n);
-  double* S = (double*)arena(kArenaDriver, sizeof(double) * (size_t)k);
-  gesvd(m, n, A, m, Q, m, S, R, k, st);
-  hipLaunchKernelGGL(scale_rows, dim3(blocks_for((int64_t)k * n)), dim3(256), 0, st, k, n, R, (int64_t)k, S);
-  TTF_HIP(hipGetLastError());
+  std::vector<double> hs;
+  int kpos = 0;
+  svd_vectors(m, n, A, m, true, S, Q, m, hs, &kpos, st);
+  complete_columns(m, kpos, k, Q, m, st);
+  GemmOp<double> g = colmajor<double>('T', 'N', k, n, m, 1.0, Q, m, A, m, 0.0, R, k);
+  gemm_launch(g, st);
 }
 
 }  // namespace ttf
@@ -146,7 +132,7 @@ extern "C" int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, con
     for (int k = 0; k < d; ++k) {
       const int mrow = ry[k] * n[k], ncol = ry[k + 1];
       const int rnew = std::min(mrow, ncol);
-      orth_factor(mrow, ncol, core[k], U, VT, st);   // U = Q, VT = R (rnew x ncol)
+      orth_factor(mrow, ncol, core[k], U, VT, S, st);   // U = Q, VT = R (rnew x ncol)
       TTF_HIP(hipMemcpyAsync(core[k], U, sizeof(double) * (size_t)mrow * rnew, hipMemcpyDeviceToDevice, st));
       if (k < d - 1) {
         const int ncol2 = n[k + 1] * ry[k + 2];
@@ -204,17 +190,18 @@ extern "C" int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, con
         const int mr = B * ry[k], nc = n[k] * ry[k + 1];
         const int kk = std::min(mr, nc);
         transpose(N, B, core[k], N, tmp, B, st);
-        gesvd(mr, nc, tmp, mr, U, mr, S, VT, kk, st, /*complete=*/0);
-        hs.assign(kk, 0.0);
-        TTF_HIP(hipMemcpyAsync(hs.data(), S, sizeof(double) * kk, hipMemcpyDeviceToHost, st));
-        TTF_HIP(hipStreamSynchronize(st));
+        // right singular vectors V (nc x kk) into VT's storage
+        int kpos = 0;
+        svd_vectors(mr, nc, tmp, mr, false, S, VT, nc, hs, &kpos, st);
         const int nn = std::min(my_chop3(hs, kk, eps2), rmax);
-        // core k = first nn rows of VT
-        TTF_HIP(hipMemcpy2DAsync(core[k], sizeof(double) * nn, VT, sizeof(double) * kk, sizeof(double) * nn, nc,
-                                 hipMemcpyDeviceToDevice, st));
-        // US = U(:, 0:nn) diag(s); core k-1 <- core(k-1) * US with (b, r_k, j) -> (r_k, j, b)
-        hipLaunchKernelGGL(scale_cols, dim3(blocks_for((int64_t)mr * nn)), dim3(256), 0, st, mr, nn, U,
-                           (int64_t)mr, S);
+        complete_columns(nc, kpos, nn, VT, nc, st);
+        // core k = V(:, 0:nn)^T; US = M V(:, 0:nn) (mr x nn)
+        transpose(nc, nn, VT, nc, core[k], nn, st);
+        {
+          GemmOp<double> g = colmajor<double>('N', 'N', mr, nn, nc, 1.0, tmp, mr, VT, nc, 0.0, U, mr);
+          gemm_launch(g, st);
+        }
+        // core k-1 <- core(k-1) * US with (b, r_k, j) -> (r_k, j, b)
         const int mp = ry[k - 1] * n[k - 1];
         GemmOp<double> g{};
         g.M = mp; g.N = nn; g.K = ry[k];
@@ -236,19 +223,22 @@ extern "C" int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, con
         Timer tv;
         const int mr = ry[k] * n[k], nc = ry[k + 1] * B;
         const int kk = std::min(mr, nc);
-        gesvd(mr, nc, core[k], mr, U, mr, S, VT, kk, st, /*complete=*/0);
-        hs.assign(kk, 0.0);
-        TTF_HIP(hipMemcpyAsync(hs.data(), S, sizeof(double) * kk, hipMemcpyDeviceToHost, st));
-        TTF_HIP(hipStreamSynchronize(st));
+        // left singular vectors U (mr x kk)
+        int kpos = 0;
+        svd_vectors(mr, nc, core[k], mr, true, S, U, mr, hs, &kpos, st);
         const int nn = std::min(my_chop3(hs, kk, eps2), rmax);
+        complete_columns(mr, kpos, nn, U, mr, st);
+        // SVT = U(:, 0:nn)^T M (nn x nc), layout (j, r_{k+1}, b)
+        {
+          GemmOp<double> g = colmajor<double>('T', 'N', nn, nc, mr, 1.0, U, mr, core[k], mr, 0.0, VT, nn);
+          gemm_launch(g, st);
+        }
         TTF_HIP(hipMemcpyAsync(core[k], U, sizeof(double) * (size_t)mr * nn, hipMemcpyDeviceToDevice, st));
-        // SVT = diag(s) VT(0:nn, :), layout (j, r_{k+1}, b); core k+1 <- (j, n, r, b)
-        hipLaunchKernelGGL(scale_rows, dim3(blocks_for((int64_t)kk * nc)), dim3(256), 0, st, kk, nc, VT,
-                           (int64_t)kk, S);
+        // core k+1 <- SVT * core(k+1): (j, n, r, b)
         const int ncol2 = n[k + 1] * ry[k + 2];
         GemmOp<double> g{};
         g.M = nn; g.N = ncol2; g.K = ry[k + 1];
-        g.A = VT; g.sAi = 1; g.sAk = kk; g.sAb0 = (int64_t)kk * ry[k + 1];
+        g.A = VT; g.sAi = 1; g.sAk = nn; g.sAb0 = (int64_t)nn * ry[k + 1];
         g.B = core[k + 1]; g.sBk = 1; g.sBj = ry[k + 1];
         g.C = tmp; g.sCi = 1; g.sCj = nn; g.sCb0 = (int64_t)nn * ncol2;
         g.nb0 = B; g.nbatch = B; g.alpha = 1.0; g.beta = 0.0; g.ksplit = 1;

@@ -5,6 +5,8 @@
 
 #include "ttf_common.h"
 
+#include <vector>
+
 namespace ttf {
 
 // G (p x q, leading dim ldg) = X^T Y for X (N x p, ldx), Y (N x q, ldy);
@@ -20,14 +22,29 @@ void syev_small(int n, int count, double* A, double* w, double* Q, int desc, hip
 
 // Thin SVD of an m x n column-major matrix A (lda): A = U diag(S) VT with
 // k = min(m, n) values in descending order; U (m x k, ldu), VT (k x n, ldvt).
-// Blocked one-sided Jacobi (pair-Gram eigenvectors as block rotations),
-// singular values from the final column norms.  Singular vectors belonging
-// to zero singular values (norm at the rounding floor mw*eps*||A||_F) are
-// completed to an orthonormal set when `complete` is set, else left zero
-// (enough whenever they are only used multiplied by their singular value).
-// Returns the number of Jacobi sweeps used.
+// Blocked one-sided Jacobi with accumulated rotations, singular values from
+// the final column norms.  Singular vectors belonging to zero singular
+// values (norm at the rounding floor sqrt(mw) eps ||A||_F) are completed to
+// an orthonormal set when `complete` is set, else left zero (enough whenever
+// they are only used multiplied by their singular value).  Returns the
+// number of Jacobi sweeps used.
 int gesvd(int m, int n, const double* A, int64_t lda, double* U, int64_t ldu, double* S,
           double* VT, int64_t ldvt, hipStream_t st, int complete = 1);
+
+// Singular values (descending: device S and host hs, k = min(m, n)) and the
+// orthonormal singular vectors of ONE side of A: left (m x k) when
+// want_left, else right (n x k), as the columns of O (ldo).  The other
+// factor is the caller's product with A (O^T A or A O).  When the wanted
+// side is the shorter dimension the Jacobi iteration runs on R^T from a
+// Householder QR of the tall orientation (the preconditioning of
+// Drmac-Veselic), which keeps the sweep count low for graded spectra and
+// needs no accumulated rotations.  Vectors of zero singular values are zero
+// columns; *kpos counts the nonzero ones.  Returns the Jacobi sweeps used.
+int svd_vectors(int m, int n, const double* A, int64_t lda, bool want_left, double* S, double* O,
+                int64_t ldo, std::vector<double>& hs, int* kpos, hipStream_t st);
+
+// Completes columns [k0, k) of U (m x k, ldu) to an orthonormal set.
+void complete_columns(int m, int k0, int k, double* U, int64_t ldu, hipStream_t st);
 
 // B (n x m) = A (m x n)^T, both column-major with leading dims.
 void transpose(int m, int n, const double* A, int64_t lda, double* B, int64_t ldb, hipStream_t st);

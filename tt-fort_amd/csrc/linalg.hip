@@ -2,9 +2,12 @@
 // symmetric eigenproblems (cyclic Jacobi in LDS) and the thin SVD (blocked
 // one-sided Jacobi) used by the rank truncation of the sweeps.
 #include "linalg.h"
+#include "gemm_mfma.h"
+#include "jacobi.h"
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -77,72 +80,7 @@ void tsgram(int N, int p, int q, const double* X, int64_t ldx, const double* Y, 
   TTF_HIP(hipGetLastError());
 }
 
-// ------------------------------------------------------------------ Jacobi
-// Cyclic Jacobi on the symmetric n x n matrix A (LDS, row stride LD) with
-// eigenvector accumulation in V (LDS).  Disjoint rotation pairs of a round
-// (circle schedule over an even index count) are applied together: first to
-// columns of A and V, then to rows of A.  Runs until a sweep applies no
-// rotation or `max_sweeps` is reached.  All threads of the block call it.
-template <int LD>
-__device__ void jacobi_lds(double (*A)[LD], double (*V)[LD], int n, int max_sweeps,
-                           double* rc, double* rs, int* rp, int* rq, int* flag) {
-  const int tid = threadIdx.x, nt = blockDim.x;
-  const int nn = n + (n & 1);
-  const int half = nn / 2;
-  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
-    if (tid == 0) *flag = 0;
-    __syncthreads();
-    for (int r = 0; r < nn - 1; ++r) {
-      for (int k = tid; k < half; k += nt) {
-        int a, b;
-        if (k == 0) { a = r; b = nn - 1; }
-        else { a = (r + k) % (nn - 1); b = (r - k + (nn - 1)) % (nn - 1); }
-        int p = min(a, b), q = max(a, b);
-        double c = 1.0, s = 0.0;
-        if (q < n) {
-          double apq = A[p][q], app = A[p][p], aqq = A[q][q];
-          if (fabs(apq) > 1e-300 && fabs(apq) > 2.2e-16 * sqrt(fabs(app) * fabs(aqq))) {
-            double tau = (aqq - app) / (2.0 * apq);
-            double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-            c = 1.0 / sqrt(1.0 + t * t);
-            s = t * c;
-            *flag = 1;
-          }
-        }
-        rp[k] = p; rq[k] = q; rc[k] = c; rs[k] = s;
-      }
-      __syncthreads();
-      // columns of A and V
-      for (int e = tid; e < half * n; e += nt) {
-        int k = e / n, i = e % n;
-        int p = rp[k], q = rq[k];
-        if (q >= n || rs[k] == 0.0) continue;
-        double c = rc[k], s = rs[k];
-        double aip = A[i][p], aiq = A[i][q];
-        A[i][p] = c * aip - s * aiq;
-        A[i][q] = s * aip + c * aiq;
-        double vip = V[i][p], viq = V[i][q];
-        V[i][p] = c * vip - s * viq;
-        V[i][q] = s * vip + c * viq;
-      }
-      __syncthreads();
-      // rows of A
-      for (int e = tid; e < half * n; e += nt) {
-        int k = e / n, j = e % n;
-        int p = rp[k], q = rq[k];
-        if (q >= n || rs[k] == 0.0) continue;
-        double c = rc[k], s = rs[k];
-        double apj = A[p][j], aqj = A[q][j];
-        A[p][j] = c * apj - s * aqj;
-        A[q][j] = s * apj + c * aqj;
-      }
-      __syncthreads();
-    }
-    if (*flag == 0) break;
-    __syncthreads();
-  }
-}
-
+// ------------------------------------------------------------------ small eig
 // Sorts eigenpairs (A diagonal, V columns) and writes w, Q (global, ld n).
 template <int LD>
 __device__ void write_sorted(double (*A)[LD], double (*V)[LD], int n, int desc, double* w,
@@ -164,16 +102,19 @@ __global__ __launch_bounds__(256) void syev_small_kernel(int n, double* Aall, do
                                                          double* Qall, int desc) {
   __shared__ double A[64][65];
   __shared__ double V[64][65];
-  __shared__ double rc[32], rs[32];
-  __shared__ int rp[32], rq[32], flag;
+  __shared__ JacobiScratch js;
+  __shared__ double red[16];
   const double* Ag = Aall + (int64_t)blockIdx.x * n * n;
+  double f2 = 0.0;
   for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
     int i = e % n, j = e / n;
-    A[i][j] = 0.5 * (Ag[i + n * j] + Ag[j + n * i]);
+    double a = 0.5 * (Ag[i + n * j] + Ag[j + n * i]);
+    A[i][j] = a;
     V[i][j] = (i == j) ? 1.0 : 0.0;
+    f2 += a * a;
   }
-  __syncthreads();
-  jacobi_lds<65>(A, V, n, 40, rc, rs, rp, rq, &flag);
+  f2 = block_sum(f2, red);
+  jacobi_sym<65>(A, V, n, 40, 2.220446049250313e-16 * sqrt(f2), &js);
   __syncthreads();
   write_sorted<65>(A, V, n, desc, wall + (int64_t)blockIdx.x * n, Qall + (int64_t)blockIdx.x * n * n);
 }
@@ -186,15 +127,21 @@ void syev_small(int n, int count, double* A, double* w, double* Q, int desc, hip
 }
 
 // ------------------------------------------------------------------ SVD
-// Blocked one-sided (Hestenes) Jacobi.  Work matrix W (mw x nw, ld mw,
-// mw >= nw) and accumulated rotations V (nw x nw).  Columns are grouped in
-// blocks of b; each round of a sweep pairs all blocks (circle schedule) and
-// one work-group per pair loads the pair's 2b columns into LDS and runs
-// one-sided Jacobi on them: 2x2 rotations computed from the columns' own dot
-// products (which keeps small singular values relatively accurate), the
-// product of the rotations accumulated in a 2b x 2b matrix that is applied
-// to V's columns at the end.  The largest relative coupling between the two
-// blocks seen before rotating drives the sweep loop.
+// One-sided (Hestenes) Jacobi on the columns of a work matrix X (mw x nw,
+// ld mw, mw >= nw), optionally accumulating the rotations in V (nw x nw).
+// Columns are grouped in blocks of b; each round of a sweep pairs all
+// blocks (circle schedule) and one work-group per pair takes the pair's
+// n = 2b columns P into LDS:
+//   1. G = P^T P by MFMA; a pair whose couplings |g_ij| / sqrt(g_ii g_jj)
+//      are all below half the tolerance is left untouched;
+//   2. a bounded number of Hestenes sweeps over the pair's columns: 2x2
+//      rotations from the columns' own dot products (which keeps small
+//      singular values accurate), their product accumulated in Qp; the
+//      largest coupling measured here (the same quantity that decides a
+//      rotation) drives the sweep loop, so a sweep without rotations ends it;
+//   3. P back to X and, with V, V(:, pair) <- V(:, pair) Qp by MFMA.
+// Columns whose squared norm is below the rounding floor small2 are treated
+// as zero and never rotated.
 constexpr int kSvdMaxP = 32;          // columns per pair (2b) at most
 constexpr int kSvdLdsDoubles = 16384; // LDS budget for the pair's columns
 
@@ -206,15 +153,182 @@ __device__ inline void circle_pair(int r, int k, int nn, int* p, int* q) {
   *q = max(a, b);
 }
 
-__global__ __launch_bounds__(256) void svd_round_kernel(int mw, int nw, int bsz, int nb, int round,
-                                                        double* __restrict__ W,
-                                                        double* __restrict__ V, double small2,
-                                                        unsigned long long* offmax) {
-  extern __shared__ double cols_lds[];   // [2b][mw]
-  __shared__ double Qp[kSvdMaxP][kSvdMaxP + 1];
+// In-place X (rows x n, column stride ld, in LDS) <- X Qp with Qp(k, j) =
+// Qc[j][k]; each wave takes 16-row slabs and loads the slab's A fragments
+// before writing any of its outputs.
+__device__ inline void lds_apply_rotation(double* X, int rows, int ld, int n,
+                                          const double (*Qc)[kSvdMaxP + 1]) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+  const int nt = (n + 15) >> 4, nk = (n + 3) >> 2;
+  for (int r0 = wv * 16; r0 < rows; r0 += 16 * nwv) {
+    const int row = r0 + (lane & 15);
+    double a[kSvdMaxP / 4];
+#pragma unroll
+    for (int t = 0; t < kSvdMaxP / 4; ++t) {
+      const int k = 4 * t + (lane >> 4);
+      a[t] = (t < nk && k < n && row < rows) ? X[k * ld + row] : 0.0;
+    }
+    for (int tj = 0; tj < nt; ++tj) {
+      const int cj = tj * 16 + (lane & 15);
+      d4_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int t = 0; t < kSvdMaxP / 4; ++t) {
+        if (t < nk) {
+          const int k = 4 * t + (lane >> 4);
+          const double bq = (k < n && cj < n) ? Qc[cj][k] : 0.0;
+          acc = mfma16(a[t], bq, acc);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int orow = r0 + mfma_row<double>(lane, r);
+        if (orow < rows && cj < n) X[cj * ld + orow] = acc[r];
+      }
+    }
+  }
+}
+
+// G (n x n, n <= 32) = P^T P for the LDS columns P (rows x n, column stride
+// ld), by MFMA: T = nt^2 tiles of 16x16, the block's 4 waves split tiles x
+// rows.  `scratch` holds 1024 doubles.  Ends with a block barrier.
+__device__ inline void lds_gram(const double* P, int rows, int ld, int n, double (*G)[kSvdMaxP + 1],
+                                double* scratch) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int nt = (n + 15) >> 4, T = nt * nt, parts = 4 / T;
+  const int tile = wv % T, part = wv / T;
+  const int ti = tile % nt, tj = tile / nt;
+  const int kchunk = (((rows + 3) / 4 + parts - 1) / parts) * 4;
+  const int kbeg = part * kchunk, kend = min(rows, kbeg + kchunk);
+  const int ci = ti * 16 + (lane & 15), cj = tj * 16 + (lane & 15), kk = lane >> 4;
+  const bool va = ci < n, vb = cj < n;
+  const double* pa = P + (va ? ci : 0) * ld;
+  const double* pb = P + (vb ? cj : 0) * ld;
+  d4_t acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = acc0;
+  for (int k0 = kbeg; k0 < kend; k0 += 8) {
+    const int r0 = k0 + kk, r1 = k0 + 4 + kk;
+    const double a0 = (va && r0 < kend) ? pa[r0] : 0.0, b0 = (vb && r0 < kend) ? pb[r0] : 0.0;
+    const double a1 = (va && r1 < kend) ? pa[r1] : 0.0, b1 = (vb && r1 < kend) ? pb[r1] : 0.0;
+    acc0 = mfma16(a0, b0, acc0);
+    acc1 = mfma16(a1, b1, acc1);
+  }
+  if (T == 4) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      G[ti * 16 + mfma_row<double>(lane, r)][tj * 16 + (lane & 15)] = acc0[r] + acc1[r];
+    __syncthreads();
+  } else {   // one tile, rows split over the 4 waves: partial tiles summed in order
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      scratch[part * 256 + mfma_row<double>(lane, r) * 16 + (lane & 15)] = acc0[r] + acc1[r];
+    __syncthreads();
+    for (int e = threadIdx.x; e < 256; e += blockDim.x)
+      G[e >> 4][e & 15] = ((scratch[e] + scratch[256 + e]) + scratch[512 + e]) + scratch[768 + e];
+    __syncthreads();
+  }
+}
+
+// Column block copies between global memory (column stride gld, columns
+// cols[0..n)) and LDS (column stride ld); each thread keeps 8 loads in flight.
+__device__ inline void cols_to_lds(const double* __restrict__ src, int64_t gld, const int* cols, int rows,
+                                   int n, double* dst, int ld) {
+  const int total = rows * n, nt = blockDim.x;
+  for (int base = threadIdx.x; base < total; base += 8 * nt) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = base + u * nt;
+      v[u] = (e < total) ? src[(int64_t)(e % rows) + gld * cols[e / rows]] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = base + u * nt;
+      if (e < total) dst[(e / rows) * ld + e % rows] = v[u];
+    }
+  }
+}
+
+__device__ inline void lds_to_cols(const double* src, int ld, int rows, int n, double* __restrict__ dst,
+                                   int64_t gld, const int* cols) {
+  const int total = rows * n;
+  for (int e = threadIdx.x; e < total; e += blockDim.x)
+    dst[(int64_t)(e % rows) + gld * cols[e / rows]] = src[(e / rows) * ld + e % rows];
+}
+
+// A column pair's 2x2 Hestenes step inside the pair block: returns the
+// measured coupling (0 for a column at the noise floor) and rotates the two
+// LDS columns when it exceeds tol.  16 threads (one row segment each) share
+// a pair; with NR > 0 each thread keeps its NR elements per column in
+// registers between the dot products and the update (mw <= 16 NR).
+template <int NR>
+__device__ inline double hestenes_pair(double* ap, double* aq, int mw, int l16, double small2, double tol,
+                                       double* c_out, double* s_out) {
+  double al = 0.0, be = 0.0, ga = 0.0;
+  double xr[NR > 0 ? NR : 1], yr[NR > 0 ? NR : 1];
+  if constexpr (NR > 0) {
+#pragma unroll
+    for (int t = 0; t < NR; ++t) {
+      const int i = l16 + 16 * t;
+      xr[t] = (i < mw) ? ap[i] : 0.0;
+      yr[t] = (i < mw) ? aq[i] : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < NR; ++t) {
+      al += xr[t] * xr[t];
+      be += yr[t] * yr[t];
+      ga += xr[t] * yr[t];
+    }
+  } else {
+    for (int i = l16; i < mw; i += 16) {
+      const double x = ap[i], y = aq[i];
+      al += x * x; be += y * y; ga += x * y;
+    }
+  }
+#pragma unroll
+  for (int o = 8; o > 0; o >>= 1) {
+    al += __shfl_xor(al, o, 16);
+    be += __shfl_xor(be, o, 16);
+    ga += __shfl_xor(ga, o, 16);
+  }
+  *s_out = 0.0;
+  if (!(al > small2 && be > small2)) return 0.0;
+  const double rel = fabs(ga) / sqrt(al * be);
+  if (rel <= tol) return rel;
+  const double zeta = (be - al) / (2.0 * ga);
+  const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+  const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+  if constexpr (NR > 0) {
+#pragma unroll
+    for (int t2 = 0; t2 < NR; ++t2) {
+      const int i = l16 + 16 * t2;
+      if (i < mw) {
+        ap[i] = c * xr[t2] - s * yr[t2];
+        aq[i] = s * xr[t2] + c * yr[t2];
+      }
+    }
+  } else {
+    for (int i = l16; i < mw; i += 16) {
+      const double x = ap[i], y = aq[i];
+      ap[i] = c * x - s * y;
+      aq[i] = s * x + c * y;
+    }
+  }
+  *c_out = c;
+  *s_out = s;
+  return rel;
+}
+
+template <int NR>
+__global__ __launch_bounds__(256) void svd_pair_kernel(int mw, int nw, int bsz, int nb, int round,
+                                                       double* __restrict__ W, double* __restrict__ V,
+                                                       double small2, double tol, int inner_sweeps,
+                                                       unsigned long long* offmax) {
+  extern __shared__ double lds[];   // pair columns: column j at lds + j * ldp
+  __shared__ double G[kSvdMaxP][kSvdMaxP + 1];    // Gram of the pair's columns
+  __shared__ double Qc[kSvdMaxP][kSvdMaxP + 1];   // Qp by columns: Qc[j][i] = Qp(i, j)
+  __shared__ double dinv[kSvdMaxP];
   __shared__ int cols[kSvdMaxP];
-  __shared__ int nv, nfirst, any;
-  __shared__ unsigned long long local_off;
+  __shared__ int nv, nfirst, any, rotated;
+  __shared__ unsigned long long cmax, rmax;
   const int tid = threadIdx.x;
   int bi, bj;
   circle_pair(round, blockIdx.x, nb, &bi, &bj);
@@ -224,97 +338,195 @@ __global__ __launch_bounds__(256) void svd_round_kernel(int mw, int nw, int bsz,
     nfirst = c;
     for (int t = 0; t < bsz; ++t) if (bj * bsz + t < nw) cols[c++] = bj * bsz + t;
     nv = c;
-    local_off = 0ull;
+    cmax = 0ull;
+    rmax = 0ull;
+    rotated = 0;
   }
   __syncthreads();
   const int n = nv, nf = nfirst;
   if (n == 0 || nf == 0 || nf == n) return;   // a block beyond the last column
-  for (int e = tid; e < n * mw; e += blockDim.x) {
-    int j = e / mw, r = e % mw;
-    cols_lds[(int64_t)j * mw + r] = W[(int64_t)r + (int64_t)mw * cols[j]];
-  }
-  for (int e = tid; e < kSvdMaxP * kSvdMaxP; e += blockDim.x)
-    Qp[e / kSvdMaxP][e % kSvdMaxP] = (e / kSvdMaxP == e % kSvdMaxP) ? 1.0 : 0.0;
+  const int ldp = mw | 1;                     // odd stride: conflict-free fragment loads
+  cols_to_lds(W, mw, cols, mw, n, lds, ldp);
   __syncthreads();
-  const double eps = 2.220446049250313e-16;
-  const double tol = 4.0 * eps * sqrt((double)mw);
-  const int nn = n + (n & 1);
-  const int g = tid >> 4, l16 = tid & 15;
-  bool rotated_any = false;
-  for (int isweep = 0; isweep < 12; ++isweep) {
-    if (tid == 0) any = 0;
-    __syncthreads();
-    for (int r = 0; r < nn - 1; ++r) {
-      if (g < nn / 2) {
-        int p, q;
-        circle_pair(r, g, nn, &p, &q);
-        if (q < n) {
-          double* ap = cols_lds + (int64_t)p * mw;
-          double* aq = cols_lds + (int64_t)q * mw;
-          double al = 0.0, be = 0.0, ga = 0.0;
-          for (int i = l16; i < mw; i += 16) {
-            double x = ap[i], y = aq[i];
-            al += x * x; be += y * y; ga += x * y;
-          }
+  // ---- 1. Gram and the skip test
+  lds_gram(lds, mw, ldp, n, G, &Qc[0][0]);
+  for (int i = tid; i < n; i += blockDim.x) {
+    const double g = G[i][i];
+    dinv[i] = g > small2 ? 1.0 / sqrt(g) : 0.0;
+  }
+  __syncthreads();
+  {
+    double cm = 0.0;
+    for (int e = tid; e < n * n; e += blockDim.x) {
+      const int i = e % n, j = e / n;
+      if (i < j) cm = fmax(cm, fabs(G[i][j]) * dinv[i] * dinv[j]);
+    }
+    atomicMax(&cmax, (unsigned long long)__double_as_longlong(cm));
+  }
+  __syncthreads();
+  if (__longlong_as_double((long long)cmax) <= 0.5 * tol) return;
+  // ---- 2. Hestenes rotations on the pair's columns; 16 groups of 16
+  // threads take one column pair each per round of the circle schedule
+  for (int e = tid; e < kSvdMaxP * kSvdMaxP; e += blockDim.x) {
+    const int j = e / kSvdMaxP, i = e % kSvdMaxP;
+    Qc[j][i] = (i == j) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  {
+    const int nn = n + (n & 1);
+    const int g = tid >> 4, l16 = tid & 15;
+    double seen = 0.0;
+    for (int isw = 0; isw < inner_sweeps; ++isw) {
+      if (tid == 0) any = 0;
+      __syncthreads();
+      for (int r = 0; r < nn - 1; ++r) {
+        if (g < nn / 2) {
+          int p, q;
+          circle_pair(r, g, nn, &p, &q);
+          if (q < n) {
+            double c, s;
+            const double rel = hestenes_pair<NR>(lds + p * ldp, lds + q * ldp, mw, l16, small2, tol, &c, &s);
+            seen = fmax(seen, rel);
+            if (s != 0.0) {
 #pragma unroll
-          for (int o = 8; o > 0; o >>= 1) {
-            al += __shfl_xor(al, o, 16);
-            be += __shfl_xor(be, o, 16);
-            ga += __shfl_xor(ga, o, 16);
-          }
-          // columns below the noise floor are numerically zero: never rotated
-          double den = (al > small2 && be > small2) ? sqrt(al * be) : 0.0;
-          double rel = den > 0.0 ? fabs(ga) / den : 0.0;
-          if (isweep == 0 && p < nf && q >= nf && l16 == 0)
-            atomicMax(&local_off, (unsigned long long)__double_as_longlong(rel));
-          if (den > 0.0 && rel > tol) {
-            double zeta = (be - al) / (2.0 * ga);
-            double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-            double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
-            for (int i = l16; i < mw; i += 16) {
-              double x = ap[i], y = aq[i];
-              ap[i] = c * x - s * y;
-              aq[i] = s * x + c * y;
+              for (int h = 0; h < 2; ++h) {
+                const int i = l16 + 16 * h;
+                const double u = Qc[p][i], v = Qc[q][i];
+                Qc[p][i] = c * u - s * v;
+                Qc[q][i] = s * u + c * v;
+              }
+              if (l16 == 0) any = 1;
             }
-            for (int i = l16; i < n; i += 16) {
-              double x = Qp[i][p], y = Qp[i][q];
-              Qp[i][p] = c * x - s * y;
-              Qp[i][q] = s * x + c * y;
-            }
-            if (l16 == 0) any = 1;
           }
         }
+        __syncthreads();
       }
+      const int stop = (any == 0);
+      if (!stop && tid == 0) rotated = 1;
       __syncthreads();
+      if (stop) break;
     }
-    if (any) rotated_any = true;
-    const int stop = (any == 0);
-    __syncthreads();
-    if (stop) break;
-  }
-  if (tid == 0) atomicMax(offmax, local_off);
-  if (!rotated_any) return;
-  for (int e = tid; e < n * mw; e += blockDim.x) {
-    int j = e / mw, r = e % mw;
-    W[(int64_t)r + (int64_t)mw * cols[j]] = cols_lds[(int64_t)j * mw + r];
+    atomicMax(&rmax, (unsigned long long)__double_as_longlong(seen));
   }
   __syncthreads();
-  // V[:, cols] <- V[:, cols] Qp, streaming 64-row chunks of V through LDS
-  double (*chunk)[kSvdMaxP + 1] = (double (*)[kSvdMaxP + 1])cols_lds;
-  for (int r0 = 0; r0 < nw; r0 += 64) {
-    for (int e = tid; e < 64 * n; e += blockDim.x) {
-      int r = e % 64, j = e / 64;
-      chunk[r][j] = (r0 + r < nw) ? V[(int64_t)(r0 + r) + (int64_t)nw * cols[j]] : 0.0;
+  if (tid == 0) atomicMax(offmax, rmax);
+  if (!rotated) return;
+  // ---- 3. the rotated columns back; V(:, pair) <- V(:, pair) Qp
+  lds_to_cols(lds, ldp, mw, n, W, mw, cols);
+  if (V == nullptr) return;
+  __syncthreads();
+  const int ldv = nw | 1;
+  cols_to_lds(V, nw, cols, nw, n, lds, ldv);
+  __syncthreads();
+  lds_apply_rotation(lds, nw, ldv, n, Qc);
+  __syncthreads();
+  lds_to_cols(lds, ldv, nw, n, V, nw, cols);
+}
+
+// ---- Householder QR (R factor only) for the preconditioning of the Jacobi
+// SVD.  One panel of pc <= 32 columns is factorised in LDS by one
+// work-group (LAPACK's dgeqr2 reflectors: H = I - tau v v^T, v(0) = 1,
+// beta = -sign(x0) ||x||); the panel's R goes back in place, the explicit
+// unit-lower V and Y = V T^T (T from the forward compact-WY recurrence of
+// dlarft) go to Vb and Yb, so that the trailing columns C are updated by two
+// GEMMs: C <- C - Y (V^T C).
+constexpr int kQrPanel = kSvdMaxP;
+
+__global__ __launch_bounds__(256) void qr_panel_kernel(int rows, int pc, double* __restrict__ X, int64_t ld,
+                                                       double* __restrict__ Vb, double* __restrict__ Yb) {
+  extern __shared__ double P[];   // panel column c at P + c * lp
+  __shared__ double T[kQrPanel][kQrPanel + 1];
+  __shared__ double S[kQrPanel][kQrPanel + 1];
+  __shared__ double tau[kQrPanel], red[16];
+  __shared__ int colidx[kQrPanel];
+  const int tid = threadIdx.x;
+  const int lp = rows | 1;
+  if (tid < kQrPanel) colidx[tid] = tid;
+  __syncthreads();
+  cols_to_lds(X, ld, colidx, rows, pc, P, lp);
+  __syncthreads();
+  const int g = tid >> 4, l16 = tid & 15;
+  for (int j = 0; j < pc; ++j) {
+    double* x = P + j * lp;
+    double s = 0.0;
+    for (int i = j + 1 + tid; i < rows; i += blockDim.x) s += x[i] * x[i];
+    s = block_sum(s, red);
+    const double x0 = (j < rows) ? x[j] : 0.0;
+    double t = 0.0, beta = x0, scal = 0.0;
+    if (s > 0.0) {
+      const double nrm = sqrt(x0 * x0 + s);
+      beta = x0 >= 0.0 ? -nrm : nrm;
+      t = (beta - x0) / beta;
+      scal = 1.0 / (x0 - beta);
     }
     __syncthreads();
-    for (int e = tid; e < 64 * n; e += blockDim.x) {
-      int r = e % 64, j = e / 64;
-      if (r0 + r >= nw) continue;
-      double s = 0.0;
-      for (int k = 0; k < n; ++k) s += chunk[r][k] * Qp[k][j];
-      V[(int64_t)(r0 + r) + (int64_t)nw * cols[j]] = s;
+    for (int i = j + 1 + tid; i < rows; i += blockDim.x) x[i] *= scal;
+    if (tid == 0) {
+      if (j < rows) x[j] = beta;
+      tau[j] = t;
     }
     __syncthreads();
+    if (t != 0.0) {
+      for (int c = j + 1 + g; c < pc; c += 16) {
+        double* y = P + c * lp;
+        double w = (l16 == 0) ? y[j] : 0.0;
+        for (int i = j + 1 + l16; i < rows; i += 16) w += x[i] * y[i];
+#pragma unroll
+        for (int o = 8; o > 0; o >>= 1) w += __shfl_xor(w, o, 16);
+        w *= t;
+        if (l16 == 0) y[j] -= w;
+        for (int i = j + 1 + l16; i < rows; i += 16) y[i] -= w * x[i];
+      }
+    }
+    __syncthreads();
+  }
+  // R (upper part of the panel's first pc rows) back in place
+  for (int e = tid; e < pc * pc; e += blockDim.x) {
+    const int i = e % pc, c = e / pc;
+    if (i <= c && i < rows) X[i + ld * c] = P[c * lp + i];
+  }
+  __syncthreads();
+  // explicit unit-lower V in the panel
+  for (int e = tid; e < pc * rows; e += blockDim.x) {
+    const int i = e % rows, c = e / rows;
+    if (i <= c) P[c * lp + i] = (i == c) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  lds_gram(P, rows, lp, pc, S, &T[0][0]);   // S = V^T V (T used as scratch)
+  // T: T(j, j) = tau_j, T(0:j, j) = -tau_j T(0:j, 0:j) S(0:j, j)
+  for (int e = tid; e < kQrPanel * (kQrPanel + 1); e += blockDim.x) (&T[0][0])[e] = 0.0;
+  __syncthreads();
+  for (int j = 0; j < pc; ++j) {
+    if (tid < j) {
+      double acc = 0.0;
+      for (int l = tid; l < j; ++l) acc += T[tid][l] * S[l][j];
+      T[tid][j] = -tau[j] * acc;
+    }
+    if (tid == 0) T[j][j] = tau[j];
+    __syncthreads();
+  }
+  for (int e = tid; e < pc * rows; e += blockDim.x) {
+    const int i = e % rows, c = e / rows;
+    Vb[i + (int64_t)rows * c] = P[c * lp + i];
+  }
+  __syncthreads();
+  // Y = V T^T: lds_apply_rotation computes X Qp with Qp(k, j) = Qc[j][k] = T[j][k]
+  lds_apply_rotation(P, rows, lp, pc, T);
+  __syncthreads();
+  for (int e = tid; e < pc * rows; e += blockDim.x) {
+    const int i = e % rows, c = e / rows;
+    Yb[i + (int64_t)rows * c] = P[c * lp + i];
+  }
+}
+
+// X (q x q, ld q) = R^T for the upper triangle R of W (ld ldw).
+__global__ void upper_transpose_kernel(int q, const double* __restrict__ W, int64_t ldw,
+                                       double* __restrict__ X) {
+  const int64_t total = (int64_t)q * q;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % q), j = (int)(e / q);
+    X[e] = (j <= i) ? W[j + ldw * i] : 0.0;
   }
 }
 
@@ -423,6 +635,159 @@ __global__ __launch_bounds__(256) void complete_basis(int m, int k0, int k, doub
   }
 }
 
+void complete_columns(int m, int k0, int k, double* U, int64_t ldu, hipStream_t st) {
+  if (k0 >= k) return;
+  TTF_REQUIRE(k <= 4096, "complete_columns: k=%d exceeds 4096", k);
+  hipLaunchKernelGGL(complete_basis, dim3(1), dim3(256), 0, st, m, k0, k, U, ldu);
+  TTF_HIP(hipGetLastError());
+}
+
+// Hestenes sweeps per pair and round (TTF_SVD_INNER overrides; default 1).
+static int svd_inner_sweeps() {
+  static int v = [] {
+    const char* e = std::getenv("TTF_SVD_INNER");
+    int x = e ? std::atoi(e) : 1;
+    return std::min(std::max(x, 1), 30);
+  }();
+  return v;
+}
+
+namespace {
+
+struct JacobiResult {
+  int sweeps = 0;
+  double small2 = 0.0;   // squared rounding floor of the column norms
+};
+
+// One-sided Jacobi on the columns of X (mw x nw, ld mw) until a sweep makes
+// no rotation; V (nw x nw) accumulates the rotations when not null.
+JacobiResult jacobi_columns(int mw, int nw, double* X, double* V, hipStream_t st) {
+  JacobiResult res;
+  double* nrm = (double*)arena(kArenaSmall, sizeof(double) * (size_t)nw + 64);
+  unsigned long long* offmax = (unsigned long long*)(nrm + nw);
+  if (V) {
+    hipLaunchKernelGGL(set_identity, dim3((unsigned)std::min<int64_t>(ceil_div((int64_t)nw * nw, 256), 4096)),
+                       dim3(256), 0, st, nw, V);
+    TTF_HIP(hipGetLastError());
+  }
+  // column norms bound the absolute noise floor of the cross products
+  hipLaunchKernelGGL(colnorm_kernel, dim3(nw), dim3(256), 0, st, mw, nw, X, nrm);
+  std::vector<double> hn(nw);
+  TTF_HIP(hipMemcpyAsync(hn.data(), nrm, sizeof(double) * nw, hipMemcpyDeviceToHost, st));
+  TTF_HIP(hipStreamSynchronize(st));
+  double fro2 = 0.0;
+  for (double v : hn) fro2 += v * v;
+  if (!(fro2 > 0.0) || nw < 2) return res;
+  const double eps = 2.220446049250313e-16;
+  // couplings are measured to about sqrt(mw) eps (dot-product rounding);
+  // the tolerance sits well above that noise
+  const double tol = std::max(8.0 * std::sqrt((double)mw), 32.0) * eps;
+  // squared column norms below (sqrt(mw) eps ||X||_F)^2 are rounding noise
+  res.small2 = mw * eps * eps * fro2;
+  // block size: the pair's 2b columns (odd stride mw | 1) must fit the LDS budget
+  const int ldp = mw | 1;
+  int two_b = std::min<int>(kSvdMaxP, kSvdLdsDoubles / ldp);
+  two_b -= (two_b & 1);
+  TTF_REQUIRE(two_b >= 2, "jacobi: %d rows exceed the pair LDS budget", mw);
+  // at least two non-empty blocks, so that every column pair is coupled
+  const int bsz = std::max(1, std::min(two_b / 2, (int)ceil_div(nw, 2)));
+  int nb = (int)ceil_div(nw, bsz);
+  if (nb % 2) ++nb;
+  if (nb < 2) nb = 2;
+  const size_t lds = sizeof(double) * (size_t)(2 * bsz) * ldp;
+  // registers per thread and column in the Hestenes steps (16 threads a pair)
+  const int nr = (int)ceil_div(mw, 16);
+  auto kern = nr <= 4 ? svd_pair_kernel<4> : nr <= 8 ? svd_pair_kernel<8>
+            : nr <= 16 ? svd_pair_kernel<16> : nr <= 32 ? svd_pair_kernel<32> : svd_pair_kernel<0>;
+  static bool attr_set = false;
+  if (!attr_set) {
+    for (auto f : {svd_pair_kernel<4>, svd_pair_kernel<8>, svd_pair_kernel<16>, svd_pair_kernel<32>,
+                   svd_pair_kernel<0>})
+      TTF_HIP(hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)(sizeof(double) * kSvdLdsDoubles)));
+    attr_set = true;
+  }
+  const int inner = svd_inner_sweeps();
+  unsigned long long hoff = 0;
+  // cyclic one-sided Jacobi needs about 40 sweeps on unpreconditioned
+  // columns with a 1e-13 graded spectrum (10 after the QR preconditioning)
+  for (res.sweeps = 1; res.sweeps <= 60; ++res.sweeps) {
+    TTF_HIP(hipMemsetAsync(offmax, 0, sizeof(unsigned long long), st));
+    for (int r = 0; r < nb - 1; ++r) {
+      hipLaunchKernelGGL(kern, dim3(nb / 2), dim3(256), lds, st, mw, nw, bsz, nb, r, X, V, res.small2, tol,
+                         inner, offmax);
+      TTF_HIP(hipGetLastError());
+    }
+    TTF_HIP(hipMemcpyAsync(&hoff, offmax, sizeof(hoff), hipMemcpyDeviceToHost, st));
+    TTF_HIP(hipStreamSynchronize(st));
+    double off;
+    std::memcpy(&off, &hoff, sizeof(off));
+    if (off <= tol) break;
+  }
+  return res;
+}
+
+// Sorted column norms of X (mw x k, ld mw): hs (host, descending), the
+// permutation (device perm) and scales 1/s (0 below the rounding floor) in
+// scale; returns the number of columns above the floor.
+int sorted_norms(int mw, int k, const double* X, double small2, std::vector<double>& hs, int* perm,
+                 double* scale, hipStream_t st) {
+  double* nrm = (double*)arena(kArenaSmall, sizeof(double) * (size_t)k);
+  hipLaunchKernelGGL(colnorm_kernel, dim3(k), dim3(256), 0, st, mw, k, X, nrm);
+  std::vector<double> hn(k);
+  TTF_HIP(hipMemcpyAsync(hn.data(), nrm, sizeof(double) * k, hipMemcpyDeviceToHost, st));
+  TTF_HIP(hipStreamSynchronize(st));
+  std::vector<int> order(k);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return hn[a] > hn[b]; });
+  const double tiny = std::max(std::sqrt(small2), 1e-300);
+  hs.assign(k, 0.0);
+  std::vector<double> hscale(k);
+  int kpos = 0;
+  for (int c = 0; c < k; ++c) {
+    hs[c] = hn[order[c]];
+    hscale[c] = hs[c] > tiny ? 1.0 / hs[c] : 0.0;
+    if (hs[c] > tiny) kpos = c + 1;
+  }
+  TTF_HIP(hipMemcpyAsync(perm, order.data(), sizeof(int) * k, hipMemcpyHostToDevice, st));
+  TTF_HIP(hipMemcpyAsync(scale, hscale.data(), sizeof(double) * k, hipMemcpyHostToDevice, st));
+  return kpos;
+}
+
+// R factor of the QR factorisation of W (p x q, ld p, p >= q, overwritten):
+// R is left in the upper triangle of W(0:q, 0:q).
+void qr_r_factor(int p, int q, double* W, hipStream_t st) {
+  const int lp = p | 1;
+  const int pcmax = std::min(kQrPanel, kSvdLdsDoubles / lp);
+  TTF_REQUIRE(pcmax >= 1, "qr: %d rows exceed the panel LDS budget", p);
+  static bool attr_set = false;
+  if (!attr_set) {
+    TTF_HIP(hipFuncSetAttribute((const void*)qr_panel_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(sizeof(double) * kSvdLdsDoubles)));
+    attr_set = true;
+  }
+  double* Vb = (double*)arena(kArenaTrunc3, sizeof(double) * ((size_t)2 * p * kQrPanel + (size_t)kQrPanel * q));
+  double* Yb = Vb + (size_t)p * kQrPanel;
+  double* Z = Yb + (size_t)p * kQrPanel;
+  for (int k0 = 0; k0 < q; k0 += pcmax) {
+    const int rows = p - k0, pc = std::min(pcmax, q - k0);
+    const size_t lds = sizeof(double) * (size_t)pc * (rows | 1);
+    double* Xp = W + k0 + (int64_t)p * k0;
+    hipLaunchKernelGGL(qr_panel_kernel, dim3(1), dim3(256), lds, st, rows, pc, Xp, (int64_t)p, Vb, Yb);
+    TTF_HIP(hipGetLastError());
+    const int nrest = q - k0 - pc;
+    if (nrest > 0) {
+      double* Cm = W + k0 + (int64_t)p * (k0 + pc);
+      GemmOp<double> g1 = colmajor<double>('T', 'N', pc, nrest, rows, 1.0, Vb, rows, Cm, p, 0.0, Z, pc);
+      gemm_launch(g1, st);
+      GemmOp<double> g2 = colmajor<double>('N', 'N', rows, nrest, pc, -1.0, Yb, rows, Z, pc, 1.0, Cm, p);
+      gemm_launch(g2, st);
+    }
+  }
+}
+
+}  // namespace
+
 int gesvd(int m, int n, const double* A, int64_t lda, double* U, int64_t ldu, double* S,
           double* VT, int64_t ldvt, hipStream_t st, int complete) {
   TTF_REQUIRE(m >= 1 && n >= 1, "gesvd: invalid sizes %d x %d", m, n);
@@ -430,102 +795,34 @@ int gesvd(int m, int n, const double* A, int64_t lda, double* U, int64_t ldu, do
   const int mw = tall ? m : n, nw = tall ? n : m;   // work matrix mw x nw
   const int k = nw;
   TTF_REQUIRE(nw <= 4096, "gesvd: min(m,n)=%d exceeds 4096", nw);
-  // arena layout: W (mw*nw) | V (nw*nw) | nrm (nw) | scale (nw) | perm (nw ints) | offmax
-  size_t need = sizeof(double) * ((size_t)mw * nw + (size_t)nw * nw + 2 * (size_t)nw) +
-                sizeof(int) * (size_t)nw + 64;
+  // arena layout: W (mw*nw) | V (nw*nw) | scale (nw) | perm (nw ints)
+  size_t need = sizeof(double) * ((size_t)mw * nw + (size_t)nw * nw + (size_t)nw) + sizeof(int) * (size_t)nw;
   char* base = (char*)arena(kArenaTrunc, need);
   double* W = (double*)base;
   double* V = W + (size_t)mw * nw;
-  double* nrm = V + (size_t)nw * nw;
-  double* scale = nrm + nw;
+  double* scale = V + (size_t)nw * nw;
   int* perm = (int*)(scale + nw);
-  unsigned long long* offmax = (unsigned long long*)(((uintptr_t)(perm + nw) + 15) & ~(uintptr_t)15);
   if (tall) {
     TTF_HIP(hipMemcpy2DAsync(W, sizeof(double) * mw, A, sizeof(double) * lda, sizeof(double) * m, n,
                              hipMemcpyDeviceToDevice, st));
   } else {
     transpose(m, n, A, lda, W, mw, st);
   }
-  hipLaunchKernelGGL(set_identity, dim3((unsigned)std::min<int64_t>(ceil_div((int64_t)nw * nw, 256), 4096)),
-                     dim3(256), 0, st, nw, V);
-  TTF_HIP(hipGetLastError());
-  // column norms bound the absolute noise floor of the cross products
-  hipLaunchKernelGGL(colnorm_kernel, dim3(nw), dim3(256), 0, st, mw, nw, W, nrm);
-  std::vector<double> hn(nw);
-  TTF_HIP(hipMemcpyAsync(hn.data(), nrm, sizeof(double) * nw, hipMemcpyDeviceToHost, st));
-  TTF_HIP(hipStreamSynchronize(st));
-  double fro2 = 0.0;
-  for (double v : hn) fro2 += v * v;
-  int sweeps = 0;
-  double small2 = 0.0;
-  if (fro2 > 0.0) {
-    const double eps = 2.220446049250313e-16;
-    const double tol = 4.0 * eps * std::sqrt((double)mw);
-    // squared column norms below (sqrt(mw) eps ||A||_F)^2 are rounding noise
-    small2 = mw * eps * eps * fro2;
-    // block size: the pair's 2b columns must fit the LDS budget
-    int two_b = std::min<int>(kSvdMaxP, kSvdLdsDoubles / mw);
-    two_b = std::max(2, two_b - (two_b & 1));
-    // at least two non-empty blocks, so that every column pair is coupled
-    const int bsz = std::max(1, std::min(two_b / 2, (int)ceil_div(nw, 2)));
-    int nb = (int)ceil_div(nw, bsz);
-    if (nb % 2) ++nb;
-    if (nb < 2) nb = 2;
-    const size_t lds = sizeof(double) * (size_t)std::max(two_b * mw, 64 * (kSvdMaxP + 1));
-    TTF_REQUIRE(lds <= 160 * 1024, "gesvd: pair block does not fit LDS (mw=%d)", mw);
-    static bool attr_set = false;
-    if (!attr_set) {
-      TTF_HIP(hipFuncSetAttribute((const void*)svd_round_kernel,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 10240));
-      attr_set = true;
-    }
-    unsigned long long hoff = 0;
-    for (sweeps = 1; sweeps <= 40; ++sweeps) {
-      TTF_HIP(hipMemsetAsync(offmax, 0, sizeof(unsigned long long), st));
-      for (int r = 0; r < nb - 1; ++r) {
-        hipLaunchKernelGGL(svd_round_kernel, dim3(nb / 2), dim3(256), lds, st, mw, nw, bsz, nb, r, W,
-                           V, small2, offmax);
-        TTF_HIP(hipGetLastError());
-      }
-      TTF_HIP(hipMemcpyAsync(&hoff, offmax, sizeof(hoff), hipMemcpyDeviceToHost, st));
-      TTF_HIP(hipStreamSynchronize(st));
-      double off;
-      std::memcpy(&off, &hoff, sizeof(off));
-      if (off <= tol) break;
-    }
-  }
-  hipLaunchKernelGGL(colnorm_kernel, dim3(nw), dim3(256), 0, st, mw, nw, W, nrm);
-  TTF_HIP(hipMemcpyAsync(hn.data(), nrm, sizeof(double) * nw, hipMemcpyDeviceToHost, st));
-  TTF_HIP(hipStreamSynchronize(st));
-  std::vector<int> order(nw);
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return hn[a] > hn[b]; });
-  // columns at the rounding-noise floor count as zero singular values
-  const double tiny = std::max(std::sqrt(small2), 1e-300);
-  std::vector<double> hs(nw), hscale(nw), hone(nw, 1.0);
-  int kpos = 0;
-  for (int c = 0; c < nw; ++c) {
-    hs[c] = hn[order[c]];
-    hscale[c] = hs[c] > tiny ? 1.0 / hs[c] : 0.0;
-    if (hs[c] > tiny) kpos = c + 1;
-  }
-  // upload permutation and scales, then form the normalised left factor of
-  // W and the permuted V
-  TTF_HIP(hipMemcpyAsync(perm, order.data(), sizeof(int) * nw, hipMemcpyHostToDevice, st));
-  TTF_HIP(hipMemcpyAsync(scale, hscale.data(), sizeof(double) * nw, hipMemcpyHostToDevice, st));
+  JacobiResult jr = jacobi_columns(mw, nw, W, V, st);
+  std::vector<double> hs;
+  const int kpos = sorted_norms(mw, nw, W, jr.small2, hs, perm, scale, st);
   TTF_HIP(hipMemcpyAsync(S, hs.data(), sizeof(double) * k, hipMemcpyHostToDevice, st));
+  // normalised left factor of W and the permuted V
   double* Uw = tall ? U : (double*)arena(kArenaTrunc2, sizeof(double) * (size_t)mw * k);
   const int64_t lduw = tall ? ldu : mw;
   int blocks = (int)std::min<int64_t>(ceil_div((int64_t)mw * k, 256), 8192);
   hipLaunchKernelGGL(gather_scale_cols, dim3(blocks), dim3(256), 0, st, mw, k, W, (int64_t)mw, perm,
                      scale, Uw, lduw);
   TTF_HIP(hipGetLastError());
-  if (kpos < k && complete) {
-    hipLaunchKernelGGL(complete_basis, dim3(1), dim3(256), 0, st, mw, kpos, k, Uw, lduw);
-    TTF_HIP(hipGetLastError());
-  }
+  if (kpos < k && complete) complete_columns(mw, kpos, k, Uw, lduw, st);
   // V permuted (scale 1): Vp(:, c) = V(:, order[c])
   double* Vp = (double*)arena(kArenaTrunc3, sizeof(double) * (size_t)nw * nw);
+  std::vector<double> hone(nw, 1.0);
   TTF_HIP(hipMemcpyAsync(scale, hone.data(), sizeof(double) * nw, hipMemcpyHostToDevice, st));
   blocks = (int)std::min<int64_t>(ceil_div((int64_t)nw * nw, 256), 8192);
   hipLaunchKernelGGL(gather_scale_cols, dim3(blocks), dim3(256), 0, st, nw, nw, V, (int64_t)nw, perm,
@@ -541,7 +838,67 @@ int gesvd(int m, int n, const double* A, int64_t lda, double* U, int64_t ldu, do
     transpose(mw, k, Uw, mw, VT, ldvt, st);
   }
   TTF_HIP(hipStreamSynchronize(st));
-  return sweeps;
+  return jr.sweeps;
+}
+
+int svd_vectors(int m, int n, const double* A, int64_t lda, bool want_left, double* S, double* O,
+                int64_t ldo, std::vector<double>& hs, int* kpos, hipStream_t st) {
+  TTF_REQUIRE(m >= 1 && n >= 1, "svd_vectors: invalid sizes %d x %d", m, n);
+  const bool tall = m >= n;
+  const int p = tall ? m : n, q = tall ? n : m;     // T = A or A^T, p x q
+  TTF_REQUIRE(q <= 4096, "svd_vectors: min(m,n)=%d exceeds 4096", q);
+  const bool short_side = want_left ? !tall : tall;  // vectors of dimension q
+  const bool precond = short_side && (p | 1) <= kSvdLdsDoubles;
+  if (short_side && !precond) {
+    // very long rows: the plain SVD, then the wanted factor
+    double* Uf = (double*)arena(kArenaDriver, sizeof(double) * ((size_t)m * q + (size_t)q * n));
+    double* VTf = Uf + (size_t)m * q;
+    gesvd(m, n, A, lda, Uf, m, S, VTf, q, st, 0);
+    hs.assign(q, 0.0);
+    TTF_HIP(hipMemcpyAsync(hs.data(), S, sizeof(double) * q, hipMemcpyDeviceToHost, st));
+    TTF_HIP(hipStreamSynchronize(st));
+    if (want_left) {
+      TTF_HIP(hipMemcpy2DAsync(O, sizeof(double) * ldo, Uf, sizeof(double) * m, sizeof(double) * m, q,
+                               hipMemcpyDeviceToDevice, st));
+    } else {
+      transpose(q, n, VTf, q, O, ldo, st);
+    }
+    int kp = 0;
+    for (int c = 0; c < q; ++c) if (hs[c] > 0.0) kp = c + 1;
+    *kpos = kp;
+    return 0;
+  }
+  size_t need = sizeof(double) * ((size_t)p * q + (size_t)q * q + (size_t)q) + sizeof(int) * (size_t)q;
+  char* base = (char*)arena(kArenaTrunc, need);
+  double* W = (double*)base;
+  double* Xr = W + (size_t)p * q;
+  double* scale = Xr + (size_t)q * q;
+  int* perm = (int*)(scale + q);
+  if (tall) {
+    TTF_HIP(hipMemcpy2DAsync(W, sizeof(double) * p, A, sizeof(double) * lda, sizeof(double) * m, n,
+                             hipMemcpyDeviceToDevice, st));
+  } else {
+    transpose(m, n, A, lda, W, p, st);
+  }
+  double* X = W;
+  int mw = p;
+  if (precond) {
+    // T = Q R; the left singular vectors of R^T are T's right singular vectors
+    qr_r_factor(p, q, W, st);
+    hipLaunchKernelGGL(upper_transpose_kernel, dim3((unsigned)std::min<int64_t>(ceil_div((int64_t)q * q, 256), 4096)),
+                       dim3(256), 0, st, q, W, (int64_t)p, Xr);
+    TTF_HIP(hipGetLastError());
+    X = Xr;
+    mw = q;
+  }
+  JacobiResult jr = jacobi_columns(mw, q, X, nullptr, st);
+  *kpos = sorted_norms(mw, q, X, jr.small2, hs, perm, scale, st);
+  TTF_HIP(hipMemcpyAsync(S, hs.data(), sizeof(double) * q, hipMemcpyHostToDevice, st));
+  const int blocks = (int)std::min<int64_t>(ceil_div((int64_t)mw * q, 256), 8192);
+  hipLaunchKernelGGL(gather_scale_cols, dim3(blocks), dim3(256), 0, st, mw, q, X, (int64_t)mw, perm, scale,
+                     O, ldo);
+  TTF_HIP(hipGetLastError());
+  return jr.sweeps;
 }
 
 }  // namespace ttf
@@ -555,6 +912,21 @@ int ttf_dgesvd(int m, int n, const double* A, int64_t lda, double* U, int64_t ld
                double* VT, int64_t ldvt, int* sweeps) {
   return guarded([&] {
     int sw = gesvd(m, n, A, lda, U, ldu, S, VT, ldvt, stream());
+    if (sweeps) *sweeps = sw;
+  });
+}
+
+int ttf_dsvd_side(int m, int n, const double* A, int64_t lda, char side, double* S, double* O,
+                  int64_t ldo, int* kpos, int* sweeps) {
+  return guarded([&] {
+    TTF_REQUIRE(side == 'L' || side == 'l' || side == 'R' || side == 'r', "ttf_dsvd_side: side '%c'", side);
+    const bool left = side == 'L' || side == 'l';
+    TTF_REQUIRE(ldo >= (left ? m : n), "ttf_dsvd_side: ldo=%lld too small", (long long)ldo);
+    std::vector<double> hs;
+    int kp = 0;
+    int sw = svd_vectors(m, n, A, lda, left, S, O, ldo, hs, &kp, stream());
+    TTF_HIP(hipStreamSynchronize(stream()));
+    if (kpos) *kpos = kp;
     if (sweeps) *sweeps = sw;
   });
 }

@@ -17,6 +17,7 @@
 // Dropped (zero) columns carry a large diagonal penalty in the projected
 // matrix so they can never produce a spurious Ritz value.
 #include "solvers.h"
+#include "jacobi.h"
 
 #include <cmath>
 #include <vector>
@@ -26,85 +27,32 @@ namespace ttf {
 namespace {
 
 constexpr int kMaxM = 64;   // largest projected basis (3 * block size)
-
-template <int LD>
-__device__ void jacobi_eig(double (*A)[LD], double (*V)[LD], int n, double* rc, double* rs, int* rp,
-                           int* rq, int* flag) {
-  const int tid = threadIdx.x, nt = blockDim.x;
-  const int nn = n + (n & 1), half = nn / 2;
-  for (int sweep = 0; sweep < 40; ++sweep) {
-    if (tid == 0) *flag = 0;
-    __syncthreads();
-    for (int r = 0; r < nn - 1; ++r) {
-      for (int k = tid; k < half; k += nt) {
-        int a, b;
-        if (k == 0) { a = r; b = nn - 1; }
-        else { a = (r + k) % (nn - 1); b = (r - k + (nn - 1)) % (nn - 1); }
-        int p = min(a, b), q = max(a, b);
-        double c = 1.0, s = 0.0;
-        if (q < n) {
-          double apq = A[p][q], app = A[p][p], aqq = A[q][q];
-          if (fabs(apq) > 1e-300 && fabs(apq) > 2.2e-16 * sqrt(fabs(app) * fabs(aqq))) {
-            double tau = (aqq - app) / (2.0 * apq);
-            double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-            c = 1.0 / sqrt(1.0 + t * t);
-            s = t * c;
-            *flag = 1;
-          }
-        }
-        rp[k] = p; rq[k] = q; rc[k] = c; rs[k] = s;
-      }
-      __syncthreads();
-      for (int e = tid; e < half * n; e += nt) {
-        int k = e / n, i = e % n;
-        int p = rp[k], q = rq[k];
-        if (q >= n || rs[k] == 0.0) continue;
-        double c = rc[k], s = rs[k];
-        double aip = A[i][p], aiq = A[i][q];
-        A[i][p] = c * aip - s * aiq;
-        A[i][q] = s * aip + c * aiq;
-        double vip = V[i][p], viq = V[i][q];
-        V[i][p] = c * vip - s * viq;
-        V[i][q] = s * vip + c * viq;
-      }
-      __syncthreads();
-      for (int e = tid; e < half * n; e += nt) {
-        int k = e / n, j = e % n;
-        int p = rp[k], q = rq[k];
-        if (q >= n || rs[k] == 0.0) continue;
-        double c = rc[k], s = rs[k];
-        double apj = A[p][j], aqj = A[q][j];
-        A[p][j] = c * apj - s * aqj;
-        A[q][j] = s * apj + c * aqj;
-      }
-      __syncthreads();
-    }
-    if (*flag == 0) break;
-    __syncthreads();
-  }
-}
+constexpr double kEps = 2.220446049250313e-16;
 
 // SVQB: G (k x k) = W^T W -> T = U diag(lambda^-1/2) over directions with
 // lambda > 1e-14 lambda_max (others get a zero column and mask 0).
 __global__ __launch_bounds__(256) void svqb_kernel(int k, const double* __restrict__ G,
                                                    double* __restrict__ T, int* __restrict__ mask) {
   __shared__ double A[kMaxM][kMaxM + 1], V[kMaxM][kMaxM + 1];
-  __shared__ double rc[kMaxM / 2], rs[kMaxM / 2], dsc[kMaxM];
-  __shared__ int rp[kMaxM / 2], rq[kMaxM / 2], flag;
+  __shared__ double dsc[kMaxM], red[16];
+  __shared__ JacobiScratch js;
+  double f2 = 0.0;
   for (int e = threadIdx.x; e < k * k; e += blockDim.x) {
     int i = e % k, j = e / k;
     double gi = G[i + k * i], gj = G[j + k * j];
     // scale to unit diagonal first (the "S" of SVQB) for a well-conditioned eig
     double di = gi > 0 ? 1.0 / sqrt(gi) : 0.0, dj = gj > 0 ? 1.0 / sqrt(gj) : 0.0;
-    A[i][j] = 0.5 * (G[i + k * j] + G[j + k * i]) * di * dj;
+    double a = 0.5 * (G[i + k * j] + G[j + k * i]) * di * dj;
+    A[i][j] = a;
     V[i][j] = (i == j) ? 1.0 : 0.0;
+    f2 += a * a;
   }
   for (int i = threadIdx.x; i < k; i += blockDim.x) {
     double gi = G[i + k * i];
     dsc[i] = gi > 0 ? 1.0 / sqrt(gi) : 0.0;
   }
-  __syncthreads();
-  jacobi_eig<kMaxM + 1>(A, V, k, rc, rs, rp, rq, &flag);
+  f2 = block_sum(f2, red);
+  jacobi_sym<kMaxM + 1>(A, V, k, 40, kEps * sqrt(f2), &js);
   __syncthreads();
   double lmax = 0.0;
   for (int i = 0; i < k; ++i) lmax = fmax(lmax, A[i][i]);
@@ -130,8 +78,8 @@ __global__ __launch_bounds__(256) void rr_kernel(int m, int bs, int nx, const do
                                                  double* __restrict__ amax) {
   __shared__ double A[kMaxM][kMaxM + 1], V[kMaxM][kMaxM + 1], Y[kMaxM][kMaxM + 1];
   __shared__ double Z[kMaxM][kMaxM / 2 + 1];
-  __shared__ double rc[kMaxM / 2], rs[kMaxM / 2], w[kMaxM], red[256];
-  __shared__ int rp[kMaxM / 2], rq[kMaxM / 2], flag;
+  __shared__ double w[kMaxM], red[16];
+  __shared__ JacobiScratch js;
   __shared__ double big;
   const int tid = threadIdx.x;
   if (tid == 0) {
@@ -140,15 +88,19 @@ __global__ __launch_bounds__(256) void rr_kernel(int m, int bs, int nx, const do
     big = 1e6 * (1.0 + mx);
   }
   __syncthreads();
+  // the rounding floor of the rotations is set by the valid block only: the
+  // penalty rows are exactly decoupled and never rotate
+  double f2 = 0.0;
   for (int e = tid; e < m * m; e += blockDim.x) {
     int i = e % m, j = e / m;
     bool ok = mask[i] && mask[j];
     double h = 0.5 * (H[i + m * j] + H[j + m * i]);
     A[i][j] = ok ? h : ((i == j) ? big : 0.0);
     V[i][j] = (i == j) ? 1.0 : 0.0;
+    if (ok) f2 += h * h;
   }
-  __syncthreads();
-  jacobi_eig<kMaxM + 1>(A, V, m, rc, rs, rp, rq, &flag);
+  f2 = block_sum(f2, red);
+  jacobi_sym<kMaxM + 1>(A, V, m, 40, kEps * sqrt(f2), &js);
   __syncthreads();
   // ascending order
   for (int i = tid; i < m; i += blockDim.x) {
@@ -196,7 +148,6 @@ __global__ __launch_bounds__(256) void rr_kernel(int m, int bs, int nx, const do
   // modified Gram-Schmidt on the columns of Z (twice).  The kernel runs as a
   // single wave with lane r owning row r (m <= 64): dot products are wave
   // reductions through lane shuffles.
-  (void)red;
   for (int pass = 0; pass < 2; ++pass) {
     for (int c = 0; c < bs; ++c) {
       for (int j = 0; j < c; ++j) {

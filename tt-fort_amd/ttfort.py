@@ -83,6 +83,8 @@ _sig("ttf_d2d_fullmat", _c_int, *([_c_int] * 5 + [_vp, _vp, _vp, _vp]))
 _sig("dbfun3", None, *([_vp] * 15))
 _sig("ttf_dgesvd", _c_int, _c_int, _c_int, _vp, _c_i64, _vp, _c_i64, _vp, _vp, _c_i64,
      C.POINTER(_c_int))
+_sig("ttf_dsvd_side", _c_int, _c_int, _c_int, _vp, _c_i64, C.c_char, _vp, _vp, _c_i64,
+     C.POINTER(_c_int), C.POINTER(_c_int))
 _sig("ttf_dsyev_small", _c_int, _c_int, _c_int, _vp, _vp, _vp, _c_int)
 _sig("ttf_dgram", _c_int, _c_int, _c_int, _c_int, _vp, _c_i64, _vp, _c_i64, _vp, _c_int)
 
@@ -307,6 +309,22 @@ def gesvd(A):
     sw = C.c_int()
     check(lib.ttf_dgesvd(m, n, dA.ptr, m, dU.ptr, m, dS.ptr, dVT.ptr, k, C.byref(sw)), lib)
     return dU.numpy(), dS.numpy(), dVT.numpy(), sw.value
+
+
+def svd_side(A, side):
+    """Singular values and one side's singular vectors through ttf_dsvd_side:
+    returns (s, O, kpos, sweeps) with O (m x k) for side 'L', (n x k) for 'R'."""
+    lib = load()
+    A = np.asfortranarray(A, dtype=np.float64)
+    m, n = A.shape
+    k = min(m, n)
+    rows = m if side in ("L", "l") else n
+    dA = DeviceArray.from_numpy(A)
+    dS, dO = DeviceArray((k,)), DeviceArray((rows, k))
+    kp, sw = C.c_int(), C.c_int()
+    check(lib.ttf_dsvd_side(m, n, dA.ptr, m, side.encode(), dS.ptr, dO.ptr, rows, C.byref(kp), C.byref(sw)),
+          lib)
+    return dS.numpy(), dO.numpy(), kp.value, sw.value
 
 
 def syev_small(As, desc=False):
