@@ -63,6 +63,7 @@ __global__ void tsgram_reduce(int p, int q, int nparts, const double* __restrict
   int pq = p * q;
   if (e >= pq) return;
   double s = 0.0;
+#pragma unroll 16
   for (int w = 0; w < nparts; ++w) s += part[(int64_t)w * pq + e];
   G[(e % p) + (int64_t)ldg * (e / p)] = s;
 }

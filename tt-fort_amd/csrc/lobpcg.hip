@@ -145,9 +145,9 @@ __global__ __launch_bounds__(256) void rr_kernel(int m, int bs, int nx, const do
     }
     __syncthreads();
   }
-  // modified Gram-Schmidt on the columns of Z (twice).  The kernel runs as a
-  // single wave with lane r owning row r (m <= 64): dot products are wave
-  // reductions through lane shuffles.
+  // modified Gram-Schmidt on the columns of Z (twice) by the first wave,
+  // lane r owning row r (m <= 64): dot products are wave reductions through
+  // lane shuffles.
   for (int pass = 0; pass < 2; ++pass) {
     for (int c = 0; c < bs; ++c) {
       for (int j = 0; j < c; ++j) {
@@ -289,7 +289,7 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
   TTF_HIP(hipMemsetAsync(AP, 0, sizeof(double) * colsz * bs, st));
   for (int attempt = 0; attempt < 3; ++attempt) {
     tsgram(N, bs, bs, X, ld, X, ld, Gm, bs, st);
-    hipLaunchKernelGGL(svqb_kernel, dim3(1), dim3(64), 0, st, bs, Gm, Tm, maskW);
+    hipLaunchKernelGGL(svqb_kernel, dim3(1), dim3(256), 0, st, bs, Gm, Tm, maskW);
     mul_small(N, bs, bs, X, ld, Tm, bs, Wt, ld, 0.0, st);
     copy_cols(Wt, ld, X, ld, N, bs, st);
     std::vector<int> hm(bs);
@@ -311,7 +311,7 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
     std::vector<int> ones(kMaxM, 1);
     TTF_HIP(hipMemcpyAsync(imask, ones.data(), sizeof(int) * kMaxM, hipMemcpyHostToDevice, st));
     tsgram(N, bs, bs, X, ld, AX, ld, Hm, bs, st);
-    hipLaunchKernelGGL(rr_kernel, dim3(1), dim3(64), 0, st, bs, bs, bs, Hm, imask, theta, CZ, maskP,
+    hipLaunchKernelGGL(rr_kernel, dim3(1), dim3(256), 0, st, bs, bs, bs, Hm, imask, theta, CZ, maskP,
                        amax);
     mul_small(N, bs, bs, X, ld, CZ, bs, Sn, ld, 0.0, st);
     mul_small(N, bs, bs, AX, ld, CZ, bs, ASn, ld, 0.0, st);
@@ -350,7 +350,7 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
     }
     // SVQB on W
     tsgram(N, bs, bs, W, ld, W, ld, Gm, bs, st);
-    hipLaunchKernelGGL(svqb_kernel, dim3(1), dim3(64), 0, st, bs, Gm, Tm, maskW);
+    hipLaunchKernelGGL(svqb_kernel, dim3(1), dim3(256), 0, st, bs, Gm, Tm, maskW);
     mul_small(N, bs, bs, W, ld, Tm, bs, Wt, ld, 0.0, st);
     copy_cols(Wt, ld, W, ld, N, bs, st);
     op.apply(bs, W, ld, AW, ld, st);
@@ -360,7 +360,7 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
     if (have_p) TTF_HIP(hipMemcpyAsync(imask + 2 * bs, maskP, sizeof(int) * bs, hipMemcpyDeviceToDevice, st));
     const int m = nb * bs;
     tsgram(N, m, m, S, ld, AS, ld, Hm, m, st);
-    hipLaunchKernelGGL(rr_kernel, dim3(1), dim3(64), 0, st, m, bs, bs, Hm, imask, theta, CZ, maskP,
+    hipLaunchKernelGGL(rr_kernel, dim3(1), dim3(256), 0, st, m, bs, bs, Hm, imask, theta, CZ, maskP,
                        amax);
     mul_small(N, m, 2 * bs, S, ld, CZ, m, Sn, ld, 0.0, st);
     mul_small(N, m, 2 * bs, AS, ld, CZ, m, ASn, ld, 0.0, st);
