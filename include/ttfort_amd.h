@@ -205,6 +205,73 @@ int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, const double* 
                 int kickrank, int nswp, int max_full_size, int verb, double* result,
                 int64_t result_cap, int64_t* result_size, ttf_eigb_stats* stats);
 
+/* ------------------------------------------------- AMEn building blocks */
+/* Block-Jacobi preconditioner of one local step (tt_linalg.f90:363-495,
+ * djac_gen bind(c)): ptype 'c' rx1*rx2 inverted n x n blocks (layout
+ * (n, n, rx2, rx1)), 'l' rx2 blocks of (rx1 n)^2, 'r' rx1 blocks of
+ * (n rx2)^2; blocks up to 128.  Device pointers; jacs holds
+ * ttf_djac_size() doubles.  A singular block is an error. */
+int ttf_djac_gen(char ptype, int rx1, int n, int rx2, int ra1, int ra2, const double* phi1,
+                 const double* A, const double* phi2, double* jacs);
+int64_t ttf_djac_size(char ptype, int rx1, int n, int rx2);
+/* y = P x with the blocks of ttf_djac_gen (tt_linalg.f90:497-533,
+ * djac_apply bind(c)); y may alias x.  Device pointers. */
+int ttf_djac_apply(char ptype, int rx1, int n, int rx2, const double* jacs, const double* x, double* y);
+/* Right-preconditioned restarted GMRES for the local system B P sol = rhs
+ * (tt_linalg.f90:651-845, dgmresr_hh_fort bind(c)): at most niters cycles
+ * of GMRES(min(nrestart, N, 63)), stopping when the residual estimate falls
+ * below tol times the initial residual; sol starts at zero.  Arnoldi by
+ * classical Gram-Schmidt applied twice.  iters (optional, 2 ints) returns
+ * (last cycle, Krylov dimension), relres (optional) the final estimate.
+ * Device pointers; synchronous. */
+int ttf_dgmresr(const double* phi1, const double* A, const double* phi2, const double* rhs, int rx1,
+                int n, int rx2, int ra1, int ra2, int nrestart, double tol, int niters, char ptype,
+                const double* jacs, double* sol, int* iters, double* relres);
+/* Pivoted incomplete Cholesky of the Gram matrix of the n items of dA
+ * (columns of an m x n dA for trans 'N', rows of an n x m dA for 'T'):
+ * up to numvec orthonormal directions du (n x numvec), count in
+ * *real_numvec, eigenvalues in lam (numvec, optional) (tt_linalg.f90:540-643,
+ * duchol_fort bind(c)).  Device pointers; synchronous. */
+int ttf_duchol(char trans, int m, int n, const double* dA, int numvec, double* du, double* lam,
+               int* real_numvec);
+/* Phi1 A x with the right interface open: y(c1, i, b2, a2), an (ry1 n) x
+ * (rx2 ra2) block, the contraction tt_linalg.f90:242-256 (dbfun3_left) is
+ * meant to compute.  Device pointers. */
+int ttf_dbfun3_left(int rx1, int m, int rx2, int ry1, int n, int ry2, int ra1, int ra2,
+                    const double* phi1, const double* A, const double* x, double* y);
+/* A Phi2 x with the left interface open: y(b1, a1, i, c2), an (rx1 ra1) x
+ * (n ry2) block (tt_linalg.f90:210-240, dbfun3_right).  Device pointers. */
+int ttf_dbfun3_right(int rx1, int m, int rx2, int ry1, int n, int ry2, int ra1, int ra2,
+                     const double* A, const double* phi2, const double* x, double* y);
+
+typedef struct {
+  int sweeps;            /* sweeps performed */
+  long total_matvecs;    /* local matvecs (vectors): residuals, GMRES, truncation */
+  double last_res_max;   /* largest local residual of the last sweep */
+  double last_err_max;   /* largest relative local correction of the last sweep */
+  int max_rank;          /* largest TT rank at exit */
+  double seconds;        /* wall time of the call */
+  double solve_seconds;  /* local residuals and GMRES solves */
+  double trunc_seconds;  /* SVD, residual-based truncation, kick, core moves */
+  double phi_seconds;    /* interface updates */
+} ttf_amen_stats;
+
+/* AMEn solver for A x = y in TT format, argument layout of tt_amen_solve
+ * (tt_adapt_als.f90:513-525): d modes, n(d) = m(d) mode sizes, ry(d+1) and
+ * crY the right-hand side, ra(d+1) and crA the operator, crX0 with ranks
+ * rx(d+1) (in/out) the initial guess, eps the tolerance.  The reference's
+ * optional arguments are explicit: kickrank (4; -1 truncates without kick,
+ * 0 only re-orthogonalises), nswp (20), verb (1), prec ('n', 'c', 'l',
+ * 'r'), nrestart (40), niters (2).  rmax > 0 caps the TT ranks (not in the
+ * reference; <= 0 means a capacity of 256).  The result cores (the
+ * reference's tt_adapt_als::result_core) go to `result` (capacity
+ * result_cap doubles; size in *result_size).  Host pointers;
+ * device-resident run. */
+int ttf_tt_amen_solve(int d, const int* n, const int* m, const int* ry, const int* ra, const double* crA,
+                      const double* crY, const double* crX0, int* rx, double eps, int kickrank, int nswp,
+                      int verb, char prec, int nrestart, int niters, int rmax, double* result,
+                      int64_t result_cap, int64_t* result_size, ttf_amen_stats* stats);
+
 /* ------------------------------------------ reference-compatible symbols */
 /* tt_linalg.f90:127-159, dbfun3 bind(c).  Host pointers; res1/res2 ignored. */
 void dbfun3(const ttf_fint* rx1, const ttf_fint* m, const ttf_fint* rx2,

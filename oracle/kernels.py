@@ -340,6 +340,42 @@ def djac_apply(ptype, rx1, n, rx2, jacs, x):
     raise ValueError(ptype)
 
 
+# ---------------------------------------------------------------- duchol
+def duchol(trans, m, n, dA, numvec):
+    """tt_linalg.f90:540-643 (duchol_fort): pivoted incomplete Cholesky of
+    the Gram matrix of the n items of dA (columns of an m x n dA for trans
+    'N', rows of an n x m dA for 'T'), with the reference's per-step
+    re-orthogonalisation (dgeqrf + dorgqr, :613-617) and eigen-rotation of
+    the accumulated directions (:619-631).
+
+    Returns (du (n x real), lam (numvec), real).
+    """
+    A = F(dA, m, n) if trans in "nN" else F(dA, n, m)
+    ddiag = (A * A).sum(0) if trans in "nN" else (A * A).sum(1)                # :576-586
+    du = np.zeros((n, numvec))
+    lam = np.zeros(numvec)
+    real = numvec
+    for cnt in range(numvec):
+        maxi = int(np.argmax(np.abs(ddiag)))                                    # :590 idamax
+        du[:, cnt] = A.T @ A[:, maxi] if trans in "nN" else A @ A[maxi, :]      # :600-604
+        for j in range(cnt):                                                    # :606-609
+            du[:, cnt] -= lam[j] * du[maxi, j] * du[:, j]
+        maxa = np.sqrt(abs(du[maxi, cnt]))
+        if maxa < 1e-300:                                                       # :611-613
+            real = cnt + 1
+            break
+        du[:, cnt] /= maxa
+        ddiag = ddiag - du[:, cnt] ** 2
+        Q, Rr = np.linalg.qr(du[:, :cnt + 1])                                   # :620-624
+        dx = Rr[:, cnt].copy()
+        du[:, :cnt + 1] = Q
+        dv = np.outer(dx, dx) + np.diag(lam[:cnt + 1])                          # :627-632
+        w, V = np.linalg.eigh(dv)                                               # :634 dsyev
+        lam[:cnt + 1] = w
+        du[:, :cnt + 1] = du[:, :cnt + 1] @ V                                   # :637-638
+    return du[:, :real], lam, real
+
+
 # ---------------------------------------------------------------- GMRES
 def dgmresr_hh(matvec, rhs, nrestart, tol, niters, prec=None):
     """tt_linalg.f90:651-845: right-preconditioned restarted GMRES with

@@ -1,0 +1,129 @@
+"""GPU parity of the AMEn path (through the C ABI) against the oracle and
+against exact solutions.
+
+* block-Jacobi blocks and their application: 1e-10 relative to the oracle;
+* local GMRES: the returned correction meets the requested relative
+  residual and agrees with the oracle's Householder GMRES to the tolerance;
+* duchol kick directions: eigenvalues to 1e-9 relative and the spanned
+  projector to 1e-9 (eigenvectors are defined up to sign);
+* the half contractions: 1e-12 relative;
+* tt_amen_solve on the Dirichlet Laplacian: the TT solution against the
+  tensor-DST exact solution (relative error <= 1e-6 for eps = 1e-8) and the
+  full residual against the dense operator.
+"""
+import numpy as np
+import pytest
+
+import mpo
+from oracle import kernels as K
+from oracle import operators as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _local_problem(rng, rx1, n, rx2, ra1, ra2, spd=True):
+    P1 = rng.standard_normal((rx1, rx1, ra1))
+    P2 = rng.standard_normal((rx2, ra2, rx2))
+    if spd:   # symmetric interfaces with a dominant identity part
+        P1 = 0.5 * (P1 + np.transpose(P1, (1, 0, 2)))
+        P2 = 0.5 * (P2 + np.transpose(P2, (2, 1, 0)))
+        P1[:, :, 0] = np.eye(rx1) * 2 + 0.1 * P1[:, :, 0]
+        P2[:, 0, :] = np.eye(rx2) * 2 + 0.1 * P2[:, 0, :]
+    A = 0.2 * rng.standard_normal((ra1, n, n, ra2))
+    A[0, :, :, 0] += 4 * np.eye(n) - np.eye(n, k=1) - np.eye(n, k=-1)
+    return K._flat(P1), K._flat(A), K._flat(P2)
+
+
+@pytest.mark.parametrize("ptype,rx1,n,rx2", [("c", 5, 6, 4), ("c", 16, 64, 16), ("l", 4, 5, 3), ("r", 3, 5, 6)])
+def test_djac(ttlib, ptype, rx1, n, rx2):
+    import ttfort
+    rng = np.random.default_rng(rx1 * 100 + n)
+    ra1, ra2 = 3, 2
+    phi1, A, phi2 = _local_problem(rng, rx1, n, rx2, ra1, ra2)
+    J = ttfort.djac_gen(ptype, rx1, n, rx2, ra1, ra2, phi1, A, phi2)
+    Jr = K.djac_gen(ptype, rx1, n, rx2, ra1, ra2, phi1, A, phi2)
+    assert np.allclose(J, Jr, rtol=1e-10, atol=1e-12 * np.abs(Jr).max())
+    x = rng.standard_normal(rx1 * n * rx2)
+    y = ttfort.djac_apply(ptype, rx1, n, rx2, J, x)
+    assert np.allclose(y, K.djac_apply(ptype, rx1, n, rx2, Jr, x), rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("ptype", ["n", "c"])
+def test_gmres(ttlib, ptype):
+    import ttfort
+    rng = np.random.default_rng(5)
+    rx1, n, rx2, ra1, ra2 = 6, 8, 5, 2, 2
+    phi1, A, phi2 = _local_problem(rng, rx1, n, rx2, ra1, ra2)
+    N = rx1 * n * rx2
+    rhs = rng.standard_normal(N)
+    jacs = ttfort.djac_gen(ptype, rx1, n, rx2, ra1, ra2, phi1, A, phi2) if ptype != "n" else None
+    tol = 1e-10
+    sol, it, relres = ttfort.gmres(phi1, A, phi2, rhs, rx1, n, rx2, ra1, ra2, 40, tol, 4, ptype, jacs)
+    B = lambda v: K.dbfun3(rx1, n, rx2, rx1, n, rx2, ra1, ra2, phi1, A, phi2, v)
+    P = (lambda v: K.djac_apply(ptype, rx1, n, rx2, jacs, v)) if ptype != "n" else None
+    Pv = P(sol) if P else sol
+    true_res = np.linalg.norm(B(Pv) - rhs) / np.linalg.norm(rhs)
+    assert relres < tol and true_res < 10 * tol, (relres, true_res)
+    ref, info = K.dgmresr_hh(B, rhs, 40, tol, 4, prec=P)
+    assert np.linalg.norm(sol - ref) <= 1e-7 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("trans,m,n,numvec", [("N", 30, 50, 4), ("T", 40, 64, 4), ("N", 7, 5, 3)])
+def test_duchol(ttlib, trans, m, n, numvec):
+    import ttfort
+    rng = np.random.default_rng(m + n)
+    # items graded in size so that the pivot order is well defined
+    if trans == "N":
+        M = rng.standard_normal((m, n)) * np.logspace(0, -3, n)[None, :]
+    else:
+        M = rng.standard_normal((n, m)) * np.logspace(0, -3, n)[:, None]
+    dA = M.reshape(-1, order="F")
+    du, lam, real = ttfort.duchol(trans, m, n, dA, numvec)
+    du_r, lam_r, real_r = K.duchol(trans, m, n, dA, numvec)
+    assert real == real_r
+    assert np.allclose(lam[:real], lam_r[:real], rtol=1e-9)
+    assert np.allclose(du @ du.T, du_r @ du_r.T, atol=1e-9)
+
+
+@pytest.mark.parametrize("side", ["L", "R"])
+def test_bfun3_half(ttlib, side):
+    import ttfort
+    rng = np.random.default_rng(9)
+    rx1, m, rx2, ry1, n, ry2, ra1, ra2 = 5, 3, 4, 5, 3, 4, 3, 2
+    A = rng.standard_normal(ra1 * n * m * ra2)
+    x = rng.standard_normal(rx1 * m * rx2)
+    if side == "L":
+        phi = rng.standard_normal(ry1 * rx1 * ra1)
+        y = ttfort.bfun3_half("L", rx1, m, rx2, ry1, n, ry2, ra1, ra2, phi, A, x)
+        ref = K.dbfun3_left_intended(rx1, m, rx2, ry1, n, ry2, ra1, ra2, phi, A, x)
+    else:
+        phi = rng.standard_normal(rx2 * ra2 * ry2)
+        y = ttfort.bfun3_half("R", rx1, m, rx2, ry1, n, ry2, ra1, ra2, phi, A, x)
+        ref = K.dbfun3_right(rx1, m, rx2, ry1, n, ry2, ra1, ra2, A, phi, x)
+    assert np.allclose(y, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+def _cores(cr, r, n, m=None):
+    out, off = [], 0
+    for k in range(len(n)):
+        sz = int(r[k] * n[k] * (m[k] if m is not None else 1) * r[k + 1])
+        out.append(cr[off:off + sz])
+        off += sz
+    return out
+
+
+@pytest.mark.parametrize("d,n,prec,kick", [(4, 8, "n", 4), (4, 8, "c", 4), (3, 12, "n", 2), (5, 10, "c", 4)])
+def test_amen_laplace(ttlib, d, n, prec, kick):
+    import ttfort
+    crA, ra, nn = mpo.laplace_dd(d, n)
+    crY, ry = mpo.ones_tt(d, n)
+    crX, rx = mpo.random_tt(d, n, 2, seed=1)
+    eps = 1e-8
+    res, rx_out, st = ttfort.tt_amen_solve(nn, ra, crA, ry, crY, rx, crX, eps, kickrank=kick, prec=prec, nswp=20)
+    x = O.tt_full(_cores(res, rx_out, nn), nn, rx_out)
+    y = np.ones(n ** d)
+    x_ref = O.laplace_dst_solve(d, n, y)
+    assert np.linalg.norm(x - x_ref) <= 1e-6 * np.linalg.norm(x_ref), (np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref), st)
+    Ax = O.mpo_apply_full(_cores(crA, ra, nn, nn), ra, nn, nn, x)
+    assert np.linalg.norm(Ax - y) <= 1e-6 * np.linalg.norm(y)
+    assert st["sweeps"] >= 1 and st["last_res_max"] < eps

@@ -1,0 +1,518 @@
+// Local building blocks of the AMEn linear solver on the device:
+//   * block-Jacobi preconditioner blocks and their application
+//     (tt_linalg.f90:363-533, djac_gen / djac_apply);
+//   * right-preconditioned restarted GMRES for the local system
+//     (tt_linalg.f90:651-845, dgmresr_hh_fort);
+//   * the pivoted incomplete Cholesky that picks the enrichment ("kick")
+//     directions (tt_linalg.f90:540-643, duchol_fort);
+//   * the half contractions of the local operator used to build the kick
+//     block (tt_linalg.f90:210-256, dbfun3_right / dbfun3_left).
+#include "amen.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace ttf {
+
+namespace {
+
+int grid_for(int64_t n) { return (int)std::min<int64_t>(std::max<int64_t>(ceil_div(n, 256), 1), 16384); }
+
+__global__ void scale_copy_kernel(int64_t n, double alpha, const double* __restrict__ x, double* __restrict__ y) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    y[e] = alpha * x[e];
+}
+
+// y = a x + b y
+__global__ void axpby_kernel(int64_t n, double a, const double* __restrict__ x, double b, double* y) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    y[e] = a * x[e] + b * y[e];
+}
+
+// ---------------------------------------------------------------- djac_gen
+// Diagonal blocks of the local matrix restricted as the reference does it
+// (tt_linalg.f90:397-491).  d1(b1, a1) = Phi1(b1, b1, a1), d2(b2, a2) =
+// Phi2(b2, a2, b2); A(a1, i, j, a2).
+//  'c': block (b1, b2) of size n: M(i, j) = sum d1(b1,a1) A(a1,i,j,a2) d2(b2,a2),
+//       stored at (b2 + rx2 b1) n^2 (layout (n, n, rx2, rx1));
+//  'l': block b2 of size rx1 n: M((c1,i),(b1,j)) = sum Phi1(c1,b1,a1) A d2(b2,a2);
+//  'r': block b1 of size n rx2: M((i,c2),(j,b2)) = sum d1(b1,a1) A Phi2(b2,a2,c2).
+__global__ void jac_blocks_kernel(char type, int rx1, int n, int rx2, int ra1, int ra2,
+                                  const double* __restrict__ Phi1, const double* __restrict__ A,
+                                  const double* __restrict__ Phi2, double* __restrict__ M) {
+  int64_t bs, nblk;
+  if (type == 'c') { bs = n; nblk = (int64_t)rx1 * rx2; }
+  else if (type == 'l') { bs = (int64_t)rx1 * n; nblk = rx2; }
+  else { bs = (int64_t)n * rx2; nblk = rx1; }
+  const int64_t total = bs * bs * nblk;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t row = e % bs, col = (e / bs) % bs, blk = e / (bs * bs);
+    double s = 0.0;
+    if (type == 'c') {
+      const int i = (int)row, j = (int)col, b2 = (int)(blk % rx2), b1 = (int)(blk / rx2);
+      for (int a2 = 0; a2 < ra2; ++a2) {
+        const double d2 = Phi2[b2 + (int64_t)rx2 * (a2 + (int64_t)ra2 * b2)];
+        double t = 0.0;
+        for (int a1 = 0; a1 < ra1; ++a1)
+          t += Phi1[b1 + (int64_t)rx1 * (b1 + (int64_t)rx1 * a1)] *
+               A[a1 + (int64_t)ra1 * (i + (int64_t)n * (j + (int64_t)n * a2))];
+        s += t * d2;
+      }
+    } else if (type == 'l') {
+      const int c1 = (int)(row % rx1), i = (int)(row / rx1), b1 = (int)(col % rx1), j = (int)(col / rx1);
+      const int b2 = (int)blk;
+      for (int a2 = 0; a2 < ra2; ++a2) {
+        const double d2 = Phi2[b2 + (int64_t)rx2 * (a2 + (int64_t)ra2 * b2)];
+        double t = 0.0;
+        for (int a1 = 0; a1 < ra1; ++a1)
+          t += Phi1[c1 + (int64_t)rx1 * (b1 + (int64_t)rx1 * a1)] *
+               A[a1 + (int64_t)ra1 * (i + (int64_t)n * (j + (int64_t)n * a2))];
+        s += t * d2;
+      }
+    } else {
+      const int i = (int)(row % n), c2 = (int)(row / n), j = (int)(col % n), b2 = (int)(col / n);
+      const int b1 = (int)blk;
+      for (int a2 = 0; a2 < ra2; ++a2) {
+        const double p2 = Phi2[b2 + (int64_t)rx2 * (a2 + (int64_t)ra2 * c2)];
+        double t = 0.0;
+        for (int a1 = 0; a1 < ra1; ++a1)
+          t += Phi1[b1 + (int64_t)rx1 * (b1 + (int64_t)rx1 * a1)] *
+               A[a1 + (int64_t)ra1 * (i + (int64_t)n * (j + (int64_t)n * a2))];
+        s += t * p2;
+      }
+    }
+    M[e] = s;
+  }
+}
+
+// In-place inverse of `count` n x n column-major matrices (n <= 128) by
+// Gauss-Jordan elimination with partial (row) pivoting in LDS, one
+// work-group per matrix; the row interchanges are undone as column
+// interchanges at the end.  A zero pivot leaves the block's info flag set
+// and the block's entries non-finite.
+constexpr int kInvMaxN = 128;
+
+__global__ __launch_bounds__(256) void batched_inverse_kernel(int n, double* __restrict__ M,
+                                                              int* __restrict__ info) {
+  extern __shared__ double a[];   // column-major, ld = n + 1
+  __shared__ int piv[kInvMaxN];
+  __shared__ double rmax[256];
+  __shared__ int rarg[256];
+  const int ld = n + 1, tid = threadIdx.x, nt = blockDim.x;
+  double* g = M + (int64_t)blockIdx.x * n * n;
+  for (int e = tid; e < n * n; e += nt) a[(e % n) + ld * (e / n)] = g[e];
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    // pivot row: largest |a(i, k)|, i >= k (first index among ties)
+    double best = -1.0;
+    int arg = k;
+    for (int i = k + tid; i < n; i += nt) {
+      const double v = fabs(a[i + ld * k]);
+      if (v > best) { best = v; arg = i; }
+    }
+    rmax[tid] = best;
+    rarg[tid] = arg;
+    __syncthreads();
+    for (int o = nt / 2; o > 0; o >>= 1) {
+      if (tid < o) {
+        const double v2 = rmax[tid + o];
+        const int a2 = rarg[tid + o];
+        if (v2 > rmax[tid] || (v2 == rmax[tid] && a2 < rarg[tid])) { rmax[tid] = v2; rarg[tid] = a2; }
+      }
+      __syncthreads();
+    }
+    const int p = rarg[0];
+    if (tid == 0) piv[k] = p;
+    if (p != k)
+      for (int j = tid; j < n; j += nt) {
+        const double t = a[k + ld * j];
+        a[k + ld * j] = a[p + ld * j];
+        a[p + ld * j] = t;
+      }
+    __syncthreads();
+    const double pv = a[k + ld * k];
+    if (pv == 0.0 && tid == 0) info[blockIdx.x] = k + 1;
+    const double inv = 1.0 / pv;
+    __syncthreads();
+    // row k scaled, with its pivot position becoming 1/pivot
+    for (int j = tid; j < n; j += nt) a[k + ld * j] = (j == k) ? inv : a[k + ld * j] * inv;
+    __syncthreads();
+    // eliminate column k from the other rows; column k becomes -f / pivot
+    for (int e = tid; e < n * n; e += nt) {
+      const int i = e % n, j = e / n;
+      if (i == k) continue;
+      const double f = a[i + ld * k];
+      if (j == k) continue;
+      a[i + ld * j] -= f * a[k + ld * j];
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += nt)
+      if (i != k) a[i + ld * k] = -a[i + ld * k] * inv;
+    __syncthreads();
+  }
+  for (int k = n - 1; k >= 0; --k) {
+    const int p = piv[k];
+    if (p != k)
+      for (int i = tid; i < n; i += nt) {
+        const double t = a[i + ld * k];
+        a[i + ld * k] = a[i + ld * p];
+        a[i + ld * p] = t;
+      }
+    __syncthreads();
+  }
+  for (int e = tid; e < n * n; e += nt) g[e] = a[(e % n) + ld * (e / n)];
+}
+
+// y(blk) = J(blk) x(blk) for all blocks: block b has size bs, matrix at
+// J + b bs^2 (column-major), vector entries at off(b) + s k with
+// off(b) = (b % nb0) s0 + (b / nb0) s1.
+__global__ void jac_apply_kernel(int bs, int64_t nblk, int nb0, int64_t s0, int64_t s1, int64_t s,
+                                 const double* __restrict__ J, const double* __restrict__ x,
+                                 double* __restrict__ y) {
+  const int64_t total = (int64_t)bs * nblk;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % bs);
+    const int64_t b = e / bs;
+    const int64_t off = (b % nb0) * s0 + (b / nb0) * s1;
+    const double* Jb = J + b * (int64_t)bs * bs + i;
+    double acc = 0.0;
+    for (int j = 0; j < bs; ++j) acc += Jb[(int64_t)bs * j] * x[off + s * j];
+    y[off + s * i] = acc;
+  }
+}
+
+void jac_layout(char type, int rx1, int n, int rx2, int* bs, int64_t* nblk, int* nb0, int64_t* s0,
+                int64_t* s1, int64_t* s) {
+  if (type == 'c') {   // block (b1, b2) = x(b1, :, b2): b = b2 + rx2 b1
+    *bs = n; *nblk = (int64_t)rx1 * rx2; *nb0 = rx2; *s0 = (int64_t)rx1 * n; *s1 = 1; *s = rx1;
+  } else if (type == 'l') {   // block b2 = x(:, :, b2)
+    *bs = rx1 * n; *nblk = rx2; *nb0 = rx2; *s0 = (int64_t)rx1 * n; *s1 = 0; *s = 1;
+  } else {   // 'r': block b1 = x(b1, :, :)
+    *bs = n * rx2; *nblk = rx1; *nb0 = rx1; *s0 = 1; *s1 = 0; *s = rx1;
+  }
+}
+
+}  // namespace
+
+char norm_ptype(char p) { return (char)((p >= 'A' && p <= 'Z') ? p - 'A' + 'a' : p); }
+
+size_t jac_size(char type, int rx1, int n, int rx2) {
+  type = norm_ptype(type);
+  if (type == 'n') return 0;
+  int bs, nb0;
+  int64_t nblk, s0, s1, s;
+  jac_layout(type, rx1, n, rx2, &bs, &nblk, &nb0, &s0, &s1, &s);
+  return (size_t)bs * bs * nblk;
+}
+
+void jac_gen(char type, int rx1, int n, int rx2, int ra1, int ra2, const double* Phi1, const double* A,
+             const double* Phi2, double* jacs, hipStream_t st) {
+  type = norm_ptype(type);
+  if (type == 'n') return;
+  TTF_REQUIRE(type == 'c' || type == 'l' || type == 'r', "djac_gen: unknown ptype '%c'", type);
+  int bs, nb0;
+  int64_t nblk, s0, s1, s;
+  jac_layout(type, rx1, n, rx2, &bs, &nblk, &nb0, &s0, &s1, &s);
+  TTF_REQUIRE(bs <= kInvMaxN, "djac_gen: block size %d exceeds %d for ptype '%c'", bs, kInvMaxN, type);
+  TTF_REQUIRE(nblk <= 0x7fffffff, "djac_gen: too many blocks");
+  const int64_t total = (int64_t)bs * bs * nblk;
+  hipLaunchKernelGGL(jac_blocks_kernel, dim3(grid_for(total)), dim3(256), 0, st, type, rx1, n, rx2, ra1, ra2,
+                     Phi1, A, Phi2, jacs);
+  TTF_HIP(hipGetLastError());
+  int* info = (int*)arena(kArenaSmall, sizeof(int) * (size_t)nblk);
+  TTF_HIP(hipMemsetAsync(info, 0, sizeof(int) * (size_t)nblk, st));
+  const size_t lds = sizeof(double) * (size_t)bs * (bs + 1);
+  static bool attr = false;
+  if (!attr) {
+    TTF_HIP(hipFuncSetAttribute((const void*)batched_inverse_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(sizeof(double) * kInvMaxN * (kInvMaxN + 1))));
+    attr = true;
+  }
+  hipLaunchKernelGGL(batched_inverse_kernel, dim3((unsigned)nblk), dim3(256), lds, st, bs, jacs, info);
+  TTF_HIP(hipGetLastError());
+  std::vector<int> hinfo(nblk);
+  TTF_HIP(hipMemcpyAsync(hinfo.data(), info, sizeof(int) * (size_t)nblk, hipMemcpyDeviceToHost, st));
+  TTF_HIP(hipStreamSynchronize(st));
+  for (int64_t b = 0; b < nblk; ++b)
+    TTF_REQUIRE(hinfo[b] == 0, "djac_gen: block %lld is singular (zero pivot %d)", (long long)b, hinfo[b]);
+}
+
+void jac_apply(char type, int rx1, int n, int rx2, const double* jacs, const double* x, double* y,
+               hipStream_t st) {
+  type = norm_ptype(type);
+  const int64_t N = (int64_t)rx1 * n * rx2;
+  if (type == 'n') {
+    if (x != y) TTF_HIP(hipMemcpyAsync(y, x, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+    return;
+  }
+  TTF_REQUIRE(type == 'c' || type == 'l' || type == 'r', "djac_apply: unknown ptype '%c'", type);
+  int bs, nb0;
+  int64_t nblk, s0, s1, s;
+  jac_layout(type, rx1, n, rx2, &bs, &nblk, &nb0, &s0, &s1, &s);
+  const double* xin = x;
+  if (x == y) {   // in place: stage the input
+    double* t = (double*)arena(kArenaSolver2, sizeof(double) * N);
+    TTF_HIP(hipMemcpyAsync(t, x, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+    xin = t;
+  }
+  hipLaunchKernelGGL(jac_apply_kernel, dim3(grid_for((int64_t)bs * nblk)), dim3(256), 0, st, bs, nblk, nb0, s0,
+                     s1, s, jacs, xin, y);
+  TTF_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- GMRES
+GmresStats gmres_solve(const LocalOp& op, const JacPrec& P, const double* rhs, double* sol, int nrestart,
+                       double tol, int niters, hipStream_t st) {
+  GmresStats gs;
+  const int64_t N = op.N();
+  const int m = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)nrestart, N, (int64_t)63}));
+  // device workspace: V (N x (m+1)) | u | t
+  double* V = (double*)arena(kArenaSolver, sizeof(double) * (size_t)N * (m + 3));
+  double* u = V + (size_t)N * (m + 1);
+  double* t = u + N;
+  double* hb = (double*)arena(kArenaSmall, sizeof(double) * (2 * 64 + 8));
+  TTF_HIP(hipMemsetAsync(sol, 0, sizeof(double) * N, st));
+  const int gN = grid_for(N);
+  std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), y(m), hh(2 * 64 + 8);
+  double nrmrhs = 0.0, curres = 1.0;
+  auto apply_BP = [&](const double* v, double* out) {
+    const double* in = v;
+    if (P.type != 'n') {
+      jac_apply(P.type, P.rx1, P.n, P.rx2, P.jacs, v, t, st);
+      in = t;
+    }
+    op.apply(1, in, N, out, N, st);
+    ++gs.matvecs;
+  };
+  for (int it = 0; it < niters; ++it) {
+    gs.it = it;
+    // r = rhs - B P sol into V(:, 0)
+    if (it == 0) {
+      TTF_HIP(hipMemcpyAsync(V, rhs, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+    } else {
+      apply_BP(sol, V);
+      hipLaunchKernelGGL(axpby_kernel, dim3(gN), dim3(256), 0, st, N, 1.0, rhs, -1.0, V);
+      TTF_HIP(hipGetLastError());
+    }
+    tsgram((int)N, 1, 1, V, N, V, N, hb, 1, st);
+    TTF_HIP(hipMemcpyAsync(hh.data(), hb, sizeof(double), hipMemcpyDeviceToHost, st));
+    TTF_HIP(hipStreamSynchronize(st));
+    const double beta = std::sqrt(std::max(hh[0], 0.0));
+    if (it == 0) nrmrhs = beta;
+    if (beta == 0.0) { curres = 0.0; break; }
+    hipLaunchKernelGGL(scale_copy_kernel, dim3(gN), dim3(256), 0, st, N, 1.0 / beta, V, V);
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int jj = 0;
+    for (int j = 0; j < m; ++j) {
+      apply_BP(V + (size_t)N * j, u);
+      const int k = j + 1;
+      // classical Gram-Schmidt, twice, against V(:, 0..j)
+      for (int pass = 0; pass < 2; ++pass) {
+        double* h = hb + 64 * pass;
+        tsgram((int)N, k, 1, V, N, u, N, h, k, st);
+        GemmOp<double> gm = colmajor<double>('N', 'N', (int)N, 1, k, -1.0, V, N, h, k, 1.0, u, N);
+        gemm_launch(gm, st);
+      }
+      tsgram((int)N, 1, 1, u, N, u, N, hb + 128, 1, st);
+      TTF_HIP(hipMemcpyAsync(hh.data(), hb, sizeof(double) * 129, hipMemcpyDeviceToHost, st));
+      TTF_HIP(hipStreamSynchronize(st));
+      double* col = &H[(size_t)(m + 1) * j];
+      for (int i = 0; i < k; ++i) col[i] = hh[i] + hh[64 + i];
+      const double hn = std::sqrt(std::max(hh[128], 0.0));
+      col[k] = hn;
+      for (int i = 0; i < j; ++i) {   // previous rotations
+        const double a = col[i], b = col[i + 1];
+        col[i] = cs[i] * a + sn[i] * b;
+        col[i + 1] = -sn[i] * a + cs[i] * b;
+      }
+      const double r = std::hypot(col[j], col[k]);
+      cs[j] = r > 0.0 ? col[j] / r : 1.0;
+      sn[j] = r > 0.0 ? col[k] / r : 0.0;
+      col[j] = r;
+      col[k] = 0.0;
+      g[k] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      curres = std::fabs(g[k]) / nrmrhs;
+      jj = k;
+      if (curres < tol || hn == 0.0) break;
+      if (j + 1 < m)
+        hipLaunchKernelGGL(scale_copy_kernel, dim3(gN), dim3(256), 0, st, N, 1.0 / hn, u,
+                           V + (size_t)N * (j + 1));
+    }
+    gs.j = jj;
+    // y = R^-1 g (upper triangular jj x jj), sol += V(:, 0..jj-1) y
+    for (int i = jj - 1; i >= 0; --i) {
+      double s = g[i];
+      for (int l = i + 1; l < jj; ++l) s -= H[(size_t)(m + 1) * l + i] * y[l];
+      y[i] = s / H[(size_t)(m + 1) * i + i];
+    }
+    TTF_HIP(hipMemcpyAsync(hb, y.data(), sizeof(double) * jj, hipMemcpyHostToDevice, st));
+    GemmOp<double> gm = colmajor<double>('N', 'N', (int)N, 1, jj, 1.0, V, N, hb, jj, 1.0, sol, N);
+    gemm_launch(gm, st);
+    TTF_HIP(hipStreamSynchronize(st));   // hb is reused by the next restart
+    if (curres < tol) break;
+  }
+  gs.relres = curres;
+  return gs;
+}
+
+// ---------------------------------------------------------------- duchol
+namespace {
+
+// squared column norms of X (m x n, ld m) ('N') or row norms of X (n x m,
+// ld n) ('T') into d (n)
+__global__ void gram_diag_kernel(char trans, int m, int n, const double* __restrict__ X, double* __restrict__ d) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double s = 0.0;
+    if (trans == 'n')
+      for (int k = 0; k < m; ++k) { const double v = X[k + (int64_t)m * i]; s += v * v; }
+    else
+      for (int k = 0; k < m; ++k) { const double v = X[i + (int64_t)n * k]; s += v * v; }
+    d[i] = s;
+  }
+}
+
+__global__ void sub_square_kernel(int n, const double* __restrict__ u, double* d) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) d[i] -= u[i] * u[i];
+}
+
+}  // namespace
+
+int duchol(char trans, int m, int n, const double* dA, int numvec, double* du, double* lam_out,
+           hipStream_t st) {
+  trans = norm_ptype(trans);
+  TTF_REQUIRE(trans == 'n' || trans == 't', "duchol: trans '%c'", trans);
+  TTF_REQUIRE(numvec >= 1 && numvec <= 32, "duchol: numvec=%d outside 1..32", numvec);
+  TTF_REQUIRE(m >= 1 && n >= 1, "duchol: sizes %d x %d", m, n);
+  double* ddiag = (double*)arena(kArenaDriver, sizeof(double) * ((size_t)n * (numvec + 2) + 64 * 64 * 3 + 256));
+  double* tmp = ddiag + n;                      // n x numvec
+  double* small = tmp + (size_t)n * numvec;     // coefficient vectors, eig workspace
+  double* hcoef = small + 64 * 64 * 3;
+  hipLaunchKernelGGL(gram_diag_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, trans, m, n, dA, ddiag);
+  TTF_HIP(hipGetLastError());
+  std::vector<double> hd(n), lam(numvec + 1, 0.0), hv(64 * 64), hw(64);
+  int cnt = 0, real = 0;
+  for (cnt = 0; cnt < numvec; ++cnt) {
+    TTF_HIP(hipMemcpyAsync(hd.data(), ddiag, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    TTF_HIP(hipStreamSynchronize(st));
+    int maxi = 0;
+    double best = -1.0;
+    for (int i = 0; i < n; ++i)
+      if (std::fabs(hd[i]) > best) { best = std::fabs(hd[i]); maxi = i; }
+    double* uc = du + (size_t)n * cnt;
+    // uc = G(:, maxi) with G the Gram matrix of the items
+    GemmOp<double> g{};
+    g.M = n; g.N = 1; g.K = m; g.nb0 = 1; g.nbatch = 1; g.alpha = 1.0; g.beta = 0.0; g.ksplit = 1;
+    g.C = uc; g.sCi = 1; g.sCj = n;
+    if (trans == 'n') {   // G = A^T A, A m x n
+      g.A = dA; g.sAi = m; g.sAk = 1;
+      g.B = dA + (int64_t)m * maxi; g.sBk = 1; g.sBj = m;
+    } else {              // G = A A^T, A n x m
+      g.A = dA; g.sAi = 1; g.sAk = n;
+      g.B = dA + maxi; g.sBk = n; g.sBj = 1;
+    }
+    gemm_launch(g, st);
+    if (cnt > 0) {
+      // uc -= sum_j lam_j du(maxi, j) du(:, j)
+      std::vector<double> hrow(cnt);
+      TTF_HIP(hipMemcpy2DAsync(hrow.data(), sizeof(double), du + maxi, sizeof(double) * n, sizeof(double), cnt,
+                               hipMemcpyDeviceToHost, st));
+      TTF_HIP(hipStreamSynchronize(st));
+      for (int j = 0; j < cnt; ++j) hrow[j] *= lam[j];
+      TTF_HIP(hipMemcpyAsync(hcoef, hrow.data(), sizeof(double) * cnt, hipMemcpyHostToDevice, st));
+      GemmOp<double> s = colmajor<double>('N', 'N', n, 1, cnt, -1.0, du, n, hcoef, cnt, 1.0, uc, n);
+      gemm_launch(s, st);
+    }
+    double piv = 0.0;
+    TTF_HIP(hipMemcpyAsync(&piv, uc + maxi, sizeof(double), hipMemcpyDeviceToHost, st));
+    TTF_HIP(hipStreamSynchronize(st));
+    const double maxa = std::sqrt(std::fabs(piv));
+    if (maxa < 1e-300) {   // exact zero remainder (tt_linalg.f90:604-607)
+      real = cnt + 1;
+      break;
+    }
+    hipLaunchKernelGGL(scale_copy_kernel, dim3(grid_for(n)), dim3(256), 0, st, (int64_t)n, 1.0 / maxa, uc, uc);
+    hipLaunchKernelGGL(sub_square_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, n, uc, ddiag);
+    // reorthogonalisation of uc against the orthonormal du(:, 0..cnt-1): the
+    // R column dx of the QR of du(:, 0..cnt) (tt_linalg.f90:613-617)
+    const int k = cnt + 1;
+    std::vector<double> dx(k, 0.0);
+    if (cnt > 0) {
+      for (int pass = 0; pass < 2; ++pass) {
+        tsgram(n, cnt, 1, du, n, uc, n, small + 64 * pass, cnt, st);
+        GemmOp<double> s = colmajor<double>('N', 'N', n, 1, cnt, -1.0, du, n, small + 64 * pass, cnt, 1.0, uc, n);
+        gemm_launch(s, st);
+      }
+    }
+    tsgram(n, 1, 1, uc, n, uc, n, small + 128, 1, st);
+    std::vector<double> hh(129);
+    TTF_HIP(hipMemcpyAsync(hh.data(), small, sizeof(double) * 129, hipMemcpyDeviceToHost, st));
+    TTF_HIP(hipStreamSynchronize(st));
+    for (int j = 0; j < cnt; ++j) dx[j] = hh[j] + hh[64 + j];
+    dx[cnt] = std::sqrt(std::max(hh[128], 0.0));
+    if (dx[cnt] > 0.0)
+      hipLaunchKernelGGL(scale_copy_kernel, dim3(grid_for(n)), dim3(256), 0, st, (int64_t)n, 1.0 / dx[cnt], uc, uc);
+    // dv = dx dx^T + diag(lam), eigen-decomposition (ascending)
+    for (int i = 0; i < k; ++i)
+      for (int j = 0; j < k; ++j) hv[i + k * j] = dx[i] * dx[j] + (i == j ? lam[i] : 0.0);
+    double* dv = small + 256;
+    double* w = dv + 64 * 64;
+    double* Q = w + 64;
+    TTF_HIP(hipMemcpyAsync(dv, hv.data(), sizeof(double) * k * k, hipMemcpyHostToDevice, st));
+    syev_small(k, 1, dv, w, Q, 0, st);
+    TTF_HIP(hipMemcpyAsync(hw.data(), w, sizeof(double) * k, hipMemcpyDeviceToHost, st));
+    // du(:, 0..cnt) <- du(:, 0..cnt) V
+    GemmOp<double> r = colmajor<double>('N', 'N', n, k, k, 1.0, du, n, Q, k, 0.0, tmp, n);
+    gemm_launch(r, st);
+    TTF_HIP(hipMemcpyAsync(du, tmp, sizeof(double) * (size_t)n * k, hipMemcpyDeviceToDevice, st));
+    TTF_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < k; ++i) lam[i] = hw[i];
+    real = k;
+  }
+  if (lam_out)
+    for (int i = 0; i < numvec; ++i) lam_out[i] = i < (int)lam.size() ? lam[i] : 0.0;
+  return real;
+}
+
+// ---------------------------------------------------------------- half contractions
+void bfun3_right_half(const Shape& s, const double* A, const double* phi2, const double* x, double* y,
+                      hipStream_t st) {
+  const size_t sz1 = (size_t)s.rx1 * s.m * s.ra2 * s.ry2;
+  double* T1 = (double*)arena(kArenaSolver2, sizeof(double) * sz1);
+  bfun3_stage12<double>(s, 1, A, phi2, x, (int64_t)s.rx1 * s.m * s.rx2, T1, y, st);
+}
+
+void bfun3_left_half(const Shape& s, const double* phi1, const double* A, const double* x, double* y,
+                     hipStream_t st) {
+  // T(c1, j, b2, a1) = sum_b1 Phi1(c1, b1, a1) x(b1, j, b2): one GEMM per a1
+  const size_t szT = (size_t)s.ry1 * s.m * s.rx2 * s.ra1;
+  double* T = (double*)arena(kArenaSolver2, sizeof(double) * szT);
+  {
+    GemmOp<double> g{};
+    g.M = s.ry1; g.N = s.m * s.rx2; g.K = s.rx1;
+    g.A = phi1; g.sAi = 1; g.sAk = s.ry1; g.sAb0 = (int64_t)s.ry1 * s.rx1;
+    g.B = x; g.sBk = 1; g.sBj = s.rx1; g.sBb0 = 0;
+    g.C = T; g.sCi = 1; g.sCj = s.ry1; g.sCb0 = (int64_t)s.ry1 * s.m * s.rx2;
+    g.nb0 = s.ra1; g.nbatch = s.ra1; g.alpha = 1.0; g.beta = 0.0; g.ksplit = 1;
+    gemm_launch(g, st);
+  }
+  // y(c1, i, b2, a2) = sum_{a1, j} T(c1, j, b2, a1) A(a1, i, j, a2): batched
+  // over (b2, a2), accumulated over a1
+  for (int a1 = 0; a1 < s.ra1; ++a1) {
+    GemmOp<double> g{};
+    g.M = s.ry1; g.N = s.n; g.K = s.m;
+    g.A = T + (size_t)a1 * s.ry1 * s.m * s.rx2; g.sAi = 1; g.sAk = s.ry1;
+    g.sAb0 = (int64_t)s.ry1 * s.m; g.sAb1 = 0;
+    g.B = A + a1; g.sBk = (int64_t)s.ra1 * s.n; g.sBj = s.ra1;
+    g.sBb0 = 0; g.sBb1 = (int64_t)s.ra1 * s.n * s.m;
+    g.C = y; g.sCi = 1; g.sCj = s.ry1; g.sCb0 = (int64_t)s.ry1 * s.n; g.sCb1 = (int64_t)s.ry1 * s.n * s.rx2;
+    g.nb0 = s.rx2; g.nbatch = s.rx2 * s.ra2; g.alpha = 1.0; g.beta = a1 == 0 ? 0.0 : 1.0; g.ksplit = 1;
+    gemm_launch(g, st);
+  }
+}
+
+}  // namespace ttf

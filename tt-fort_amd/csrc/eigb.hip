@@ -50,20 +50,6 @@ struct Timer {
 
 }  // namespace
 
-// Orthogonal factorisation A (m x n, ld m) = Q R with Q (m x k) orthonormal,
-// R = Q^T A (k x n), k = min(m, n).  Q is the left singular basis (completed
-// where A is rank deficient); any orthonormal Q with Q R = A serves the
-// sweep (tt_eigb.f90:206-210 uses it only as such).  S: k device doubles.
-void orth_factor(int m, int n, const double* A, double* Q, double* R, double* S, hipStream_t st) {
-  const int k = std::min(m, n);
-  std::vector<double> hs;
-  int kpos = 0;
-  svd_vectors(m, n, A, m, true, S, Q, m, hs, &kpos, st);
-  complete_columns(m, kpos, k, Q, m, st);
-  GemmOp<double> g = colmajor<double>('T', 'N', k, n, m, 1.0, Q, m, A, m, 0.0, R, k);
-  gemm_launch(g, st);
-}
-
 }  // namespace ttf
 
 using namespace ttf;

@@ -43,6 +43,13 @@ int gesvd(int m, int n, const double* A, int64_t lda, double* U, int64_t ldu, do
 int svd_vectors(int m, int n, const double* A, int64_t lda, bool want_left, double* S, double* O,
                 int64_t ldo, std::vector<double>& hs, int* kpos, hipStream_t st);
 
+// Orthogonal factorisation A (m x n, ld m) = Q R with Q (m x k) orthonormal
+// and R = Q^T A (k x n), k = min(m, n).  Q is the left singular basis,
+// completed where A is rank deficient; any orthonormal Q with Q R = A serves
+// the sweeps (the reference's dqr, tt_linalg.f90:30-51, is used only as
+// such).  S: k device doubles of workspace.
+void orth_factor(int m, int n, const double* A, double* Q, double* R, double* S, hipStream_t st);
+
 // Completes columns [k0, k) of U (m x k, ldu) to an orthonormal set.
 void complete_columns(int m, int k0, int k, double* U, int64_t ldu, hipStream_t st);
 

@@ -842,6 +842,16 @@ int gesvd(int m, int n, const double* A, int64_t lda, double* U, int64_t ldu, do
   return jr.sweeps;
 }
 
+void orth_factor(int m, int n, const double* A, double* Q, double* R, double* S, hipStream_t st) {
+  const int k = std::min(m, n);
+  std::vector<double> hs;
+  int kpos = 0;
+  svd_vectors(m, n, A, m, true, S, Q, m, hs, &kpos, st);
+  complete_columns(m, kpos, k, Q, m, st);
+  GemmOp<double> g = colmajor<double>('T', 'N', k, n, m, 1.0, Q, m, A, m, 0.0, R, k);
+  gemm_launch(g, st);
+}
+
 int svd_vectors(int m, int n, const double* A, int64_t lda, bool want_left, double* S, double* O,
                 int64_t ldo, std::vector<double>& hs, int* kpos, hipStream_t st) {
   TTF_REQUIRE(m >= 1 && n >= 1, "svd_vectors: invalid sizes %d x %d", m, n);

@@ -67,3 +67,30 @@ def random_block_tt(d, n, r0, B, seed):
     ry = [1] + [r0] * (d - 1) + [B]
     crY = np.concatenate([rng.standard_normal(ry[k] * int(n[k]) * ry[k + 1]) for k in range(d)])
     return crY, ry
+
+
+def laplace_dd(d, n):
+    """Dirichlet Laplacian on the interior grid of [0,1]^d with n points per
+    direction, h = 1/(n+1): A = sum_k I x .. x L_k x .. x I with
+    L = tridiag(-1, 2, -1) / h^2 (rank-2 TT-matrix).  The a == 1 case of the
+    diffusion operator of the linear-solver workload.  Returns (crA, ra, nn)."""
+    L = (n + 1) ** 2 * (2.0 * np.eye(n) - np.eye(n, k=1) - np.eye(n, k=-1))
+    W = np.zeros((2, 2, n, n))
+    W[0, 0] = np.eye(n)
+    W[1, 0] = L
+    W[1, 1] = np.eye(n)
+    cores, ra = _cores_from_bond_matrix(d, W, 1, 0)
+    return np.concatenate(cores), ra, np.full(d, n, dtype=np.int64)
+
+
+def ones_tt(d, n):
+    """The all-ones vector as a rank-1 TT.  Returns (cr, r)."""
+    return np.ones(d * n), np.ones(d + 1, dtype=np.int64)
+
+
+def random_tt(d, n, r, seed):
+    """N(0,1) cores of rank r (boundary ranks 1) from PCG64(seed).  Returns (cr, rr)."""
+    rng = np.random.default_rng(seed)
+    rr = [1] + [r] * (d - 1) + [1]
+    cr = np.concatenate([rng.standard_normal(rr[k] * n * rr[k + 1]) for k in range(d)])
+    return cr, np.array(rr, dtype=np.int64)

@@ -110,6 +110,27 @@ _sig("ttf_tt_eigb", _c_int, _c_int, _IP, _IP, _IP, _DP, _DP, _IP, C.c_double, _c
      _c_int, _c_int, _c_int, _c_int, _DP, _c_i64, C.POINTER(_c_i64), C.POINTER(EigbStats))
 
 
+class AmenStats(C.Structure):
+    _fields_ = [("sweeps", C.c_int), ("total_matvecs", C.c_long), ("last_res_max", C.c_double),
+                ("last_err_max", C.c_double), ("max_rank", C.c_int), ("seconds", C.c_double),
+                ("solve_seconds", C.c_double), ("trunc_seconds", C.c_double), ("phi_seconds", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_sig("ttf_djac_gen", _c_int, C.c_char, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp)
+_sig("ttf_djac_size", _c_i64, C.c_char, _c_int, _c_int, _c_int)
+_sig("ttf_djac_apply", _c_int, C.c_char, _c_int, _c_int, _c_int, _vp, _vp, _vp)
+_sig("ttf_dgmresr", _c_int, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, C.c_double,
+     _c_int, C.c_char, _vp, _vp, _IP, _DP)
+_sig("ttf_duchol", _c_int, C.c_char, _c_int, _c_int, _vp, _c_int, _vp, _DP, _IP)
+_sig("ttf_dbfun3_left", _c_int, *_S8, _vp, _vp, _vp, _vp)
+_sig("ttf_dbfun3_right", _c_int, *_S8, _vp, _vp, _vp, _vp)
+_sig("ttf_tt_amen_solve", _c_int, _c_int, _IP, _IP, _IP, _IP, _DP, _DP, _DP, _IP, C.c_double, _c_int, _c_int,
+     _c_int, C.c_char, _c_int, _c_int, _c_int, _DP, _c_i64, C.POINTER(_c_i64), C.POINTER(AmenStats))
+
+
 def _declare(lib):
     for name, res, args in _SIGS:
         f = getattr(lib, name, None)
@@ -372,3 +393,85 @@ def tt_eigb(n, ra, crA, crY0, ry, eps, rmax, B, nswp=20, max_full_size=2000, ver
                           _dp(lam), B, kickrank, nswp, max_full_size, verb, _dp(res), cap,
                           C.byref(size), C.byref(st)), lib)
     return lam, res[:size.value].copy(), ry_, st.as_dict()
+
+
+def djac_gen(ptype, rx1, n, rx2, ra1, ra2, phi1, A, phi2):
+    """Block-Jacobi blocks through ttf_djac_gen (host arrays in and out)."""
+    lib = load()
+    sz = int(lib.ttf_djac_size(ptype.encode(), rx1, n, rx2))
+    d1, dA, d2 = (DeviceArray.from_numpy(np.ascontiguousarray(v, dtype=np.float64)) for v in (phi1, A, phi2))
+    dj = DeviceArray((max(sz, 1),))
+    check(lib.ttf_djac_gen(ptype.encode(), rx1, n, rx2, ra1, ra2, d1.ptr, dA.ptr, d2.ptr, dj.ptr), lib)
+    return dj.numpy()[:sz]
+
+
+def djac_apply(ptype, rx1, n, rx2, jacs, x):
+    lib = load()
+    dj = DeviceArray.from_numpy(np.ascontiguousarray(jacs, dtype=np.float64))
+    dx = DeviceArray.from_numpy(np.ascontiguousarray(x, dtype=np.float64))
+    dy = DeviceArray((len(x),))
+    check(lib.ttf_djac_apply(ptype.encode(), rx1, n, rx2, dj.ptr, dx.ptr, dy.ptr), lib)
+    return dy.numpy()
+
+
+def gmres(phi1, A, phi2, rhs, rx1, n, rx2, ra1, ra2, nrestart, tol, niters, ptype="n", jacs=None):
+    """Local GMRES through ttf_dgmresr: returns (sol, (cycle, j), relres)."""
+    lib = load()
+    arrs = [DeviceArray.from_numpy(np.ascontiguousarray(v, dtype=np.float64)) for v in (phi1, A, phi2, rhs)]
+    dj = DeviceArray.from_numpy(np.ascontiguousarray(jacs, dtype=np.float64)) if jacs is not None else None
+    ds = DeviceArray((len(rhs),))
+    it = np.zeros(2, dtype=np.int32)
+    rr = np.zeros(1)
+    check(lib.ttf_dgmresr(arrs[0].ptr, arrs[1].ptr, arrs[2].ptr, arrs[3].ptr, rx1, n, rx2, ra1, ra2, nrestart, tol,
+                          niters, ptype.encode(), dj.ptr if dj else None, ds.ptr, _ip(it), _dp(rr)), lib)
+    return ds.numpy(), tuple(it), float(rr[0])
+
+
+def duchol(trans, m, n, dA, numvec):
+    """Kick directions through ttf_duchol: returns (du (n x real), lam, real)."""
+    lib = load()
+    dd = DeviceArray.from_numpy(np.ascontiguousarray(dA, dtype=np.float64))
+    du = DeviceArray((n * numvec,))
+    lam = np.zeros(numvec)
+    real = np.zeros(1, dtype=np.int32)
+    check(lib.ttf_duchol(trans.encode(), m, n, dd.ptr, numvec, du.ptr, _dp(lam), _ip(real)), lib)
+    r = int(real[0])
+    return du.numpy()[: n * r].reshape((n, r), order="F"), lam, r
+
+
+def bfun3_half(side, rx1, m, rx2, ry1, n, ry2, ra1, ra2, phi, A, x):
+    """ttf_dbfun3_left (side 'L': phi = Phi1) or ttf_dbfun3_right ('R': phi = Phi2)."""
+    lib = load()
+    dp, dA, dx = (DeviceArray.from_numpy(np.ascontiguousarray(v, dtype=np.float64)) for v in (phi, A, x))
+    if side == "L":
+        dy = DeviceArray((ry1 * n * rx2 * ra2,))
+        check(lib.ttf_dbfun3_left(rx1, m, rx2, ry1, n, ry2, ra1, ra2, dp.ptr, dA.ptr, dx.ptr, dy.ptr), lib)
+    else:
+        dy = DeviceArray((rx1 * ra1 * n * ry2,))
+        check(lib.ttf_dbfun3_right(rx1, m, rx2, ry1, n, ry2, ra1, ra2, dA.ptr, dp.ptr, dx.ptr, dy.ptr), lib)
+    return dy.numpy()
+
+
+def tt_amen_solve(n, ra, crA, ry, crY, rx, crX0, eps, kickrank=4, nswp=20, verb=0, prec="n", nrestart=40,
+                  niters=2, rmax=0):
+    """Run ttf_tt_amen_solve with the reference's argument layout (host arrays).
+
+    Returns (result cores flat, rx out, stats dict)."""
+    lib = load()
+    d = len(n)
+    n_ = np.ascontiguousarray(n, dtype=np.int32)
+    ra_ = np.ascontiguousarray(ra, dtype=np.int32)
+    ry_ = np.ascontiguousarray(ry, dtype=np.int32)
+    rx_ = np.ascontiguousarray(rx, dtype=np.int32).copy()
+    crA_ = np.ascontiguousarray(crA, dtype=np.float64)
+    crY_ = np.ascontiguousarray(crY, dtype=np.float64)
+    crX_ = np.ascontiguousarray(crX0, dtype=np.float64)
+    R = rmax if rmax > 0 else 256
+    cap = int(sum(R * int(nn) * R for nn in n) + 16)
+    res = np.zeros(cap)
+    size = C.c_int64()
+    st = AmenStats()
+    check(lib.ttf_tt_amen_solve(d, _ip(n_), _ip(n_), _ip(ry_), _ip(ra_), _dp(crA_), _dp(crY_), _dp(crX_), _ip(rx_),
+                                eps, kickrank, nswp, verb, prec.encode(), nrestart, niters, rmax, _dp(res), cap,
+                                C.byref(size), C.byref(st)), lib)
+    return res[:size.value].copy(), rx_, st.as_dict()
