@@ -272,6 +272,49 @@ int ttf_tt_amen_solve(int d, const int* n, const int* m, const int* ry, const in
                       int verb, char prec, int nrestart, int niters, int rmax, double* result,
                       int64_t result_cap, int64_t* result_size, ttf_amen_stats* stats);
 
+/* ------------------------------------------------------ KSL time stepping */
+/* exp(t H) for a small complex m x m column-major device matrix (m <= 40):
+ * the irreducible (6, 6) Pade form with scaling and squaring of
+ * expokit.f:767-935 (ZGPADM), one work-group.  Device pointers. */
+int ttf_zexpm(int m, double t, const ttf_zdouble* H, ttf_zdouble* E);
+/* w = exp(tau B) v for the local operator B of zbfun3 (rx1, n, rx2, ra1,
+ * ra2; phi1 (rx1, rx1, ra1), A (ra1, n, n, ra2), phi2 (rx2, ra2, rx2)):
+ * explib.f90:53-125 (zexp_mv) with the adaptive Krylov steps of ZGEXPV
+ * (expokit.f:2403-2791), Krylov size `order` (<= 38), tolerance tol, the
+ * operator norm from Hager's 1-norm estimate (returned in *anorm_out) and
+ * the matvec count in *nmult.  Device pointers; synchronous. */
+int ttf_zexp_bfun3(int rx1, int n, int rx2, int ra1, int ra2, const ttf_zdouble* phi1, const ttf_zdouble* A,
+                   const ttf_zdouble* phi2, const ttf_zdouble* v, ttf_zdouble* w, double tau, int order,
+                   double tol, double* anorm_out, int* nmult);
+/* Q (rows x k, orthonormal columns) and R = Q^H A (k x cols), k =
+ * min(rows, cols), for a complex device matrix with rows * cols <= 8192 and
+ * k <= 64 (zqr, matrix_util.f90:64-90); rank-deficient blocks get an
+ * orthonormal completion. */
+int ttf_zqr(int rows, int cols, const ttf_zdouble* A, ttf_zdouble* Q, ttf_zdouble* R);
+
+typedef struct {
+  long nsolves;       /* local exponentials (cores and S-steps) */
+  long matvecs;       /* Krylov operator applications */
+  long krylov_steps;  /* ZGEXPV integration steps */
+  double seconds;     /* wall time of the call */
+} ttf_ksl_stats;
+
+/* One KSL time step for dY/dt = A Y in TT format, argument layout of
+ * ztt_ksl (tt_ksl.f90:431-445): d modes, n = m mode sizes, ra and crA the
+ * complex operator (the reference's convention: A carries the factor i of
+ * Schroedinger dynamics), crY0 with ranks ry (in/out, kept fixed), tau the
+ * time step.  rmax, kickrank and nswp are accepted and unused, as in the
+ * reference; typ 1 (KSL) or 2 (symmetrised, default in the reference);
+ * order the Krylov size (reference default 8, at most 38); usenrm 2 uses a
+ * unit norm estimate, anything else Hager's 1-norm estimate.  The result
+ * cores (the reference's dyn_tt::zresult_core) go to `result` (capacity in
+ * complex entries; size in *result_size).  Host pointers; device-resident
+ * run. */
+int ttf_ztt_ksl(int d, const int* n, const int* m, const int* ra, const ttf_zdouble* crA,
+                const ttf_zdouble* crY0, int* ry, double tau, int rmax, int kickrank, int nswp, int verb,
+                int typ, int order, int usenrm, ttf_zdouble* result, int64_t result_cap,
+                int64_t* result_size, ttf_ksl_stats* stats);
+
 /* ------------------------------------------ reference-compatible symbols */
 /* tt_linalg.f90:127-159, dbfun3 bind(c).  Host pointers; res1/res2 ignored. */
 void dbfun3(const ttf_fint* rx1, const ttf_fint* m, const ttf_fint* rx2,

@@ -21,6 +21,7 @@ from __future__ import annotations
 import numpy as np
 
 from . import kernels as K
+from .kernels import F, _flat
 
 
 def _svd(M):
@@ -138,3 +139,97 @@ def tt_eigb(d, n, ra, crA, crY0, ry, eps, rmax, B, nswp=20, max_full_size=2000, 
             k += direction
     ry[d] = B
     return lam, cores, ry, dict(sweeps=last, total_matvecs=total_mv)
+
+
+def _zqr(A):
+    """zqr (matrix_util.f90:64-90): thin Householder QR, Q (m x k), R (k x n)."""
+    Q, R = np.linalg.qr(A, mode="reduced")
+    return Q, R
+
+
+def ztt_ksl(d, n, ra, crA, crY0, ry, tau, order=8, typ=2, tol=1e-8):
+    """tt_ksl.f90:431-668 (ztt_ksl): one step of the projector-splitting KSL
+    integrator for dY/dt = A Y on fixed TT ranks.  typ 2 (default) is the
+    symmetrised scheme (half steps tau/2 right-to-left then left-to-right),
+    typ 1 one right-to-left pass.  Local exponentials by zexp_mv with Krylov
+    size `order` and tolerance 1e-8; the operator-norm estimate is the exact
+    1-norm of the local operator (the reference's usenrm = 0 mode; the
+    default Higham estimate only steers the first step size).
+
+    Returns (cores list, ry)."""
+    ry = list(int(r) for r in ry)
+    pa = np.concatenate([[0], np.cumsum([ra[k] * n[k] * n[k] * ra[k + 1] for k in range(d)])]).astype(int)
+    cr, off = [], 0
+    for k in range(d):
+        sz = ry[k] * n[k] * ry[k + 1]
+        cr.append(np.array(crY0[off:off + sz], dtype=np.complex128))
+        off += sz
+    phi = [None] * (d + 1)
+    phi[0] = np.ones(1, dtype=np.complex128)
+    phi[d] = np.ones(1, dtype=np.complex128)
+    Ak = [np.asarray(crA[pa[k]:pa[k + 1]], dtype=np.complex128) for k in range(d)]
+    # initial left-to-right QR and left interfaces (:538-556)
+    for i in range(d - 1):
+        M = F(cr[i], ry[i] * n[i], ry[i + 1])
+        Q, R = _zqr(M)
+        rnew = Q.shape[1]
+        cr[i] = _flat(Q)
+        cr[i + 1] = _flat(R @ F(cr[i + 1], ry[i + 1], n[i + 1] * ry[i + 2]))
+        ry[i + 1] = rnew
+        phi[i + 1] = K.zphi_left(ry[i], n[i], ry[i + 1], ry[i], n[i], ry[i + 1], ra[i], ra[i + 1],
+                                 phi[i], Ak[i], cr[i], cr[i])
+    tau0 = tau if typ == 1 else tau / 2.0                                # :566-570
+
+    def local(i, v, t):
+        sh = (ry[i], n[i], ry[i + 1], ry[i], n[i], ry[i + 1], ra[i], ra[i + 1])
+        mv = lambda x: K.zbfun3(*sh, phi[i], Ak[i], phi[i + 1], x)
+        an = K.onenorm_dense(mv, v.size)
+        return K.zexp_mv(mv, v, order, t, tol, an)[0]
+
+    def sstep(rx1, ry1, ra_, rx2, ry2, p1, p2, S, t):
+        mv = lambda x: K.zsfun(rx1, ry1, ra_, rx2, ry2, p1, p2, x)
+        an = K.onenorm_dense(mv, S.size)
+        return K.zexp_mv(mv, S, order, t, tol, an)[0]
+
+    i, dir_ = d - 1, -1
+    while True:                                                          # :575-651
+        if dir_ < 0:
+            cur = local(i, cr[i], tau0)
+            if i > 0:
+                M = F(cur, ry[i], n[i] * ry[i + 1]).T                    # (n r2) x r1
+                Q, R = _zqr(M)
+                rnew = Q.shape[1]
+                cr[i] = _flat(Q.T)
+                Rt = R.T                                                 # ry_i x rnew
+                phitmp = K.zphi_right(rnew, n[i], ry[i + 1], rnew, n[i], ry[i + 1], ra[i], ra[i + 1],
+                                      phi[i + 1], Ak[i], cr[i], cr[i])
+                St = sstep(ry[i], ry[i], ra[i], rnew, rnew, phi[i], phitmp, _flat(Rt), -tau0)
+                cr[i - 1] = _flat(F(cr[i - 1], ry[i - 1] * n[i - 1], ry[i]) @ F(St, ry[i], rnew))
+                ry[i] = rnew
+                phi[i] = phitmp
+            else:
+                cr[i] = cur
+        else:
+            cur = local(i, cr[i], tau0)
+            if i < d - 1:
+                M = F(cur, ry[i] * n[i], ry[i + 1])
+                Q, R = _zqr(M)
+                rnew = Q.shape[1]
+                cr[i] = _flat(Q)
+                phitmp = K.zphi_left(ry[i], n[i], rnew, ry[i], n[i], rnew, ra[i], ra[i + 1],
+                                     phi[i], Ak[i], cr[i], cr[i])
+                St = sstep(rnew, rnew, ra[i + 1], ry[i + 1], ry[i + 1], phitmp, phi[i + 1], _flat(R), -tau0)
+                cr[i + 1] = _flat(F(St, rnew, ry[i + 1]) @ F(cr[i + 1], ry[i + 1], n[i + 1] * ry[i + 2]))
+                ry[i + 1] = rnew
+                phi[i + 1] = phitmp
+            else:
+                cr[i] = cur
+        if dir_ > 0 and i == d - 1:                                      # :637-651
+            break
+        if dir_ < 0 and i == 0:
+            if typ == 1:
+                break
+            dir_ = 1
+            continue
+        i += dir_
+    return cr, ry

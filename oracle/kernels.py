@@ -460,3 +460,185 @@ def dgmresr_hh(matvec, rhs, nrestart, tol, niters, prec=None):
         if curres < tol:
             break
     return sol, dict(it=it, j=j, relres=curres)
+
+
+# ---------------------------------------------------------------- Krylov exponential
+def zgpadm(ideg, H, t):
+    """expokit.f:767-935 (ZGPADM): exp(t H) for a small complex matrix by the
+    irreducible (ideg, ideg) Pade form +/-(I + 2 q^-1 p) with
+    scaling-and-squaring.  Returns (E, ns)."""
+    H = np.asarray(H, dtype=np.complex128)
+    m = H.shape[0]
+    hnorm = abs(t * np.abs(H).sum(axis=1).max())                         # :833-842 (max row sum)
+    if hnorm == 0.0:
+        raise ValueError("null H in ZGPADM")
+    ns = max(0, int(np.log(hnorm) / np.log(2.0)) + 2)                    # :844
+    scale = t / 2.0 ** ns
+    scale2 = scale * scale
+    c = np.zeros(ideg + 1)                                               # :849-854
+    c[0] = 1.0
+    i, j = ideg + 1, 2 * ideg + 1
+    for k in range(1, ideg + 1):
+        c[k] = c[k - 1] * (i - k) / (k * (j - k))
+    H2 = scale2 * (H @ H)                                                # :858
+    p = c[ideg - 1] * np.eye(m, dtype=np.complex128)                     # :862-871
+    q = c[ideg] * np.eye(m, dtype=np.complex128)
+    odd = 1
+    k = ideg - 1
+    while k > 0:                                                         # :875-889
+        used = q if odd else p
+        f = used @ H2
+        f = f + c[k - 1] * np.eye(m)
+        if odd:
+            q = f
+        else:
+            p = f
+        odd = 1 - odd
+        k -= 1
+    if odd:                                                              # :893-901
+        q = scale * (q @ H)
+    else:
+        p = scale * (p @ H)
+    q = q - p                                                            # :902
+    p = np.linalg.solve(q, p)                                            # :903 zgesv
+    p = 2.0 * p + np.eye(m)                                              # :905-908
+    if ns == 0 and odd:                                                  # :910-913
+        return -p, ns
+    E = p
+    for _ in range(ns):                                                  # :917-923
+        E = E @ E
+    return E, ns
+
+
+def _nint(x):
+    """Fortran NINT: round half away from zero."""
+    return int(np.sign(x) * np.floor(abs(x) + 0.5))
+
+
+def zgexpv(matvec, v, m, t, tol, anorm, mxstep=500):
+    """expokit.f:2403-2791 (ZGEXPV): w = exp(t A) v by adaptive Krylov
+    steps (Arnoldi basis of size m, Pade on the (m+2)-augmented Hessenberg,
+    local error estimate and step-size control).  Returns (w, info)."""
+    v = np.asarray(v, dtype=np.complex128)
+    n = v.size
+    eps = np.finfo(float).eps
+    if tol <= eps:                                                       # :2591-2593
+        tol = np.sqrt(eps)
+    rndoff = eps * anorm
+    break_tol = 1.0e-7                                                   # :2596
+    delta, gamma = 1.2, 0.9                                              # :2505-2510
+    sgn = np.sign(t) if t != 0 else 1.0
+    t_out = abs(t)
+    w = v.copy()
+    beta = np.linalg.norm(w)
+    vnorm = beta
+    hump = beta
+    SQR1 = np.sqrt(0.1)                                                  # :2609
+    xm = 1.0 / m
+    p2 = tol * (((m + 1) / 2.72) ** (m + 1)) * np.sqrt(2.0 * 3.14 * (m + 1))
+    t_new = (1.0 / anorm) * (p2 / (4.0 * beta * anorm)) ** xm
+    p1 = 10.0 ** (_nint(np.log10(t_new) - SQR1) - 1)
+    t_new = np.trunc(t_new / p1 + 0.55) * p1
+    t_now, nstep, nmult = 0.0, 0, 0
+    mh = m + 2
+    while t_now < t_out:                                                 # :2618
+        nstep += 1
+        t_step = min(t_out - t_now, t_new)
+        V = np.zeros((n, m + 1), dtype=np.complex128)
+        H = np.zeros((mh, mh), dtype=np.complex128)
+        V[:, 0] = w / beta
+        k1, mbrkdwn = 2, m
+        avnorm = 0.0
+        for j in range(1, m + 1):                                        # :2632-2657 Arnoldi (MGS)
+            nmult += 1
+            p = matvec(V[:, j - 1])
+            for i in range(1, j + 1):
+                hij = np.vdot(V[:, i - 1], p)
+                p = p - hij * V[:, i - 1]
+                H[i - 1, j - 1] = hij
+            hj1j = np.linalg.norm(p)
+            if hj1j <= break_tol:                                        # :2645-2654 happy breakdown
+                k1, mbrkdwn = 0, j
+                t_step = t_out - t_now
+                break
+            H[j, j - 1] = hj1j
+            V[:, j] = p / hj1j
+        else:
+            nmult += 1
+            avnorm = np.linalg.norm(matvec(V[:, m]))                     # :2658-2660
+        H[m + 1, m] = 1.0                                                # :2666
+        while True:                                                      # :2670-2716
+            mx = mbrkdwn + k1
+            E, _ = zgpadm(6, H[:mx, :mx], sgn * t_step)
+            if k1 == 0:
+                err_loc = tol
+            else:
+                p1 = abs(E[m, 0]) * beta
+                p2 = abs(E[m + 1, 0]) * beta * avnorm
+                if p1 > 10.0 * p2:
+                    err_loc, xm = p2, 1.0 / m
+                elif p1 > p2:
+                    err_loc, xm = (p1 * p2) / (p1 - p2), 1.0 / m
+                else:
+                    err_loc, xm = p1, 1.0 / (m - 1)
+            if k1 != 0 and err_loc > delta * t_step * tol:               # :2702-2716
+                t_step = gamma * t_step * (t_step * tol / err_loc) ** xm
+                p1 = 10.0 ** (_nint(np.log10(t_step) - SQR1) - 1)
+                t_step = np.trunc(t_step / p1 + 0.55) * p1
+                continue
+            break
+        mx = mbrkdwn + max(0, k1 - 1)                                    # :2720-2723
+        w = beta * (V[:, :mx] @ E[:mx, 0])
+        beta = np.linalg.norm(w)
+        hump = max(hump, beta)
+        t_new = gamma * t_step * (t_step * tol / err_loc) ** xm          # :2727-2729
+        p1 = 10.0 ** (_nint(np.log10(t_new) - SQR1) - 1)
+        t_new = np.trunc(t_new / p1 + 0.55) * p1
+        t_now += t_step
+        if nstep >= mxstep:
+            break
+    return w, dict(nmult=nmult, nstep=nstep, hump=hump / vnorm)
+
+
+def zexp_mv(matvec, v, m, tau, tol, anorm):
+    """explib.f90:53-125 (zexp_mv): w = exp(tau A) v through ZGEXPV on the
+    normalised vector, with the 1 x 1 case done directly."""
+    v = np.asarray(v, dtype=np.complex128)
+    n = v.size
+    m1 = min(m, n)
+    if m1 == 1:                                                          # :103-105
+        a = matvec(np.ones(1, dtype=np.complex128))[0]
+        return v * np.exp(a * tau), dict(nmult=1, nstep=1)
+    nrm = np.linalg.norm(v)
+    if nrm == 0.0:
+        return np.zeros_like(v), dict(nmult=0, nstep=0)
+    w, info = zgexpv(matvec, v / nrm, m1, tau, tol, anorm)
+    return w * nrm, info
+
+
+def zsfun(rx1, ry1, ra, rx2, ry2, phi1, phi2, S):
+    """tt_ksl.f90:130-136 (zsfun_matvec): S (rx1 x rx2) -> (ry1 x ry2),
+    Sy(c1, c2) = sum phi1(c1, b1, a) S(b1, b2) phi2(b2, a, c2), phi1 viewed
+    (ry1, rx1 ra) and phi2 (rx2, ra ry2)."""
+    res1 = F(S, rx1, rx2) @ F(phi2, rx2, ra * ry2)                       # (rx1, ra*ry2)
+    res1 = np.reshape(res1, (rx1 * ra, ry2), order="F")
+    return _flat(F(phi1, ry1, rx1 * ra) @ res1)
+
+
+def zsfun_transp(rx1, ry1, ra, rx2, ry2, phi1, phi2, Sy):
+    """The adjoint of zsfun (what normest's transposed product must apply)."""
+    P1 = F(phi1, ry1, rx1, ra)
+    P2 = F(phi2, rx2, ra, ry2)
+    Y = F(Sy, ry1, ry2)
+    return _flat(np.einsum("cba,dae,ce->bd", np.conj(P1), np.conj(P2), Y))
+
+
+def onenorm_dense(matvec, n, dtype=np.complex128):
+    """Exact 1-norm of the operator through n products with unit vectors
+    (the reference's usenrm = 0 mode, normest.f90:112-140)."""
+    best = 0.0
+    for j in range(n):
+        e = np.zeros(n, dtype=dtype)
+        e[j] = 1.0
+        best = max(best, np.abs(matvec(e)).sum())
+    return best

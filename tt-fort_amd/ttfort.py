@@ -131,6 +131,21 @@ _sig("ttf_tt_amen_solve", _c_int, _c_int, _IP, _IP, _IP, _IP, _DP, _DP, _DP, _IP
      _c_int, C.c_char, _c_int, _c_int, _c_int, _DP, _c_i64, C.POINTER(_c_i64), C.POINTER(AmenStats))
 
 
+class KslStats(C.Structure):
+    _fields_ = [("nsolves", C.c_long), ("matvecs", C.c_long), ("krylov_steps", C.c_long), ("seconds", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_sig("ttf_zexpm", _c_int, _c_int, C.c_double, _vp, _vp)
+_sig("ttf_zexp_bfun3", _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, C.c_double, _c_int,
+     C.c_double, _DP, _IP)
+_sig("ttf_zqr", _c_int, _c_int, _c_int, _vp, _vp, _vp)
+_sig("ttf_ztt_ksl", _c_int, _c_int, _IP, _IP, _IP, _vp, _vp, _IP, C.c_double, _c_int, _c_int, _c_int, _c_int,
+     _c_int, _c_int, _c_int, _vp, _c_i64, C.POINTER(_c_i64), C.POINTER(KslStats))
+
+
 def _declare(lib):
     for name, res, args in _SIGS:
         f = getattr(lib, name, None)
@@ -475,3 +490,58 @@ def tt_amen_solve(n, ra, crA, ry, crY, rx, crX0, eps, kickrank=4, nswp=20, verb=
                                 eps, kickrank, nswp, verb, prec.encode(), nrestart, niters, rmax, _dp(res), cap,
                                 C.byref(size), C.byref(st)), lib)
     return res[:size.value].copy(), rx_, st.as_dict()
+
+
+def zexpm(H, t):
+    """exp(t H) of a small complex matrix through ttf_zexpm."""
+    lib = load()
+    H = np.asfortranarray(H, dtype=np.complex128)
+    m = H.shape[0]
+    dH = DeviceArray.from_numpy(H)
+    dE = DeviceArray((m, m), np.complex128)
+    check(lib.ttf_zexpm(m, t, dH.ptr, dE.ptr), lib)
+    return dE.numpy()
+
+
+def zexp_bfun3(rx1, n, rx2, ra1, ra2, phi1, A, phi2, v, tau, order=8, tol=1e-8):
+    """exp(tau B) v for the zbfun3 local operator: returns (w, anorm, nmult)."""
+    lib = load()
+    arrs = [DeviceArray.from_numpy(np.asarray(x, dtype=np.complex128)) for x in (phi1, A, phi2, v)]
+    dw = DeviceArray((len(v),), np.complex128)
+    an = np.zeros(1)
+    nm = np.zeros(1, dtype=np.int32)
+    check(lib.ttf_zexp_bfun3(rx1, n, rx2, ra1, ra2, arrs[0].ptr, arrs[1].ptr, arrs[2].ptr, arrs[3].ptr, dw.ptr, tau,
+                             order, tol, _dp(an), _ip(nm)), lib)
+    return dw.numpy(), float(an[0]), int(nm[0])
+
+
+def zqr(A):
+    """Complex QR through ttf_zqr: returns (Q, R)."""
+    lib = load()
+    A = np.asfortranarray(A, dtype=np.complex128)
+    rows, cols = A.shape
+    k = min(rows, cols)
+    dA = DeviceArray.from_numpy(A)
+    dQ, dR = DeviceArray((rows, k), np.complex128), DeviceArray((k, cols), np.complex128)
+    check(lib.ttf_zqr(rows, cols, dA.ptr, dQ.ptr, dR.ptr), lib)
+    return dQ.numpy(), dR.numpy()
+
+
+def ztt_ksl(n, ra, crA, crY0, ry, tau, typ=2, order=8, usenrm=1, verb=0):
+    """Run ttf_ztt_ksl with the reference's argument layout (host arrays).
+
+    Returns (result cores flat (complex), ry out, stats dict)."""
+    lib = load()
+    d = len(n)
+    n_ = np.ascontiguousarray(n, dtype=np.int32)
+    ra_ = np.ascontiguousarray(ra, dtype=np.int32)
+    ry_ = np.ascontiguousarray(ry, dtype=np.int32).copy()
+    crA_ = np.ascontiguousarray(crA, dtype=np.complex128)
+    crY_ = np.ascontiguousarray(crY0, dtype=np.complex128)
+    cap = int(sum(max(ry_) ** 2 * int(nn) for nn in n) + 16)
+    res = np.zeros(cap, dtype=np.complex128)
+    size = C.c_int64()
+    st = KslStats()
+    check(lib.ttf_ztt_ksl(d, _ip(n_), _ip(n_), _ip(ra_), crA_.ctypes.data, crY_.ctypes.data, _ip(ry_), tau, 0, 0, 0,
+                          verb, typ, order, usenrm, res.ctypes.data, cap, C.byref(size), C.byref(st)), lib)
+    return res[:size.value].copy(), ry_, st.as_dict()
