@@ -4,17 +4,21 @@
 
 Prints one JSON line.  Workloads:
 
-  matvec  (default) -- the local projected matvec y = Phi_L A_k Phi_R x of the
-          scaled Heisenberg XXX eigenproblem (d=64, max TT rank 128, MPO rank
-          5, 5 eigenpairs): the largest local problem of that sweep,
-          (r_left, n, r_right, R) = (128, 2, 128, 5), applied to the block of
-          5 vectors the block eigensolver works on.  One step = one block
-          matvec on inputs already resident in HBM.
+  eigb    (default) -- tt_eigb on the Heisenberg XXX chain d=64, 5 lowest
+          eigenpairs, rmax 128, tol 1e-10, rank-8 N(0,1) start (seed 0).
+          W warm-up sweeps, then K timed sweeps of the same run; one step =
+          one sweep.  `value` = algorithmic local-matvec flops of the timed
+          sweeps / their wall time; all cores and interfaces stay in HBM.
+  matvec  -- one block matvec y = Phi_L A_k Phi_R x of the largest local
+          problem of that sweep, (r_left, n, r_right, R) = (128, 2, 128, 5),
+          block of 8 vectors (the block eigensolver's width), inputs resident
+          in HBM.
 
-The `roofline` object describes the dominant kernel (the MFMA GEMM that runs
-the two large contraction stages) measured with HIP events on the library
-stream; `cpu_baseline` times the oracle (numpy + host BLAS) on the same
-workload on this host's cores.
+The `roofline` object describes the dominant kernel (the MFMA GEMM of the
+slower large contraction stage at that local shape), timed with HIP events
+on the library stream over 50 launches; `traffic` is its HBM bytes per
+launch from the committed counter summary.  `cpu_baseline` times the oracle
+(numpy + host BLAS) on the same local shape on this host's cores.
 """
 from __future__ import annotations
 
@@ -33,6 +37,30 @@ sys.path.insert(0, os.path.join(ROOT, "tt-fort_amd"))
 MI355X_PEAK_F64_TFLOPS = 78.6
 MI355X_PEAK_F32_TFLOPS = 157.3
 MI355X_HBM_GBS = 8000.0
+
+# Largest local problem of the d=64 Heisenberg sweep at rmax 128:
+# (rx1, m, rx2, ry1, n, ry2, ra1, ra2), and the block eigensolver's width
+# (5 wanted eigenpairs + 3 guard vectors).
+C2_LOCAL_SHAPE = (128, 2, 128, 128, 2, 128, 5, 5)
+C2_BLOCK = 8
+
+# HBM bytes per launch of the dominant GEMM, from two rocprofv3 --pmc passes
+# (FETCH_SIZE, WRITE_SIZE) scaled by the same counters of a known-size copy.
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "bfun3_r128_block8_hbm_traffic.md")
+
+
+def measured_traffic(stage):
+    """HBM bytes per launch of bfun3 stage `stage` (1-based) at the C2 local
+    shape, block 8, as recorded in the committed counter summary; None when
+    the summary has no figure for it."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_stage%d =" % stage):
+                    return int(line.split("=")[1])
+    except OSError:
+        pass
+    return None
 
 
 def bfun3_stage_flops(rx1, m, rx2, ry1, n, ry2, ra1, ra2):
@@ -90,11 +118,13 @@ def cpu_baseline_matvec(shape, nvec, budget_s=12.0):
 
 
 def run_matvec(args):
+    """One block matvec of the C2 local problem per step (block of 8, the
+    block eigensolver's width), inputs resident in HBM."""
     import ttfort
     lib = ttfort.load()
     ttfort.check(lib.ttf_init(0), lib)
-    shape = (128, 2, 128, 128, 2, 128, 5, 5)
-    nvec = 5
+    shape = C2_LOCAL_SHAPE
+    nvec = C2_BLOCK
     rx1, m, rx2, ry1, n, ry2, ra1, ra2 = shape
     N = rx1 * m * rx2
     rng = np.random.default_rng(0)
@@ -118,33 +148,15 @@ def run_matvec(args):
     ttfort.sync(lib)
     wall = time.perf_counter() - t0
     ms_step = tm.ms / args.steps
-    fl = bfun3_stage_flops(*shape)
-    flops_step = sum(fl) * nvec
+    flops_step = sum(bfun3_stage_flops(*shape)) * nvec
     value = flops_step / (ms_step * 1e-3) / 1e12
-
-    # per-stage kernel durations (events around each launch on the library stream)
-    import ctypes
-    st = (ctypes.c_float * 3)()
-    ttfort.check(lib.ttf_dbfun3_profile(*shape, nvec, d["phi1"].ptr, d["A"].ptr, d["phi2"].ptr,
-                                        d["x"].ptr, N, d["y"].ptr, N, max(args.steps, 20), st), lib)
-    stage_ms = [float(v) for v in st]
-    # the dominant kernel is the MFMA GEMM launch of the slower of the two
-    # large stages (1: x*phi2, 3: phi1*T2); stage 2 is a small panel product
-    dom = 0 if stage_ms[0] >= stage_ms[2] else 2
-    achieved = fl[dom] * nvec / (stage_ms[dom] * 1e-3) / 1e12
-    roofline = dict(bound="mfma", achieved=round(achieved, 3), peak=MI355X_PEAK_F64_TFLOPS,
-                    unit="TFLOP/s", frac=round(achieved / MI355X_PEAK_F64_TFLOPS, 4), traffic=None,
-                    kernel="gemm_mfma_kernel<double,...> of contraction stage %d" % (dom + 1),
-                    stage_ms=[round(v, 5) for v in stage_ms],
-                    stage_tflops=[round(fl[k] * nvec / (stage_ms[k] * 1e-3) / 1e12, 3)
-                                  for k in range(3)])
     out = dict(metric="local-matvec TFLOP/s (FP64, Heisenberg d=64 r=128 local problem)",
                value=round(value, 4), unit="TFLOP/s", n_gpus=1, steps=args.steps,
                warmup=args.warmup, ms_per_step=round(ms_step, 5), higher_is_better=True,
                scaling="weak", vs_baseline=None, dtype="f64", data="synthetic",
                config=dict(workload="eigb-local-matvec", r_left=rx1, n=m, r_right=rx2, R=ra2,
                            block=nvec, wall_s=round(wall, 3)),
-               roofline=roofline)
+               roofline=dominant_kernel_roofline(lib, shape, nvec))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_matvec(shape, nvec, budget_s=args.cpu_budget)
     return out
@@ -152,7 +164,8 @@ def run_matvec(args):
 
 def dominant_kernel_roofline(lib, shape, nvec, iters=50):
     """Per-stage HIP-event timings of the bfun3 contraction on the library
-    stream; the roofline reports the slower of the two large MFMA stages."""
+    stream (each stage launched `iters` times back to back between two
+    events); the roofline reports the stage-1 MFMA GEMM."""
     import ctypes
     import ttfort
     rx1, m, rx2, ry1, n, ry2, ra1, ra2 = shape
@@ -170,12 +183,16 @@ def dominant_kernel_roofline(lib, shape, nvec, iters=50):
                                         d["x"].ptr, N, d["y"].ptr, N, iters, st), lib)
     stage_ms = [float(v) for v in st]
     fl = bfun3_stage_flops(*shape)
-    dom = 0 if stage_ms[0] >= stage_ms[2] else 2
+    # stage 1 (x * phi2) is one GEMM launch; stage 3 (phi1 * T2) does the same
+    # flops, split-K into a GEMM and a reduction launch, and stage 2 is the
+    # small panel product -- stage 1 is the kernel the roofline reports
+    dom = 0
     achieved = fl[dom] * nvec / (stage_ms[dom] * 1e-3) / 1e12
+    traffic = measured_traffic(dom + 1) if (shape, nvec) == (C2_LOCAL_SHAPE, C2_BLOCK) else None
     return dict(bound="mfma", achieved=round(achieved, 3), peak=MI355X_PEAK_F64_TFLOPS,
-                unit="TFLOP/s", frac=round(achieved / MI355X_PEAK_F64_TFLOPS, 4), traffic=None,
-                kernel="gemm_mfma_kernel<double,...> of bfun3 stage %d at (r,n,r,R)=(%d,%d,%d,%d), "
-                       "block %d" % (dom + 1, rx1, m, rx2, ra2, nvec),
+                unit="TFLOP/s", frac=round(achieved / MI355X_PEAK_F64_TFLOPS, 4), traffic=traffic,
+                kernel="gemm_mfma_kernel<double,...> of bfun3 stage 1 at (r,n,r,R)=(%d,%d,%d,%d), "
+                       "block %d" % (rx1, m, rx2, ra2, nvec),
                 stage_ms=[round(v, 5) for v in stage_ms],
                 stage_tflops=[round(fl[k] * nvec / (stage_ms[k] * 1e-3) / 1e12, 3) for k in range(3)])
 
@@ -202,7 +219,7 @@ def run_eigb(args):
     t_timed = sum(secs[i] for i in timed)
     f_timed = sum(flops[i] for i in timed)
     value = f_timed / t_timed / 1e12
-    roof = dominant_kernel_roofline(lib, (rmax, 2, rmax, rmax, 2, rmax, 5, 5), 8)
+    roof = dominant_kernel_roofline(lib, (rmax, 2, rmax, rmax, 2, rmax, 5, 5), C2_BLOCK)
     out = dict(metric="sweep time + local-matvec TFLOP/s (FP64 tt_eigb sweep, Heisenberg XXX d=%d)" % d,
                value=round(value, 4), unit="TFLOP/s", n_gpus=1, steps=len(timed), warmup=args.warmup,
                ms_per_step=round(1e3 * t_timed / len(timed), 2), higher_is_better=True, scaling="weak",
@@ -217,7 +234,7 @@ def run_eigb(args):
                            phi_s=round(st["phi_seconds"], 3), wall_s=round(wall, 2)),
                roofline=roof)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline_matvec((rmax, 2, rmax, rmax, 2, rmax, 5, 5), B,
+        out["cpu_baseline"] = cpu_baseline_matvec((rmax, 2, rmax, rmax, 2, rmax, 5, 5), C2_BLOCK,
                                                   budget_s=args.cpu_budget)
         out["cpu_baseline"]["sample"] += " (the oracle's local matvec at the sweep's largest local shape)"
     return out

@@ -88,9 +88,11 @@ long ttf_autotune_count(void);
 int ttf_dbfun3(int rx1, int m, int rx2, int ry1, int n, int ry2, int ra1, int ra2,
                int nvec, const double* phi1, const double* A, const double* phi2,
                const double* x, int64_t ldx, double* y, int64_t ldy);
-/* Runs the three dbfun3 GEMM stages `iters` times with an event around each
- * stage and returns the mean milliseconds per stage in stage_ms[3] (for the
- * benchmark's per-kernel roofline).  Synchronous. */
+/* Times the three dbfun3 GEMM stages: each stage is launched `iters` times
+ * back to back between two events on the library stream (after one launch
+ * that settles its tuning) and stage_ms[k] receives the mean milliseconds
+ * per launch (for the benchmark's per-kernel roofline).  Synchronous; y and
+ * the contraction arena are overwritten. */
 int ttf_dbfun3_profile(int rx1, int m, int rx2, int ry1, int n, int ry2, int ra1, int ra2,
                        int nvec, const double* phi1, const double* A, const double* phi2,
                        const double* x, int64_t ldx, double* y, int64_t ldy, int iters,

@@ -18,25 +18,47 @@ namespace ttf {
 //                           layout the reference reaches after dtransp :26)
 // Stage 3 (ttals.f90:29):  y(c1,i,c2) = sum_{b1,a1} phi1(c1,b1,a1) T2(b1,a1,i,c2)
 template <typename T>
-void bfun3_stage12(const Shape& s, int nvec, const T* A, const T* phi2, const T* x, int64_t ldx,
-                   T* T1, T* T2, hipStream_t st) {
+static GemmOp<T> bfun3_op1(const Shape& s, int nvec, const T* phi2, const T* x, int64_t ldx, T* T1) {
+  const int64_t sz1 = (int64_t)s.rx1 * s.m * s.ra2 * s.ry2;
+  GemmOp<T> g{};
+  g.M = s.rx1 * s.m; g.N = s.ra2 * s.ry2; g.K = s.rx2;
+  g.A = x; g.sAi = 1; g.sAk = (int64_t)s.rx1 * s.m; g.sAb0 = ldx; g.sAb1 = 0;
+  g.B = phi2; g.sBk = 1; g.sBj = s.rx2; g.sBb0 = 0; g.sBb1 = 0;
+  g.C = T1; g.sCi = 1; g.sCj = (int64_t)s.rx1 * s.m; g.sCb0 = sz1; g.sCb1 = 0;
+  g.nb0 = nvec; g.nbatch = nvec; g.alpha = T(1); g.beta = T(0); g.ksplit = 1;
+  return g;
+}
+
+template <typename T>
+static GemmOp<T> bfun3_op2(const Shape& s, int nvec, const T* A, const T* T1, T* T2) {
   const int64_t sz1 = (int64_t)s.rx1 * s.m * s.ra2 * s.ry2;
   const int64_t sz2 = (int64_t)s.rx1 * s.ra1 * s.n * s.ry2;
-  GemmOp<T> g1{};
-  g1.M = s.rx1 * s.m; g1.N = s.ra2 * s.ry2; g1.K = s.rx2;
-  g1.A = x; g1.sAi = 1; g1.sAk = (int64_t)s.rx1 * s.m; g1.sAb0 = ldx; g1.sAb1 = 0;
-  g1.B = phi2; g1.sBk = 1; g1.sBj = s.rx2; g1.sBb0 = 0; g1.sBb1 = 0;
-  g1.C = T1; g1.sCi = 1; g1.sCj = (int64_t)s.rx1 * s.m; g1.sCb0 = sz1; g1.sCb1 = 0;
-  g1.nb0 = nvec; g1.nbatch = nvec; g1.alpha = T(1); g1.beta = T(0); g1.ksplit = 1;
-  gemm_launch(g1, st);
+  GemmOp<T> g{};
+  g.M = s.rx1; g.N = s.ra1 * s.n; g.K = s.m * s.ra2;
+  g.A = T1; g.sAi = 1; g.sAk = s.rx1; g.sAb0 = (int64_t)s.rx1 * s.m * s.ra2; g.sAb1 = sz1;
+  g.B = A; g.sBk = (int64_t)s.ra1 * s.n; g.sBj = 1; g.sBb0 = 0; g.sBb1 = 0;
+  g.C = T2; g.sCi = 1; g.sCj = s.rx1; g.sCb0 = (int64_t)s.rx1 * s.ra1 * s.n; g.sCb1 = sz2;
+  g.nb0 = s.ry2; g.nbatch = s.ry2 * nvec; g.alpha = T(1); g.beta = T(0); g.ksplit = 1;
+  return g;
+}
 
-  GemmOp<T> g2{};
-  g2.M = s.rx1; g2.N = s.ra1 * s.n; g2.K = s.m * s.ra2;
-  g2.A = T1; g2.sAi = 1; g2.sAk = s.rx1; g2.sAb0 = (int64_t)s.rx1 * s.m * s.ra2; g2.sAb1 = sz1;
-  g2.B = A; g2.sBk = (int64_t)s.ra1 * s.n; g2.sBj = 1; g2.sBb0 = 0; g2.sBb1 = 0;
-  g2.C = T2; g2.sCi = 1; g2.sCj = s.rx1; g2.sCb0 = (int64_t)s.rx1 * s.ra1 * s.n; g2.sCb1 = sz2;
-  g2.nb0 = s.ry2; g2.nbatch = s.ry2 * nvec; g2.alpha = T(1); g2.beta = T(0); g2.ksplit = 1;
-  gemm_launch(g2, st);
+template <typename T>
+static GemmOp<T> bfun3_op3(const Shape& s, int nvec, const T* phi1, const T* T2, T* y, int64_t ldy) {
+  const int64_t sz2 = (int64_t)s.rx1 * s.ra1 * s.n * s.ry2;
+  GemmOp<T> g{};
+  g.M = s.ry1; g.N = s.n * s.ry2; g.K = s.rx1 * s.ra1;
+  g.A = phi1; g.sAi = 1; g.sAk = s.ry1; g.sAb0 = 0; g.sAb1 = 0;
+  g.B = T2; g.sBk = 1; g.sBj = (int64_t)s.rx1 * s.ra1; g.sBb0 = sz2; g.sBb1 = 0;
+  g.C = y; g.sCi = 1; g.sCj = s.ry1; g.sCb0 = ldy; g.sCb1 = 0;
+  g.nb0 = nvec; g.nbatch = nvec; g.alpha = T(1); g.beta = T(0); g.ksplit = 1;
+  return g;
+}
+
+template <typename T>
+void bfun3_stage12(const Shape& s, int nvec, const T* A, const T* phi2, const T* x, int64_t ldx,
+                   T* T1, T* T2, hipStream_t st) {
+  gemm_launch(bfun3_op1(s, nvec, phi2, x, ldx, T1), st);
+  gemm_launch(bfun3_op2(s, nvec, A, (const T*)T1, T2), st);
 }
 
 template <typename T>
@@ -48,15 +70,11 @@ void bfun3_chain(const Shape& s, int nvec, const T* phi1, const T* A, const T* p
   T* T1 = (T*)arena(kArenaBfun, sizeof(T) * (size_t)(sz1 + sz2) * nvec);
   T* T2 = T1 + sz1 * nvec;
   bfun3_stage12(s, nvec, A, phi2, x, ldx, T1, T2, st);
-  GemmOp<T> g3{};
-  g3.M = s.ry1; g3.N = s.n * s.ry2; g3.K = s.rx1 * s.ra1;
-  g3.A = phi1; g3.sAi = 1; g3.sAk = s.ry1; g3.sAb0 = 0; g3.sAb1 = 0;
-  g3.B = T2; g3.sBk = 1; g3.sBj = (int64_t)s.rx1 * s.ra1; g3.sBb0 = sz2; g3.sBb1 = 0;
-  g3.C = y; g3.sCi = 1; g3.sCj = s.ry1; g3.sCb0 = ldy; g3.sCb1 = 0;
-  g3.nb0 = nvec; g3.nbatch = nvec; g3.alpha = T(1); g3.beta = T(0); g3.ksplit = 1;
-  gemm_launch(g3, st);
+  gemm_launch(bfun3_op3(s, nvec, phi1, (const T*)T2, y, ldy), st);
 }
 
+template void bfun3_stage12<double>(const Shape&, int, const double*, const double*, const double*,
+                                    int64_t, double*, double*, hipStream_t);
 template void bfun3_chain<double>(const Shape&, int, const double*, const double*, const double*,
                                   const double*, int64_t, double*, int64_t, hipStream_t);
 template void bfun3_chain<float>(const Shape&, int, const float*, const float*, const float*,
@@ -423,51 +441,32 @@ int ttf_dbfun3_profile(int rx1, int m, int rx2, int ry1, int n, int ry2, int ra1
                        float* stage_ms) {
   return guarded([&] {
     Shape s = mkshape(rx1, m, rx2, ry1, n, ry2, ra1, ra2);
-    TTF_REQUIRE(iters > 0 && stage_ms != nullptr, "ttf_dbfun3_profile: bad arguments");
+    TTF_REQUIRE(iters > 0 && nvec > 0 && stage_ms != nullptr, "ttf_dbfun3_profile: bad arguments");
     hipStream_t st = stream();
     const int64_t sz1 = (int64_t)s.rx1 * s.m * s.ra2 * s.ry2;
     const int64_t sz2 = (int64_t)s.rx1 * s.ra1 * s.n * s.ry2;
     double* T1 = (double*)arena(kArenaBfun, sizeof(double) * (size_t)(sz1 + sz2) * nvec);
     double* T2 = T1 + sz1 * nvec;
-    hipEvent_t ev[4];
-    for (auto& e : ev) TTF_HIP(hipEventCreate(&e));
-    double acc[3] = {0, 0, 0};
-    for (int it = 0; it < iters; ++it) {
-      TTF_HIP(hipEventRecord(ev[0], st));
-      // stages 1 and 2 are launched separately to time each
-      GemmOp<double> g1{};
-      g1.M = s.rx1 * s.m; g1.N = s.ra2 * s.ry2; g1.K = s.rx2;
-      g1.A = x; g1.sAi = 1; g1.sAk = (int64_t)s.rx1 * s.m; g1.sAb0 = ldx;
-      g1.B = phi2; g1.sBk = 1; g1.sBj = s.rx2;
-      g1.C = T1; g1.sCi = 1; g1.sCj = (int64_t)s.rx1 * s.m; g1.sCb0 = sz1;
-      g1.nb0 = nvec; g1.nbatch = nvec; g1.alpha = 1.0; g1.beta = 0.0; g1.ksplit = 1;
-      gemm_launch(g1, st);
-      TTF_HIP(hipEventRecord(ev[1], st));
-      GemmOp<double> g2{};
-      g2.M = s.rx1; g2.N = s.ra1 * s.n; g2.K = s.m * s.ra2;
-      g2.A = T1; g2.sAi = 1; g2.sAk = s.rx1; g2.sAb0 = (int64_t)s.rx1 * s.m * s.ra2; g2.sAb1 = sz1;
-      g2.B = A; g2.sBk = (int64_t)s.ra1 * s.n; g2.sBj = 1;
-      g2.C = T2; g2.sCi = 1; g2.sCj = s.rx1; g2.sCb0 = (int64_t)s.rx1 * s.ra1 * s.n; g2.sCb1 = sz2;
-      g2.nb0 = s.ry2; g2.nbatch = s.ry2 * nvec; g2.alpha = 1.0; g2.beta = 0.0; g2.ksplit = 1;
-      gemm_launch(g2, st);
-      TTF_HIP(hipEventRecord(ev[2], st));
-      GemmOp<double> g3{};
-      g3.M = s.ry1; g3.N = s.n * s.ry2; g3.K = s.rx1 * s.ra1;
-      g3.A = phi1; g3.sAi = 1; g3.sAk = s.ry1;
-      g3.B = T2; g3.sBk = 1; g3.sBj = (int64_t)s.rx1 * s.ra1; g3.sBb0 = sz2;
-      g3.C = y; g3.sCi = 1; g3.sCj = s.ry1; g3.sCb0 = ldy;
-      g3.nb0 = nvec; g3.nbatch = nvec; g3.alpha = 1.0; g3.beta = 0.0; g3.ksplit = 1;
-      gemm_launch(g3, st);
-      TTF_HIP(hipEventRecord(ev[3], st));
-      TTF_HIP(hipEventSynchronize(ev[3]));
-      for (int k = 0; k < 3; ++k) {
-        float t = 0.f;
-        TTF_HIP(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
-        acc[k] += t;
-      }
+    const GemmOp<double> ops[3] = {bfun3_op1(s, nvec, phi2, x, ldx, T1),
+                                   bfun3_op2(s, nvec, A, (const double*)T1, T2),
+                                   bfun3_op3(s, nvec, phi1, (const double*)T2, y, ldy)};
+    hipEvent_t e0, e1;
+    TTF_HIP(hipEventCreate(&e0));
+    TTF_HIP(hipEventCreate(&e1));
+    // each stage is launched `iters` times back to back between two events
+    // (the stages are idempotent), after one launch that settles its tuning
+    for (int k = 0; k < 3; ++k) {
+      gemm_launch(ops[k], st);
+      TTF_HIP(hipEventRecord(e0, st));
+      for (int it = 0; it < iters; ++it) gemm_launch(ops[k], st);
+      TTF_HIP(hipEventRecord(e1, st));
+      TTF_HIP(hipEventSynchronize(e1));
+      float t = 0.f;
+      TTF_HIP(hipEventElapsedTime(&t, e0, e1));
+      stage_ms[k] = t / (float)iters;
     }
-    for (int k = 0; k < 3; ++k) stage_ms[k] = (float)(acc[k] / iters);
-    for (auto& e : ev) TTF_HIP(hipEventDestroy(e));
+    TTF_HIP(hipEventDestroy(e0));
+    TTF_HIP(hipEventDestroy(e1));
   });
 }
 

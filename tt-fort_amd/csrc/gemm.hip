@@ -3,6 +3,7 @@
 #include "gemm_mfma.h"
 
 #include <chrono>
+#include <cstdio>
 #include <cstdlib>
 #include <mutex>
 #include <unordered_map>
@@ -91,7 +92,46 @@ std::mutex g_tune_mu;
 std::unordered_map<Key, Cfg, KeyHash> g_tuned;
 int g_autotune = -1;   // -1: read TTF_AUTOTUNE on first use
 long g_tune_count = 0;
+bool g_table_loaded = false;
 }  // namespace
+
+// Persistent tuning table: when TTF_TUNE_FILE names a file, its entries
+// ("type M N K nbatch flags bm bk ks" per line) are loaded on first use and
+// every newly tuned shape is appended, so that separate processes (a timed
+// run and its counter-collection reruns, or successive jobs) launch the same
+// configuration for the same product.  Called with g_tune_mu held.
+static const char* tune_file() {
+  const char* f = std::getenv("TTF_TUNE_FILE");
+  return (f && f[0]) ? f : nullptr;
+}
+
+static void load_tune_table() {
+  if (g_table_loaded) return;
+  g_table_loaded = true;
+  const char* path = tune_file();
+  if (!path) return;
+  FILE* fp = std::fopen(path, "r");
+  if (!fp) return;
+  Key k{};
+  Cfg c{};
+  while (std::fscanf(fp, "%d %d %d %d %d %d %d %d %d", &k.type, &k.M, &k.N, &k.K, &k.nbatch, &k.flags,
+                     &c.bm, &c.bk, &c.ks) == 9) {
+    const bool valid = (c.bm == 32 || c.bm == 64) && (c.bk == 16 || c.bk == 32 || c.bk == 64) &&
+                       c.ks >= 1 && c.ks <= 16;
+    if (valid) g_tuned[k] = c;
+  }
+  std::fclose(fp);
+}
+
+static void record_tuned(const Key& k, const Cfg& c) {
+  const char* path = tune_file();
+  if (!path) return;
+  FILE* fp = std::fopen(path, "a");
+  if (!fp) return;
+  std::fprintf(fp, "%d %d %d %d %d %d %d %d %d\n", k.type, k.M, k.N, k.K, k.nbatch, k.flags, c.bm, c.bk,
+               c.ks);
+  std::fclose(fp);
+}
 
 static bool autotune_enabled() {
   if (g_autotune < 0) {
@@ -218,18 +258,17 @@ void gemm_launch(GemmOp<T> op, hipStream_t st) {
   bool have = false;
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
+    load_tune_table();
     auto it = g_tuned.find(key);
     if (it != g_tuned.end()) { c = it->second; have = true; }
   }
   if (!have) {
     const bool alias = (const void*)op.C == (const void*)op.A || (const void*)op.C == (const void*)op.B;
-    if (autotune_enabled() && op.beta == T(0) && !alias && flops >= 4e6) {
-      c = tune(op, st);
-    } else {
-      c = heuristic(op.M, op.N, op.K, op.nbatch);
-    }
+    const bool tuned = autotune_enabled() && op.beta == T(0) && !alias && flops >= 4e6;
+    c = tuned ? tune(op, st) : heuristic(op.M, op.N, op.K, op.nbatch);
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tuned[key] = c;
+    if (tuned) record_tuned(key, c);
   }
   if (c.ks > 1) op.work = splitk_work(op, c.ks);
   launch_cfg(op, c, st);

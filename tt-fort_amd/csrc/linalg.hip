@@ -101,9 +101,9 @@ __device__ void write_sorted(double (*A)[LD], double (*V)[LD], int n, int desc, 
 
 __global__ __launch_bounds__(256) void syev_small_kernel(int n, double* Aall, double* wall,
                                                          double* Qall, int desc) {
-  __shared__ double A[64][65];
+  __shared__ double A[64][65], B[64][65];
   __shared__ double V[64][65];
-  __shared__ JacobiScratch js;
+  __shared__ int flags[3];
   __shared__ double red[16];
   const double* Ag = Aall + (int64_t)blockIdx.x * n * n;
   double f2 = 0.0;
@@ -115,9 +115,8 @@ __global__ __launch_bounds__(256) void syev_small_kernel(int n, double* Aall, do
     f2 += a * a;
   }
   f2 = block_sum(f2, red);
-  jacobi_sym<65>(A, V, n, 40, 2.220446049250313e-16 * sqrt(f2), &js);
-  __syncthreads();
-  write_sorted<65>(A, V, n, desc, wall + (int64_t)blockIdx.x * n, Qall + (int64_t)blockIdx.x * n * n);
+  double(*D)[65] = jacobi_sym<65>(A, B, V, n, 40, 2.220446049250313e-16 * sqrt(f2), flags);
+  write_sorted<65>(D, V, n, desc, wall + (int64_t)blockIdx.x * n, Qall + (int64_t)blockIdx.x * n * n);
 }
 
 void syev_small(int n, int count, double* A, double* w, double* Q, int desc, hipStream_t st) {

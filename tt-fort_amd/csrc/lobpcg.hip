@@ -33,9 +33,9 @@ constexpr double kEps = 2.220446049250313e-16;
 // lambda > 1e-14 lambda_max (others get a zero column and mask 0).
 __global__ __launch_bounds__(256) void svqb_kernel(int k, const double* __restrict__ G,
                                                    double* __restrict__ T, int* __restrict__ mask) {
-  __shared__ double A[kMaxM][kMaxM + 1], V[kMaxM][kMaxM + 1];
+  __shared__ double A[kMaxM][kMaxM + 1], B[kMaxM][kMaxM + 1], V[kMaxM][kMaxM + 1];
   __shared__ double dsc[kMaxM], red[16];
-  __shared__ JacobiScratch js;
+  __shared__ int flags[3];
   double f2 = 0.0;
   for (int e = threadIdx.x; e < k * k; e += blockDim.x) {
     int i = e % k, j = e / k;
@@ -52,18 +52,17 @@ __global__ __launch_bounds__(256) void svqb_kernel(int k, const double* __restri
     dsc[i] = gi > 0 ? 1.0 / sqrt(gi) : 0.0;
   }
   f2 = block_sum(f2, red);
-  jacobi_sym<kMaxM + 1>(A, V, k, 40, kEps * sqrt(f2), &js);
-  __syncthreads();
+  double(*D)[kMaxM + 1] = jacobi_sym<kMaxM + 1>(A, B, V, k, 40, kEps * sqrt(f2), flags);
   double lmax = 0.0;
-  for (int i = 0; i < k; ++i) lmax = fmax(lmax, A[i][i]);
+  for (int i = 0; i < k; ++i) lmax = fmax(lmax, D[i][i]);
   for (int e = threadIdx.x; e < k * k; e += blockDim.x) {
     int i = e % k, j = e / k;
-    double l = A[j][j];
+    double l = D[j][j];
     bool keep = l > 1e-14 * lmax && lmax > 0.0;
     T[i + k * j] = keep ? dsc[i] * V[i][j] / sqrt(l) : 0.0;
   }
   for (int j = threadIdx.x; j < k; j += blockDim.x) {
-    double l = A[j][j];
+    double l = D[j][j];
     mask[j] = (l > 1e-14 * lmax && lmax > 0.0) ? 1 : 0;
   }
 }
@@ -78,16 +77,13 @@ __global__ __launch_bounds__(256) void rr_kernel(int m, int bs, int nx, const do
                                                  double* __restrict__ amax) {
   __shared__ double A[kMaxM][kMaxM + 1], V[kMaxM][kMaxM + 1], Y[kMaxM][kMaxM + 1];
   __shared__ double Z[kMaxM][kMaxM / 2 + 1];
-  __shared__ double w[kMaxM], red[16];
-  __shared__ JacobiScratch js;
-  __shared__ double big;
+  __shared__ double w[kMaxM], dg[kMaxM], red[16];
+  __shared__ int flags[3];
   const int tid = threadIdx.x;
-  if (tid == 0) {
-    double mx = 0.0;
-    for (int e = 0; e < m * m; ++e) mx = fmax(mx, fabs(H[e]));
-    big = 1e6 * (1.0 + mx);
-  }
-  __syncthreads();
+  double mx = 0.0;
+  for (int e = tid; e < m * m; e += blockDim.x) mx = fmax(mx, fabs(H[e]));
+  mx = block_max(mx, red);
+  const double big = 1e6 * (1.0 + mx);
   // the rounding floor of the rotations is set by the valid block only: the
   // penalty rows are exactly decoupled and never rotate
   double f2 = 0.0;
@@ -100,14 +96,17 @@ __global__ __launch_bounds__(256) void rr_kernel(int m, int bs, int nx, const do
     if (ok) f2 += h * h;
   }
   f2 = block_sum(f2, red);
-  jacobi_sym<kMaxM + 1>(A, V, m, 40, kEps * sqrt(f2), &js);
+  // Y is the second Jacobi buffer; the diagonal is copied out before Y is
+  // reused for the sorted vectors
+  double(*D)[kMaxM + 1] = jacobi_sym<kMaxM + 1>(A, Y, V, m, 40, kEps * sqrt(f2), flags);
+  for (int i = tid; i < m; i += blockDim.x) dg[i] = D[i][i];
   __syncthreads();
   // ascending order
   for (int i = tid; i < m; i += blockDim.x) {
-    double wi = A[i][i];
+    double wi = dg[i];
     int pos = 0;
     for (int j = 0; j < m; ++j) {
-      double wj = A[j][j];
+      double wj = dg[j];
       pos += (wj < wi || (wj == wi && j < i)) ? 1 : 0;
     }
     w[pos] = wi;
@@ -147,26 +146,28 @@ __global__ __launch_bounds__(256) void rr_kernel(int m, int bs, int nx, const do
   }
   // modified Gram-Schmidt on the columns of Z (twice) by the first wave,
   // lane r owning row r (m <= 64): dot products are wave reductions through
-  // lane shuffles.
-  for (int pass = 0; pass < 2; ++pass) {
-    for (int c = 0; c < bs; ++c) {
-      for (int j = 0; j < c; ++j) {
-        double s = (tid < m) ? Z[tid][j] * Z[tid][c] : 0.0;
+  // lane shuffles and each lane touches only its own row, so the wave needs
+  // no barrier.
+  if (tid < 64) {
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int c = 0; c < bs; ++c) {
+        for (int j = 0; j < c; ++j) {
+          double s = (tid < m) ? Z[tid][j] * Z[tid][c] : 0.0;
+#pragma unroll
+          for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+          if (tid < m) Z[tid][c] -= s * Z[tid][j];
+        }
+        double s = (tid < m) ? Z[tid][c] * Z[tid][c] : 0.0;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-        if (tid < m) Z[tid][c] -= s * Z[tid][j];
-        __syncthreads();
+        double nrm = sqrt(s);
+        double inv = nrm > 1e-10 ? 1.0 / nrm : 0.0;
+        if (tid < m) Z[tid][c] *= inv;
+        if (tid == 0 && pass == 1) maskP[c] = nrm > 1e-10 ? 1 : 0;
       }
-      double s = (tid < m) ? Z[tid][c] * Z[tid][c] : 0.0;
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-      double nrm = sqrt(s);
-      double inv = nrm > 1e-10 ? 1.0 / nrm : 0.0;
-      if (tid < m) Z[tid][c] *= inv;
-      if (tid == 0 && pass == 1) maskP[c] = nrm > 1e-10 ? 1 : 0;
-      __syncthreads();
     }
   }
+  __syncthreads();
   for (int e = tid; e < m * 2 * bs; e += blockDim.x) {
     int r = e % m, c = e / m;
     CZ[r + m * c] = (c < bs) ? Y[r][c] : Z[r][c - bs];
