@@ -240,12 +240,65 @@ def run_eigb(args):
     return out
 
 
+def _tt_expect(d, n, ra, crA, cr, ry):
+    """<y|A|y> / <y|y> for a TT y and TT-matrix A (host numpy contraction)."""
+    pa, py = 0, 0
+    L = np.ones((1, 1, 1), dtype=np.complex128)     # (ry, ra, ry)
+    N = np.ones((1, 1), dtype=np.complex128)
+    for k in range(d):
+        a = crA[pa:pa + ra[k] * n[k] * n[k] * ra[k + 1]].reshape((ra[k], n[k], n[k], ra[k + 1]), order="F")
+        y = cr[py:py + ry[k] * n[k] * ry[k + 1]].reshape((ry[k], n[k], ry[k + 1]), order="F")
+        pa += a.size
+        py += y.size
+        L = np.einsum("xay,xiX,aijA,yjY->XAY", L, y.conj(), a, y, optimize=True)
+        N = np.einsum("xy,xiX,yiY->XY", N, y.conj(), y, optimize=True)
+    return complex(L.reshape(-1)[0] / N.reshape(-1)[0])
+
+
+def run_ksl(args):
+    """Transverse-field Ising chain g=1, d=40, A = -iH, all-up product state
+    zero-padded to fixed TT rank 64, dt 0.05, Krylov order 30: W untimed then
+    K timed ztt_ksl steps (one step = one typ-2 symmetric KSL step; the
+    driver twin takes and returns host cores, so each step includes their
+    upload and download, about 5 MB each way)."""
+    import ttfort
+    import mpo
+    lib = ttfort.load()
+    ttfort.check(lib.ttf_init(0), lib)
+    d, r, tau, order = 40, 64, 0.05, 30
+    crA, ra, n = mpo.tfim_schroedinger(d)
+    cr, ry = mpo.product_state_padded(d, r)
+    sz0 = int(ra[0] * n[0] * n[0] * ra[1])
+    crH = np.concatenate([1j * crA[:sz0], crA[sz0:]])      # H = i A (A carries -i on core 1)
+    e0 = _tt_expect(d, n, ra, crH, cr, ry)
+    for _ in range(args.warmup):
+        cr, ry, _ = ttfort.ztt_ksl(n, ra, crA, cr, ry, tau, order=order)
+    t0 = time.perf_counter()
+    mv = 0
+    fl = 0.0
+    for _ in range(args.steps):
+        cr, ry, st = ttfort.ztt_ksl(n, ra, crA, cr, ry, tau, order=order)
+        mv += st["matvecs"]
+        fl += st["flops"]
+    wall = time.perf_counter() - t0
+    e1 = _tt_expect(d, n, ra, crH, cr, ry)
+    ms = 1e3 * wall / args.steps
+    return dict(metric="KSL step time + local-matvec TFLOP/s (complex128 ztt_ksl, TFIM d=40 r=64)",
+                value=round(fl / wall / 1e12, 4), unit="TFLOP/s", n_gpus=1, steps=args.steps,
+                warmup=args.warmup, ms_per_step=round(ms, 2), higher_is_better=True, scaling="weak",
+                vs_baseline=None, dtype="c128", data="synthetic",
+                config=dict(workload="ztt_ksl TFIM g=1 d=40 rank 64 dt=0.05 order 30 all-up start",
+                            d=d, r=r, tau=tau, order=order, matvecs=mv,
+                            energy_start=round(e0.real, 10), energy_end=round(e1.real, 10),
+                            host_transfer_bytes_per_step=int(2 * 16 * cr.size), wall_s=round(wall, 2)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="eigb", choices=["eigb", "matvec"])
+    ap.add_argument("--workload", default="eigb", choices=["eigb", "matvec", "ksl"])
     ap.add_argument("--eigb-d", type=int, default=64)
     ap.add_argument("--eigb-rmax", type=int, default=128)
     ap.add_argument("--verb", type=int, default=1, help="per-step progress lines on stderr")
@@ -256,11 +309,12 @@ def main():
         print("bench.py: this build measures one GPU only (the local-solver path has no "
               "multi-GPU mode); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    default_steps = {"eigb": (2, 2), "matvec": (200, 20), "ksl": (10, 1)}[args.workload]
     if args.steps is None:
-        args.steps = 2 if args.workload == "eigb" else 200
+        args.steps = default_steps[0]
     if args.warmup is None:
-        args.warmup = 2 if args.workload == "eigb" else 20
-    out = {"matvec": run_matvec, "eigb": run_eigb}[args.workload](args)
+        args.warmup = default_steps[1]
+    out = {"matvec": run_matvec, "eigb": run_eigb, "ksl": run_ksl}[args.workload](args)
     print(json.dumps(out), flush=True)
 
 

@@ -76,7 +76,9 @@ int ttf_timer_stop(float* ms);
 /* GEMM tile autotuning (on by default; env TTF_AUTOTUNE=0 disables).  The
  * first product of each new shape times the tile / split-K candidates on the
  * library stream and caches the fastest; disabling clears the cache and uses
- * a fixed heuristic.  ttf_autotune_count returns how many shapes were tuned. */
+ * a fixed heuristic.  ttf_autotune_count returns how many shapes were tuned.
+ * With env TTF_TUNE_FILE set, tuned shapes are appended to that file and
+ * read back by later processes, which then launch the same configurations. */
 int ttf_set_autotune(int enable);
 long ttf_autotune_count(void);
 
@@ -299,6 +301,7 @@ typedef struct {
   long matvecs;       /* Krylov operator applications */
   long krylov_steps;  /* ZGEXPV integration steps */
   double seconds;     /* wall time of the call */
+  double flops;       /* algorithmic real flops of those operator applications */
 } ttf_ksl_stats;
 
 /* One KSL time step for dY/dt = A Y in TT format, argument layout of

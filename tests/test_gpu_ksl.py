@@ -120,3 +120,26 @@ def test_ksl_full_rank_tfim(ttlib, d, typ):
     Ad = O.mpo_to_dense(_split(crA, ra, n, n), ra, n, n)
     ex = sl.expm(tau * Ad) @ y0
     assert np.linalg.norm(x - ex) <= 1e-7 * np.linalg.norm(ex)
+
+
+def test_ksl_padded_product_start(ttlib):
+    """The fixed-rank start of the TFIM benchmark: the all-up product state
+    zero-padded to full rank (d = 8, ranks 2^k).  The padded directions are
+    completed by the device QR; at full rank the projector-splitting step is
+    exact, so several steps must match exp(t A) y0 and keep the energy
+    -(d-1) of the all-up state (H commutes with exp(-i t H))."""
+    import ttfort
+    import mpo
+    d, tau, steps = 8, 0.05, 4
+    crA, ra, n = mpo.tfim_schroedinger(d)
+    cr, ry = mpo.product_state_padded(d, 64)
+    y0 = O.tt_full(_split(cr, ry, n), n, ry)
+    Ad = O.mpo_to_dense(_split(crA, ra, n, n), ra, n, n)
+    for _ in range(steps):
+        cr, ry, _ = ttfort.ztt_ksl(n, ra, crA, cr, ry, tau, order=30)
+    x = O.tt_full(_split(cr, ry, n), n, ry)
+    ex = sl.expm(steps * tau * Ad) @ y0
+    assert np.linalg.norm(x - ex) <= 1e-7 * np.linalg.norm(ex)
+    H = 1j * Ad
+    energy = (x.conj() @ H @ x / (x.conj() @ x)).real
+    assert abs(energy + (d - 1)) <= 1e-8 * (d - 1)

@@ -11,6 +11,13 @@ struct Shape {
   int rx1, m, rx2, ry1, n, ry2, ra1, ra2;
 };
 
+// Real flops of one bfun3 application to one vector (the three GEMM stages);
+// a complex application costs 4 times this.
+inline double bfun3_flops(const Shape& s) {
+  return 2.0 * s.rx1 * s.m * s.rx2 * s.ra2 * s.ry2 + 2.0 * s.ra1 * s.n * s.m * s.ra2 * s.ry2 * s.rx1 +
+         2.0 * s.ry1 * s.n * s.ry2 * s.rx1 * s.ra1;
+}
+
 template <typename T>
 void bfun3_chain(const Shape& s, int nvec, const T* phi1, const T* A, const T* phi2, const T* x,
                  int64_t ldx, T* y, int64_t ldy, hipStream_t st);

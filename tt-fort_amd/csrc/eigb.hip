@@ -162,13 +162,7 @@ extern "C" int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, con
                     k + 1, d, N, es.matvecs, es.res, ermax, fvold - fv);
       total_mv += es.matvecs;
       sweep_mv += es.matvecs;
-      {
-        const Shape& s = op.s;
-        const double f = 2.0 * s.rx1 * s.m * s.rx2 * s.ra2 * s.ry2 +
-                         2.0 * s.ra1 * s.n * s.m * s.ra2 * s.ry2 * s.rx1 +
-                         2.0 * s.ry1 * s.n * s.ry2 * s.rx1 * s.ra1;
-        sweep_flops += f * es.matvecs;
-      }
+      sweep_flops += bfun3_flops(op.s) * es.matvecs;
       fvold = fv;
       if (dir < 0 && k > 0) {
         // (tt_eigb.f90:305-355) M = X^T viewed as (B*ry_k) x (n_k*ry_{k+1})

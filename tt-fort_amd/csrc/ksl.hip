@@ -736,6 +736,7 @@ extern "C" int ttf_ztt_ksl(int d, const int* n, const int* m, const int* ra, con
     zc* phit = bphit.get((size_t)R * R * ramax);
     zc* work = bwork.get(core_cap + (size_t)R * R * ramax);
     long nmult = 0, nsteps = 0, nsolves = 0;
+    double flops = 0.0;   // algorithmic real flops of the Krylov operator applications
     // initial left-to-right QR and left interfaces (tt_ksl.f90:538-556)
     for (int i = 0; i < d - 1; ++i) {
       const int rows = ry[i] * n[i], cols = ry[i + 1];
@@ -763,6 +764,7 @@ extern "C" int ttf_ztt_ksl(int d, const int* n, const int* m, const int* ra, con
       nmult += info.nmult;
       nsteps += info.nstep;
       ++nsolves;
+      flops += 4.0 * info.nmult * bfun3_flops(s);
     };
     auto s_exp = [&](const SOp& so, const zc* v, zc* w, double t) {
       const int64_t N = (int64_t)so.ry1 * so.ry2;
@@ -773,6 +775,8 @@ extern "C" int ttf_ztt_ksl(int d, const int* n, const int* m, const int* ra, con
       nmult += info.nmult;
       nsteps += info.nstep;
       ++nsolves;
+      flops += 4.0 * info.nmult *
+               (2.0 * so.rx1 * so.rx2 * so.ra * so.ry2 + 2.0 * so.ry1 * so.ry2 * so.rx1 * so.ra);
     };
     auto ztrans = [&](int rows, int cols, const zc* A, zc* B) {   // B (cols x rows) = A^T
       hipLaunchKernelGGL(ztranspose_kernel, dim3((unsigned)ceil_div((int64_t)rows * cols, 256)), dim3(256), 0, st, rows,
@@ -846,6 +850,7 @@ extern "C" int ttf_ztt_ksl(int d, const int* n, const int* m, const int* ra, con
       stats->nsolves = nsolves;
       stats->matvecs = nmult;
       stats->krylov_steps = nsteps;
+      stats->flops = flops;
       stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
   });

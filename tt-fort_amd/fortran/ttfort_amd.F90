@@ -46,7 +46,7 @@ module ttfort_amd
 
   type, bind(c) :: ttf_ksl_stats
     integer(c_long) :: nsolves, matvecs, krylov_steps
-    real(c_double) :: seconds
+    real(c_double) :: seconds, flops
   end type ttf_ksl_stats
 
   interface

@@ -61,6 +61,31 @@ def tfim(d, g=1.0, J=1.0):
     return np.concatenate(cores), ra, np.full(d, 2, dtype=np.int64)
 
 
+def tfim_schroedinger(d, g=1.0, J=1.0):
+    """A = -i H for the transverse-field Ising chain (the reference's time
+    stepping convention: dY/dt = A Y with the factor i carried by A, as in
+    test_ksl.dat).  Returns (crA complex128, ra, n)."""
+    crH, ra, n = tfim(d, g, J)
+    # -i scales the operator: put it on the first core only
+    sz0 = int(ra[0] * n[0] * n[0] * ra[1])
+    crA = crH.astype(np.complex128)
+    crA[:sz0] *= -1j
+    return crA, ra, n
+
+
+def product_state_padded(d, r, index=0, n=2):
+    """The product state e_index x ... x e_index as a TT with ranks
+    min(r, n^k, n^(d-k)), zero-padded beyond the first rank component (the
+    fixed-rank start of a KSL run).  Returns (cr complex128, ry)."""
+    ry = [1] + [int(min(r, n ** k, n ** (d - k))) for k in range(1, d)] + [1]
+    cores = []
+    for k in range(d):
+        c = np.zeros((ry[k], n, ry[k + 1]), dtype=np.complex128)
+        c[0, index, 0] = 1.0
+        cores.append(c.reshape(-1, order="F"))
+    return np.concatenate(cores), np.array(ry, dtype=np.int64)
+
+
 def random_block_tt(d, n, r0, B, seed):
     """Initial block TT: N(0,1) cores of rank r0 from PCG64(seed), last rank B."""
     rng = np.random.default_rng(seed)

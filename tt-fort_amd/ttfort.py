@@ -132,7 +132,8 @@ _sig("ttf_tt_amen_solve", _c_int, _c_int, _IP, _IP, _IP, _IP, _DP, _DP, _DP, _IP
 
 
 class KslStats(C.Structure):
-    _fields_ = [("nsolves", C.c_long), ("matvecs", C.c_long), ("krylov_steps", C.c_long), ("seconds", C.c_double)]
+    _fields_ = [("nsolves", C.c_long), ("matvecs", C.c_long), ("krylov_steps", C.c_long), ("seconds", C.c_double),
+                ("flops", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
