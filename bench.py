@@ -293,12 +293,60 @@ def run_ksl(args):
                             host_transfer_bytes_per_step=int(2 * 16 * cr.size), wall_s=round(wall, 2)))
 
 
+def run_amen(args):
+    """Variable-coefficient diffusion -div(a grad u) = 1 in 6D, n=64 per mode,
+    a = 1 + 0.5 tanh(sum c_k x_k) (c_k ~ U(-1,1), seed 0) as a rank-3 TT
+    coefficient, 7-point stencil TT-matrix of rank 6; rank-1 all-ones rhs,
+    rank-2 N(0,1) start (seed 1), tol 1e-8, max rank 64, block-Jacobi 'c',
+    nrestart 40, niters 2, kick 4.  W untimed solves, then K timed solves to
+    convergence; a step is one solve, and the global FP64 residual of the
+    last result is computed in TT format on the host."""
+    import ttfort
+    import mpo
+    import ttops
+    lib = ttfort.load()
+    ttfort.check(lib.ttf_init(0), lib)
+    d, n, tol, rmax = 6, 64, 1e-8, 64
+    crA, ra, nn, info = mpo.diffusion_7pt(d, n, seed=0, coef_rank=3)
+    crY, ry = mpo.ones_tt(d, n)
+    crX, rx = mpo.random_tt(d, n, 2, seed=1)
+
+    def solve():
+        return ttfort.tt_amen_solve(nn, ra, crA, ry, crY, rx, crX, tol, kickrank=4, nswp=20, prec=args.prec,
+                                    nrestart=40, niters=2, rmax=rmax, verb=args.verb)
+
+    for _ in range(args.warmup):
+        solve()
+    secs, flops, sweeps = 0.0, 0.0, 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res, rx_out, st = solve()
+        secs += st["seconds"]
+        flops += st["flops"]
+        sweeps += st["sweeps"]
+    wall = time.perf_counter() - t0
+    resid = ttops.residual(crA, ra, nn, res, rx_out, crY, ry)
+    return dict(metric="sweep time + local-matvec TFLOP/s (FP64 AMEn solve, 6D variable-coefficient diffusion)",
+                value=round(flops / secs / 1e12, 4), unit="TFLOP/s", n_gpus=1, steps=args.steps,
+                warmup=args.warmup, ms_per_step=round(1e3 * wall / args.steps, 1), higher_is_better=True,
+                scaling="weak", vs_baseline=None, dtype="f64", data="synthetic",
+                config=dict(workload="tt_amen_solve 6D diffusion n=64 coef rank 3 (operator rank 6) tol 1e-8 "
+                                     "rmax 64 prec %s" % args.prec,
+                            d=d, n=n, tol=tol, rmax=rmax, prec=args.prec, sweeps_per_solve=sweeps / args.steps,
+                            ms_per_sweep=round(1e3 * secs / max(sweeps, 1), 1), max_rank=st["max_rank"],
+                            matvecs=st["total_matvecs"], coef_maxerr=info["coef_maxerr"],
+                            global_residual=resid, last_local_residual=st["last_res_max"],
+                            solve_s=round(st["solve_seconds"], 3), trunc_s=round(st["trunc_seconds"], 3),
+                            phi_s=round(st["phi_seconds"], 3), wall_s=round(wall, 2)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="eigb", choices=["eigb", "matvec", "ksl"])
+    ap.add_argument("--workload", default="eigb", choices=["eigb", "matvec", "ksl", "amen"])
+    ap.add_argument("--prec", default="c", help="AMEn local preconditioner: n, c, l, r")
     ap.add_argument("--eigb-d", type=int, default=64)
     ap.add_argument("--eigb-rmax", type=int, default=128)
     ap.add_argument("--verb", type=int, default=1, help="per-step progress lines on stderr")
@@ -309,12 +357,12 @@ def main():
         print("bench.py: this build measures one GPU only (the local-solver path has no "
               "multi-GPU mode); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    default_steps = {"eigb": (2, 2), "matvec": (200, 20), "ksl": (10, 1)}[args.workload]
+    default_steps = {"eigb": (2, 2), "matvec": (200, 20), "ksl": (10, 1), "amen": (1, 1)}[args.workload]
     if args.steps is None:
         args.steps = default_steps[0]
     if args.warmup is None:
         args.warmup = default_steps[1]
-    out = {"matvec": run_matvec, "eigb": run_eigb, "ksl": run_ksl}[args.workload](args)
+    out = {"matvec": run_matvec, "eigb": run_eigb, "ksl": run_ksl, "amen": run_amen}[args.workload](args)
     print(json.dumps(out), flush=True)
 
 

@@ -258,6 +258,7 @@ typedef struct {
   double solve_seconds;  /* local residuals and GMRES solves */
   double trunc_seconds;  /* SVD, residual-based truncation, kick, core moves */
   double phi_seconds;    /* interface updates */
+  double flops;          /* algorithmic flops of the local matvecs */
 } ttf_amen_stats;
 
 /* AMEn solver for A x = y in TT format, argument layout of tt_amen_solve

@@ -127,3 +127,23 @@ def test_amen_laplace(ttlib, d, n, prec, kick):
     Ax = O.mpo_apply_full(_cores(crA, ra, nn, nn), ra, nn, nn, x)
     assert np.linalg.norm(Ax - y) <= 1e-6 * np.linalg.norm(y)
     assert st["sweeps"] >= 1 and st["last_res_max"] < eps
+
+
+@pytest.mark.parametrize("d,n,prec", [(3, 12, "c"), (4, 8, "n"), (4, 8, "c")])
+def test_amen_diffusion(ttlib, d, n, prec):
+    """Variable-coefficient 7-point diffusion (rank-3 coefficient, operator
+    rank 6) against a dense solve of the same TT-matrix, and the FP64 TT
+    residual of the returned solution."""
+    import ttfort
+    import ttops
+    crA, ra, nn, _ = mpo.diffusion_7pt(d, n, seed=5, coef_rank=3)
+    crY, ry = mpo.ones_tt(d, n)
+    crX, rx = mpo.random_tt(d, n, 2, seed=1)
+    eps = 1e-8
+    res, rx_out, st = ttfort.tt_amen_solve(nn, ra, crA, ry, crY, rx, crX, eps, kickrank=4, prec=prec, nswp=20)
+    x = O.tt_full(_cores(res, rx_out, nn), nn, rx_out)
+    A = O.mpo_to_dense(_cores(crA, ra, nn, nn), ra, nn, nn)
+    x_ref = np.linalg.solve(A, np.ones(n ** d))
+    assert np.linalg.norm(x - x_ref) <= 1e-6 * np.linalg.norm(x_ref), (np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref), st)
+    assert ttops.residual(crA, ra, nn, res, rx_out, crY, ry) <= 1e-6
+    assert st["flops"] > 0 and st["last_res_max"] < eps

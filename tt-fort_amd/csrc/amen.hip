@@ -180,6 +180,7 @@ extern "C" int ttf_tt_amen_solve(int d, const int* n, const int* m, const int* r
     std::vector<double> hs;
     double t_solve = 0, t_trunc = 0, t_phi = 0;
     long mv_total = 0;
+    double flops_total = 0.0;   // algorithmic flops of those local matvecs
     // ---- initial QR, left to right (:604-645)
     for (int i = 0; i < d - 1; ++i) {
       const int mrow = rx[i] * n[i], ncol = rx[i + 1];
@@ -221,6 +222,7 @@ extern "C" int ttf_tt_amen_solve(int d, const int* n, const int* m, const int* r
       // residual of the current core (:733-740)
       op.apply(1, x, N, res, N, st);
       ++mv_total;
+      flops_total += bfun3_flops(op.s);
       hipLaunchKernelGGL(axpy_kernel, dim3(grid_for(N)), dim3(256), 0, st, N, -1.0, rhs, res);
       const double norm_rhs = dev_norm(N, rhs, scr, st);
       TTF_REQUIRE(norm_rhs > 0.0, "ttf_tt_amen_solve: zero local right-hand side at core %d", i + 1);
@@ -238,11 +240,13 @@ extern "C" int ttf_tt_amen_solve(int d, const int* n, const int* m, const int* r
         }
         gs = gmres_solve(op, P, res, sol, std::min<int64_t>(nrestart, N), gtol, niters, st);
         mv_total += gs.matvecs;
+        flops_total += bfun3_flops(op.s) * gs.matvecs;
         if (ptype != 'n') jac_apply(ptype, r1, nn_i, r2, P.jacs, sol, dx, st);
         else TTF_HIP(hipMemcpyAsync(dx, sol, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
         hipLaunchKernelGGL(axpy_kernel, dim3(grid_for(N)), dim3(256), 0, st, N, -1.0, dx, x);
         op.apply(1, x, N, res, N, st);
         ++mv_total;
+        flops_total += bfun3_flops(op.s);
         hipLaunchKernelGGL(axpy_kernel, dim3(grid_for(N)), dim3(256), 0, st, N, -1.0, rhs, res);
         res_new = dev_norm(N, res, scr, st) / norm_rhs;
         const double nx = dev_norm(N, x, scr, st);
@@ -276,6 +280,7 @@ extern "C" int ttf_tt_amen_solve(int d, const int* n, const int* m, const int* r
               gemm_nn(mrow, ncol, c0 - t, 1.0, U, mrow, Rm, k, 0.0, cand + (size_t)N * t, mrow, st);
             op.apply(nc, cand, N, candy, N, st);
             mv_total += nc;
+            flops_total += bfun3_flops(op.s) * nc;
             hipLaunchKernelGGL(resid_norms_kernel, dim3(nc), dim3(256), 0, st, N, candy, N, rhs, scr);
             std::vector<double> hr(nc);
             TTF_HIP(hipMemcpyAsync(hr.data(), scr, sizeof(double) * nc, hipMemcpyDeviceToHost, st));
@@ -353,6 +358,7 @@ extern "C" int ttf_tt_amen_solve(int d, const int* n, const int* m, const int* r
               gemm_nn(mrow, ncol, c0 - t, 1.0, Rm, mrow, U, ncol, 0.0, cand + (size_t)N * t, mrow, st, 'N', 'T');
             op.apply(nc, cand, N, candy, N, st);
             mv_total += nc;
+            flops_total += bfun3_flops(op.s) * nc;
             hipLaunchKernelGGL(resid_norms_kernel, dim3(nc), dim3(256), 0, st, N, candy, N, rhs, scr);
             std::vector<double> hr(nc);
             TTF_HIP(hipMemcpyAsync(hr.data(), scr, sizeof(double) * nc, hipMemcpyDeviceToHost, st));
@@ -476,6 +482,7 @@ extern "C" int ttf_tt_amen_solve(int d, const int* n, const int* m, const int* r
     if (stats) {
       stats->sweeps = sweeps_done;
       stats->total_matvecs = mv_total;
+      stats->flops = flops_total;
       stats->last_res_max = last_res_max;
       stats->last_err_max = last_err_max;
       int mx = 0;

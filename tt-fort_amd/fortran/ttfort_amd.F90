@@ -41,7 +41,7 @@ module ttfort_amd
     integer(c_long) :: total_matvecs
     real(c_double) :: last_res_max, last_err_max
     integer(c_int) :: max_rank
-    real(c_double) :: seconds, solve_seconds, trunc_seconds, phi_seconds
+    real(c_double) :: seconds, solve_seconds, trunc_seconds, phi_seconds, flops
   end type ttf_amen_stats
 
   type, bind(c) :: ttf_ksl_stats

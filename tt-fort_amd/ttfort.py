@@ -113,7 +113,8 @@ _sig("ttf_tt_eigb", _c_int, _c_int, _IP, _IP, _IP, _DP, _DP, _IP, C.c_double, _c
 class AmenStats(C.Structure):
     _fields_ = [("sweeps", C.c_int), ("total_matvecs", C.c_long), ("last_res_max", C.c_double),
                 ("last_err_max", C.c_double), ("max_rank", C.c_int), ("seconds", C.c_double),
-                ("solve_seconds", C.c_double), ("trunc_seconds", C.c_double), ("phi_seconds", C.c_double)]
+                ("solve_seconds", C.c_double), ("trunc_seconds", C.c_double), ("phi_seconds", C.c_double),
+                ("flops", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
