@@ -259,6 +259,8 @@ typedef struct {
   double trunc_seconds;  /* SVD, residual-based truncation, kick, core moves */
   double phi_seconds;    /* interface updates */
   double flops;          /* algorithmic flops of the local matvecs */
+  long learned_solves;   /* local solves with the learned preconditioner (prec 'm') */
+  long learned_fallbacks;/* of those, solves finished by plain GMRES */
 } ttf_amen_stats;
 
 /* AMEn solver for A x = y in TT format, argument layout of tt_amen_solve
@@ -267,7 +269,9 @@ typedef struct {
  * rx(d+1) (in/out) the initial guess, eps the tolerance.  The reference's
  * optional arguments are explicit: kickrank (4; -1 truncates without kick,
  * 0 only re-orthogonalises), nswp (20), verb (1), prec ('n', 'c', 'l',
- * 'r'), nrestart (40), niters (2).  rmax > 0 caps the TT ranks (not in the
+ * 'r', or 'm': the learned block preconditioner below in flexible GMRES,
+ * with plain GMRES finishing any solve it leaves above the tolerance; not
+ * in the reference), nrestart (40), niters (2).  rmax > 0 caps the TT ranks (not in the
  * reference; <= 0 means a capacity of 256).  The result cores (the
  * reference's tt_adapt_als::result_core) go to `result` (capacity
  * result_cap doubles; size in *result_size).  Host pointers;
@@ -276,6 +280,54 @@ int ttf_tt_amen_solve(int d, const int* n, const int* m, const int* ry, const in
                       const double* crY, const double* crX0, int* rx, double eps, int kickrank, int nswp,
                       int verb, char prec, int nrestart, int niters, int rmax, double* result,
                       int64_t result_cap, int64_t* result_size, ttf_amen_stats* stats);
+
+/* ------------------------------------- learned block preconditioner (MLP) */
+/* No counterpart in the reference: a replacement for the block-Jacobi 'c'
+ * solve of the local GMRES (tt_linalg.f90:397-429, :497-533) by a small
+ * network trained on the GPU.  Per diagonal block B_b (b = (b1, b2), mode
+ * size n = 64) of the local matrix Phi1 A Phi2, the features are the three
+ * central diagonals of B_b divided by s_b = mean |diag B_b| and the residual
+ * slice r_b / |r_b| (256 values, one float column per block); the network
+ * y = W3^T relu(W2^T relu(W1^T x + b1) + b2) + b3 + x_r / x_d (256-256-256-64,
+ * bf16 weights, fp32 activations and accumulation; the last term is the
+ * point-Jacobi solution of the block, residual features over diagonal
+ * features) predicts the normalised block correction, returned as
+ * y_b |r_b| / s_b.  Parameters are one flat float
+ * vector [W1 (256x256), b1, W2 (256x256), b2, W3 (256x64), b3], matrices
+ * column-major; ttf_mlp_param_count() gives its length. */
+long ttf_mlp_param_count(void);
+/* Uniform(+-1/sqrt(fan_in)) W1, W2 from `seed`; W3 and the biases zero,
+ * so the untrained network is the point-Jacobi solve. */
+int ttf_mlp_init(unsigned seed);
+/* Host parameter vector in/out; setting resets the Adam state. */
+int ttf_mlp_set_params(const float* params);
+int ttf_mlp_get_params(float* params);
+/* Y (64 x M) = network(X (256 x M)); device pointers. */
+int ttf_mlp_forward(int M, const float* X, float* Y);
+/* Loss L = sum (Y - T)^2 / (64 M) over the batch and its gradient with
+ * respect to the parameters (taken at the bf16 weights); X, T device,
+ * loss and grad (param_count floats) host. */
+int ttf_mlp_loss_grad(int M, const float* X, const float* T, double* loss, float* grad);
+/* Adam training (lr, beta 0.9 / 0.999, eps 1e-8) over `epochs` passes of
+ * shuffled minibatches of `batch` columns of the device data X (256 x n),
+ * T (64 x n); the mean batch loss of each epoch goes to loss_hist. */
+int ttf_mlp_train(long nsamples, const float* X, const float* T, int epochs, int batch, float lr, unsigned seed,
+                  double* loss_hist, int hist_cap, int* hist_len);
+/* Features X (256 x rx1 rx2) and output scales |r_b| / s_b of the local
+ * vector v (rx1, n, rx2) for the operator Phi1 A Phi2 (device pointers). */
+int ttf_mlp_features(int rx1, int n, int rx2, int ra1, int ra2, const double* phi1, const double* A,
+                     const double* phi2, const double* v, float* X, float* scale);
+/* z = learned preconditioner applied to v (device pointers). */
+int ttf_mlp_precond(int rx1, int n, int rx2, int ra1, int ra2, const double* phi1, const double* A,
+                    const double* phi2, const double* v, double* z);
+/* Training-pair collection: while active, every block-Jacobi 'c' solve of
+ * the local GMRES (any ttf_tt_amen_solve with prec 'c', or ttf_dgmresr)
+ * records up to per_apply sampled blocks as (features, normalised exact
+ * block correction) into library-owned device buffers of `capacity`
+ * columns; ttf_mlp_collected returns them. */
+int ttf_mlp_collect_begin(long capacity, int per_apply);
+int ttf_mlp_collect_end(long* count);
+int ttf_mlp_collected(float** X, float** T, long* count);
 
 /* ------------------------------------------------------ KSL time stepping */
 /* exp(t H) for a small complex m x m column-major device matrix (m <= 40):

@@ -4,6 +4,8 @@
 
 #include "solvers.h"
 
+#include <functional>
+
 namespace ttf {
 
 // 'N'/'C'/'L'/'R' -> lower case (the reference accepts either case).
@@ -16,6 +18,12 @@ struct JacPrec {
   char type = 'n';
   int rx1 = 0, n = 0, rx2 = 0;
   const double* jacs = nullptr;
+  // the local operator the blocks came from (for recording training pairs
+  // of the learned preconditioner; may be left null)
+  int ra1 = 0, ra2 = 0;
+  const double* phi1 = nullptr;
+  const double* A = nullptr;
+  const double* phi2 = nullptr;
 };
 size_t jac_size(char type, int rx1, int n, int rx2);   // doubles
 void jac_gen(char type, int rx1, int n, int rx2, int ra1, int ra2, const double* Phi1, const double* A,
@@ -35,6 +43,14 @@ struct GmresStats {
 // the initial residual norm (tt_linalg.f90:797-803).  sol starts at zero.
 GmresStats gmres_solve(const LocalOp& op, const JacPrec& P, const double* rhs, double* sol, int nrestart,
                        double tol, int niters, hipStream_t st);
+
+// Flexible GMRES (right preconditioning by a possibly nonlinear map M, with
+// the preconditioned directions kept: x = sum_j y_j M(v_j)).  Solves
+// B x = rhs from x = 0 with the same restart / stopping rules as
+// gmres_solve; x is the solution itself.
+typedef std::function<void(const double* v, double* z)> PrecFn;
+GmresStats gmres_flex(const LocalOp& op, const PrecFn& M, const double* rhs, double* x, int nrestart, double tol,
+                      int niters, hipStream_t st);
 
 // Pivoted incomplete Cholesky of the Gram matrix of the n "items" of dA
 // (columns of the m x n dA for trans 'N', rows of the n x m dA for 'T'),

@@ -18,6 +18,7 @@
 //   * stop after the sweep following the first one whose largest local
 //     residual is below eps, which runs without kick (:1056-1101).
 #include "amen.h"
+#include "mlp.h"
 
 #include <algorithm>
 #include <chrono>
@@ -79,6 +80,12 @@ __global__ void axpy_kernel(int64_t n, double a, const double* __restrict__ x, d
     y[e] += a * x[e];
 }
 
+// y = x - y
+__global__ void rsub_kernel(int64_t n, const double* __restrict__ x, double* y) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    y[e] = x[e] - y[e];
+}
+
 int grid_for(int64_t n) { return (int)std::min<int64_t>(std::max<int64_t>(ceil_div(n, 256), 1), 16384); }
 
 double dev_norm(int64_t N, const double* v, double* scratch, hipStream_t st) {
@@ -114,7 +121,9 @@ extern "C" int ttf_tt_amen_solve(int d, const int* n, const int* m, const int* r
     TTF_REQUIRE(kickrank >= -1, "ttf_tt_amen_solve: kickrank=%d", kickrank);
     TTF_REQUIRE(nrestart >= 1 && niters >= 1, "ttf_tt_amen_solve: nrestart=%d niters=%d", nrestart, niters);
     const char ptype = norm_ptype(prec);
-    TTF_REQUIRE(ptype == 'n' || ptype == 'c' || ptype == 'l' || ptype == 'r', "ttf_tt_amen_solve: prec '%c'", prec);
+    TTF_REQUIRE(ptype == 'n' || ptype == 'c' || ptype == 'l' || ptype == 'r' || ptype == 'm',
+                "ttf_tt_amen_solve: prec '%c'", prec);
+    TTF_REQUIRE(ptype != 'm' || mlp_ready(), "ttf_tt_amen_solve: prec 'm' needs the network parameters set first");
     Timer total;
     hipStream_t st = stream();
     const double eps2 = eps / std::sqrt((double)d);   // :576
@@ -181,6 +190,7 @@ extern "C" int ttf_tt_amen_solve(int d, const int* n, const int* m, const int* r
     double t_solve = 0, t_trunc = 0, t_phi = 0;
     long mv_total = 0;
     double flops_total = 0.0;   // algorithmic flops of those local matvecs
+    long mlp_solves = 0, mlp_fallbacks = 0;
     // ---- initial QR, left to right (:604-645)
     for (int i = 0; i < d - 1; ++i) {
       const int mrow = rx[i] * n[i], ncol = rx[i + 1];
@@ -232,17 +242,53 @@ extern "C" int ttf_tt_amen_solve(int d, const int* n, const int* m, const int* r
       const double gtol = eps2 / (err * 2.0);
       GmresStats gs;
       if (gtol < 1.0) {
-        JacPrec P;
-        if (ptype != 'n') {
-          double* jacs = bjacs.get(jac_size(ptype, r1, nn_i, r2));
-          jac_gen(ptype, r1, nn_i, r2, ra[i], ra[i + 1], phiA[i].p, dA + pa[i], phiA[i + 1].p, jacs, st);
-          P = JacPrec{ptype, r1, nn_i, r2, jacs};
+        const int nrs = (int)std::min<int64_t>(nrestart, N);
+        if (ptype == 'm') {
+          // learned block preconditioner in flexible GMRES; plain GMRES takes
+          // over from its iterate when it misses the tolerance
+          const double* p1 = phiA[i].p;
+          const double* Ai = dA + pa[i];
+          const double* p2 = phiA[i + 1].p;
+          const int ra1 = ra[i], ra2 = ra[i + 1];
+          PrecFn M = [&, p1, Ai, p2, ra1, ra2, r1, nn_i, r2](const double* v, double* z) {
+            mlp_precond(r1, nn_i, r2, ra1, ra2, p1, Ai, p2, v, z, st);
+          };
+          gs = gmres_flex(op, M, res, dx, nrs, gtol, niters, st);
+          mv_total += gs.matvecs;
+          flops_total += bfun3_flops(op.s) * gs.matvecs;
+          ++mlp_solves;
+          if (!(gs.relres < gtol)) {
+            ++mlp_fallbacks;
+            if (!std::isfinite(gs.relres)) TTF_HIP(hipMemsetAsync(dx, 0, sizeof(double) * N, st));
+            // remaining residual r2 = res - B dx (in candy), solved by plain
+            // GMRES to the same absolute target (correction in cand)
+            op.apply(1, dx, N, candy, N, st);
+            hipLaunchKernelGGL(rsub_kernel, dim3(grid_for(N)), dim3(256), 0, st, N, res, candy);
+            const double n0 = dev_norm(N, res, scr, st), n2 = dev_norm(N, candy, scr, st);
+            int extra = 1;
+            if (n2 > 0.0) {
+              GmresStats g2 = gmres_solve(op, JacPrec{}, candy, cand, nrs, std::min(1.0, gtol * n0 / n2), niters, st);
+              hipLaunchKernelGGL(axpy_kernel, dim3(grid_for(N)), dim3(256), 0, st, N, 1.0, cand, dx);
+              extra += g2.matvecs;
+              gs.relres = g2.relres * n2 / n0;
+            }
+            mv_total += extra;
+            flops_total += bfun3_flops(op.s) * extra;
+            gs.matvecs += extra;
+          }
+        } else {
+          JacPrec P;
+          if (ptype != 'n') {
+            double* jacs = bjacs.get(jac_size(ptype, r1, nn_i, r2));
+            jac_gen(ptype, r1, nn_i, r2, ra[i], ra[i + 1], phiA[i].p, dA + pa[i], phiA[i + 1].p, jacs, st);
+            P = JacPrec{ptype, r1, nn_i, r2, jacs, ra[i], ra[i + 1], phiA[i].p, dA + pa[i], phiA[i + 1].p};
+          }
+          gs = gmres_solve(op, P, res, sol, nrs, gtol, niters, st);
+          mv_total += gs.matvecs;
+          flops_total += bfun3_flops(op.s) * gs.matvecs;
+          if (ptype != 'n') jac_apply(ptype, r1, nn_i, r2, P.jacs, sol, dx, st);
+          else TTF_HIP(hipMemcpyAsync(dx, sol, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
         }
-        gs = gmres_solve(op, P, res, sol, std::min<int64_t>(nrestart, N), gtol, niters, st);
-        mv_total += gs.matvecs;
-        flops_total += bfun3_flops(op.s) * gs.matvecs;
-        if (ptype != 'n') jac_apply(ptype, r1, nn_i, r2, P.jacs, sol, dx, st);
-        else TTF_HIP(hipMemcpyAsync(dx, sol, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
         hipLaunchKernelGGL(axpy_kernel, dim3(grid_for(N)), dim3(256), 0, st, N, -1.0, dx, x);
         op.apply(1, x, N, res, N, st);
         ++mv_total;
@@ -483,6 +529,8 @@ extern "C" int ttf_tt_amen_solve(int d, const int* n, const int* m, const int* r
       stats->sweeps = sweeps_done;
       stats->total_matvecs = mv_total;
       stats->flops = flops_total;
+      stats->learned_solves = mlp_solves;
+      stats->learned_fallbacks = mlp_fallbacks;
       stats->last_res_max = last_res_max;
       stats->last_err_max = last_err_max;
       int mx = 0;

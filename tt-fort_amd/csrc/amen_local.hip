@@ -8,6 +8,7 @@
 //   * the half contractions of the local operator used to build the kick
 //     block (tt_linalg.f90:210-256, dbfun3_right / dbfun3_left).
 #include "amen.h"
+#include "mlp.h"
 
 #include <algorithm>
 #include <cmath>
@@ -282,6 +283,8 @@ GmresStats gmres_solve(const LocalOp& op, const JacPrec& P, const double* rhs, d
     const double* in = v;
     if (P.type != 'n') {
       jac_apply(P.type, P.rx1, P.n, P.rx2, P.jacs, v, t, st);
+      if (P.type == 'c' && P.phi1 && mlp_collecting())
+        mlp_collect_pair(P.rx1, P.n, P.rx2, P.ra1, P.ra2, P.phi1, P.A, P.phi2, v, t, st);
       in = t;
     }
     op.apply(1, in, N, out, N, st);
@@ -355,6 +358,93 @@ GmresStats gmres_solve(const LocalOp& op, const JacPrec& P, const double* rhs, d
     gemm_launch(gm, st);
     TTF_HIP(hipStreamSynchronize(st));   // hb is reused by the next restart
     if (curres < tol) break;
+  }
+  gs.relres = curres;
+  return gs;
+}
+
+GmresStats gmres_flex(const LocalOp& op, const PrecFn& M, const double* rhs, double* x, int nrestart, double tol,
+                      int niters, hipStream_t st) {
+  GmresStats gs;
+  const int64_t N = op.N();
+  const int m = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)nrestart, N, (int64_t)63}));
+  // V (N x (m+1)) | u ; Z (N x m) holds the preconditioned directions
+  double* V = (double*)arena(kArenaSolver, sizeof(double) * (size_t)N * (m + 2));
+  double* u = V + (size_t)N * (m + 1);
+  double* Z = (double*)arena(kArenaZ, sizeof(double) * (size_t)N * m);
+  double* hb = (double*)arena(kArenaSmall, sizeof(double) * (2 * 64 + 8));
+  TTF_HIP(hipMemsetAsync(x, 0, sizeof(double) * N, st));
+  const int gN = grid_for(N);
+  std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), y(m), hh(2 * 64 + 8);
+  double nrmrhs = 0.0, curres = 1.0;
+  for (int it = 0; it < niters; ++it) {
+    gs.it = it;
+    if (it == 0) {
+      TTF_HIP(hipMemcpyAsync(V, rhs, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+    } else {
+      op.apply(1, x, N, V, N, st);
+      ++gs.matvecs;
+      hipLaunchKernelGGL(axpby_kernel, dim3(gN), dim3(256), 0, st, N, 1.0, rhs, -1.0, V);
+      TTF_HIP(hipGetLastError());
+    }
+    tsgram((int)N, 1, 1, V, N, V, N, hb, 1, st);
+    TTF_HIP(hipMemcpyAsync(hh.data(), hb, sizeof(double), hipMemcpyDeviceToHost, st));
+    TTF_HIP(hipStreamSynchronize(st));
+    const double beta = std::sqrt(std::max(hh[0], 0.0));
+    if (it == 0) nrmrhs = beta;
+    if (!(beta > 0.0)) { curres = beta == 0.0 ? 0.0 : curres; break; }
+    hipLaunchKernelGGL(scale_copy_kernel, dim3(gN), dim3(256), 0, st, N, 1.0 / beta, V, V);
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int jj = 0;
+    for (int j = 0; j < m; ++j) {
+      double* zj = Z + (size_t)N * j;
+      M(V + (size_t)N * j, zj);
+      op.apply(1, zj, N, u, N, st);
+      ++gs.matvecs;
+      const int k = j + 1;
+      for (int pass = 0; pass < 2; ++pass) {
+        double* h = hb + 64 * pass;
+        tsgram((int)N, k, 1, V, N, u, N, h, k, st);
+        GemmOp<double> gm = colmajor<double>('N', 'N', (int)N, 1, k, -1.0, V, N, h, k, 1.0, u, N);
+        gemm_launch(gm, st);
+      }
+      tsgram((int)N, 1, 1, u, N, u, N, hb + 128, 1, st);
+      TTF_HIP(hipMemcpyAsync(hh.data(), hb, sizeof(double) * 129, hipMemcpyDeviceToHost, st));
+      TTF_HIP(hipStreamSynchronize(st));
+      double* col = &H[(size_t)(m + 1) * j];
+      for (int i = 0; i < k; ++i) col[i] = hh[i] + hh[64 + i];
+      const double hn = std::sqrt(std::max(hh[128], 0.0));
+      col[k] = hn;
+      for (int i = 0; i < j; ++i) {
+        const double a = col[i], b = col[i + 1];
+        col[i] = cs[i] * a + sn[i] * b;
+        col[i + 1] = -sn[i] * a + cs[i] * b;
+      }
+      const double r = std::hypot(col[j], col[k]);
+      cs[j] = r > 0.0 ? col[j] / r : 1.0;
+      sn[j] = r > 0.0 ? col[k] / r : 0.0;
+      col[j] = r;
+      col[k] = 0.0;
+      g[k] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      curres = std::fabs(g[k]) / nrmrhs;
+      jj = k;
+      if (!(curres >= tol) || hn == 0.0) break;   // also stops on a NaN from the preconditioner
+      if (j + 1 < m)
+        hipLaunchKernelGGL(scale_copy_kernel, dim3(gN), dim3(256), 0, st, N, 1.0 / hn, u, V + (size_t)N * (j + 1));
+    }
+    gs.j = jj;
+    for (int i = jj - 1; i >= 0; --i) {
+      double s = g[i];
+      for (int l = i + 1; l < jj; ++l) s -= H[(size_t)(m + 1) * l + i] * y[l];
+      y[i] = s / H[(size_t)(m + 1) * i + i];
+    }
+    TTF_HIP(hipMemcpyAsync(hb, y.data(), sizeof(double) * jj, hipMemcpyHostToDevice, st));
+    GemmOp<double> gm = colmajor<double>('N', 'N', (int)N, 1, jj, 1.0, Z, N, hb, jj, 1.0, x, N);
+    gemm_launch(gm, st);
+    TTF_HIP(hipStreamSynchronize(st));
+    if (!(curres >= tol)) break;
   }
   gs.relres = curres;
   return gs;

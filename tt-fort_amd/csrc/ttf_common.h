@@ -76,6 +76,10 @@ enum Arena {
   kArenaSplitK,      // split-K partial sums of the GEMM engine
   kArenaSolver2,
   kArenaDriver,      // sweep-driver staging (truncation factors)
+  kArenaMlp,         // learned preconditioner: features and activations
+  kArenaMlp2,        // learned preconditioner: gradients, loss partials, indices
+  kArenaMlp3,        // learned preconditioner: batch targets
+  kArenaZ,           // flexible GMRES: preconditioned directions
   kArenaCount
 };
 void* arena(Arena a, size_t bytes);

@@ -42,6 +42,7 @@ module ttfort_amd
     real(c_double) :: last_res_max, last_err_max
     integer(c_int) :: max_rank
     real(c_double) :: seconds, solve_seconds, trunc_seconds, phi_seconds, flops
+    integer(c_long) :: learned_solves, learned_fallbacks
   end type ttf_amen_stats
 
   type, bind(c) :: ttf_ksl_stats

@@ -114,7 +114,7 @@ class AmenStats(C.Structure):
     _fields_ = [("sweeps", C.c_int), ("total_matvecs", C.c_long), ("last_res_max", C.c_double),
                 ("last_err_max", C.c_double), ("max_rank", C.c_int), ("seconds", C.c_double),
                 ("solve_seconds", C.c_double), ("trunc_seconds", C.c_double), ("phi_seconds", C.c_double),
-                ("flops", C.c_double)]
+                ("flops", C.c_double), ("learned_solves", C.c_long), ("learned_fallbacks", C.c_long)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -146,6 +146,20 @@ _sig("ttf_zexp_bfun3", _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp,
 _sig("ttf_zqr", _c_int, _c_int, _c_int, _vp, _vp, _vp)
 _sig("ttf_ztt_ksl", _c_int, _c_int, _IP, _IP, _IP, _vp, _vp, _IP, C.c_double, _c_int, _c_int, _c_int, _c_int,
      _c_int, _c_int, _c_int, _vp, _c_i64, C.POINTER(_c_i64), C.POINTER(KslStats))
+
+_FP = C.POINTER(C.c_float)
+_sig("ttf_mlp_param_count", C.c_long)
+_sig("ttf_mlp_init", _c_int, C.c_uint)
+_sig("ttf_mlp_set_params", _c_int, _vp)
+_sig("ttf_mlp_get_params", _c_int, _vp)
+_sig("ttf_mlp_forward", _c_int, _c_int, _vp, _vp)
+_sig("ttf_mlp_loss_grad", _c_int, _c_int, _vp, _vp, _DP, _vp)
+_sig("ttf_mlp_train", _c_int, C.c_long, _vp, _vp, _c_int, _c_int, C.c_float, C.c_uint, _vp, _c_int, _IP)
+_sig("ttf_mlp_features", _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp)
+_sig("ttf_mlp_precond", _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp)
+_sig("ttf_mlp_collect_begin", _c_int, C.c_long, _c_int)
+_sig("ttf_mlp_collect_end", _c_int, C.POINTER(C.c_long))
+_sig("ttf_mlp_collected", _c_int, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(C.c_long))
 
 
 def _declare(lib):
@@ -547,3 +561,112 @@ def ztt_ksl(n, ra, crA, crY0, ry, tau, typ=2, order=8, usenrm=1, verb=0):
     check(lib.ttf_ztt_ksl(d, _ip(n_), _ip(n_), _ip(ra_), crA_.ctypes.data, crY_.ctypes.data, _ip(ry_), tau, 0, 0, 0,
                           verb, typ, order, usenrm, res.ctypes.data, cap, C.byref(size), C.byref(st)), lib)
     return res[:size.value].copy(), ry_, st.as_dict()
+
+
+# ---------------------------------------------------- learned preconditioner
+def mlp_param_count():
+    return int(load().ttf_mlp_param_count())
+
+
+def mlp_init(seed=0):
+    lib = load()
+    check(lib.ttf_mlp_init(seed), lib)
+
+
+def mlp_set_params(theta):
+    lib = load()
+    theta = np.ascontiguousarray(theta, dtype=np.float32)
+    if theta.size != mlp_param_count():
+        raise ValueError("parameter vector has %d entries, expected %d" % (theta.size, mlp_param_count()))
+    check(lib.ttf_mlp_set_params(theta.ctypes.data), lib)
+
+
+def mlp_get_params():
+    lib = load()
+    out = np.empty(mlp_param_count(), dtype=np.float32)
+    check(lib.ttf_mlp_get_params(out.ctypes.data), lib)
+    return out
+
+
+def mlp_forward(X):
+    """Network output (64 x M) for host features X (256 x M)."""
+    lib = load()
+    X = np.asfortranarray(X, dtype=np.float32)
+    M = X.shape[1]
+    dX = DeviceArray.from_numpy(X)
+    dY = DeviceArray((64, M), np.float32)
+    check(lib.ttf_mlp_forward(M, dX.ptr, dY.ptr), lib)
+    return dY.numpy()
+
+
+def mlp_loss_grad(X, T):
+    lib = load()
+    X = np.asfortranarray(X, dtype=np.float32)
+    T = np.asfortranarray(T, dtype=np.float32)
+    dX, dT = DeviceArray.from_numpy(X), DeviceArray.from_numpy(T)
+    loss = C.c_double()
+    g = np.empty(mlp_param_count(), dtype=np.float32)
+    check(lib.ttf_mlp_loss_grad(X.shape[1], dX.ptr, dT.ptr, C.byref(loss), g.ctypes.data), lib)
+    return loss.value, g
+
+
+def mlp_train_device(X_ptr, T_ptr, nsamples, epochs, batch=1024, lr=1e-3, seed=0):
+    """Train on device data (256 x n, 64 x n floats); returns the per-epoch
+    mean batch loss."""
+    lib = load()
+    hist = np.zeros(max(epochs, 1))
+    nh = C.c_int()
+    check(lib.ttf_mlp_train(nsamples, X_ptr, T_ptr, epochs, batch, lr, seed, hist.ctypes.data, hist.size,
+                            C.byref(nh)), lib)
+    return hist[:nh.value].copy()
+
+
+def mlp_train(X, T, epochs, batch=1024, lr=1e-3, seed=0):
+    X = np.asfortranarray(X, dtype=np.float32)
+    T = np.asfortranarray(T, dtype=np.float32)
+    dX, dT = DeviceArray.from_numpy(X), DeviceArray.from_numpy(T)
+    return mlp_train_device(dX.ptr, dT.ptr, X.shape[1], epochs, batch, lr, seed)
+
+
+def mlp_features(rx1, n, rx2, ra1, ra2, phi1, A, phi2, v):
+    lib = load()
+    d = [_dev(a, np.float64) for a in (phi1, A, phi2, v)]
+    X = DeviceArray((256, rx1 * rx2), np.float32)
+    sc = DeviceArray((rx1 * rx2,), np.float32)
+    check(lib.ttf_mlp_features(rx1, n, rx2, ra1, ra2, d[0].ptr, d[1].ptr, d[2].ptr, d[3].ptr, X.ptr, sc.ptr), lib)
+    return X.numpy(), sc.numpy()
+
+
+def mlp_precond(rx1, n, rx2, ra1, ra2, phi1, A, phi2, v):
+    lib = load()
+    d = [_dev(a, np.float64) for a in (phi1, A, phi2, v)]
+    z = DeviceArray((rx1 * n * rx2,), np.float64)
+    check(lib.ttf_mlp_precond(rx1, n, rx2, ra1, ra2, d[0].ptr, d[1].ptr, d[2].ptr, d[3].ptr, z.ptr), lib)
+    return z.numpy()
+
+
+def mlp_collect_begin(capacity, per_apply):
+    lib = load()
+    check(lib.ttf_mlp_collect_begin(capacity, per_apply), lib)
+
+
+def mlp_collect_end():
+    lib = load()
+    c = C.c_long()
+    check(lib.ttf_mlp_collect_end(C.byref(c)), lib)
+    return c.value
+
+
+def mlp_collected(host=False):
+    """(X_ptr, T_ptr, count) of the collected pairs, or host copies."""
+    lib = load()
+    xp, tp, c = _vp(), _vp(), C.c_long()
+    check(lib.ttf_mlp_collected(C.byref(xp), C.byref(tp), C.byref(c)), lib)
+    if not host:
+        return xp.value, tp.value, c.value
+    X = np.empty((256, c.value), dtype=np.float32, order="F")
+    T = np.empty((64, c.value), dtype=np.float32, order="F")
+    if c.value:
+        check(lib.ttf_memcpy_d2h(X.ctypes.data, xp.value, X.nbytes), lib)
+        check(lib.ttf_memcpy_d2h(T.ctypes.data, tp.value, T.nbytes), lib)
+    return X, T, c.value
