@@ -306,6 +306,11 @@ def run_amen(args):
     import ttops
     lib = ttfort.load()
     ttfort.check(lib.ttf_init(0), lib)
+    if args.prec == "m":
+        import learned
+        if learned.load(args.model) is None:
+            raise SystemExit("bench.py: --prec m needs trained parameters at %s "
+                             "(python tt-fort_amd/learned.py)" % args.model)
     d, n, tol, rmax = 6, 64, 1e-8, 64
     crA, ra, nn, info = mpo.diffusion_7pt(d, n, seed=0, coef_rank=3)
     crY, ry = mpo.ones_tt(d, n)
@@ -336,6 +341,7 @@ def run_amen(args):
                             ms_per_sweep=round(1e3 * secs / max(sweeps, 1), 1), max_rank=st["max_rank"],
                             matvecs=st["total_matvecs"], coef_maxerr=info["coef_maxerr"],
                             global_residual=resid, last_local_residual=st["last_res_max"],
+                            learned_solves=st["learned_solves"], learned_fallbacks=st["learned_fallbacks"],
                             solve_s=round(st["solve_seconds"], 3), trunc_s=round(st["trunc_seconds"], 3),
                             phi_s=round(st["phi_seconds"], 3), wall_s=round(wall, 2)))
 
@@ -346,7 +352,9 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="eigb", choices=["eigb", "matvec", "ksl", "amen"])
-    ap.add_argument("--prec", default="c", help="AMEn local preconditioner: n, c, l, r")
+    ap.add_argument("--prec", default="c", help="AMEn local preconditioner: n, c, l, r, m (learned)")
+    ap.add_argument("--model", default=os.path.join(ROOT, "tt-fort_amd", "models", "mlp_precond_n64.npy"),
+                    help="parameters of the learned preconditioner (--prec m)")
     ap.add_argument("--eigb-d", type=int, default=64)
     ap.add_argument("--eigb-rmax", type=int, default=128)
     ap.add_argument("--verb", type=int, default=1, help="per-step progress lines on stderr")
