@@ -102,7 +102,8 @@ def _graded(rng, m, n, lo):
 @pytest.mark.parametrize("m,n,kind", [(640, 256, "rand"), (256, 640, "rand"), (640, 256, "graded"),
                                       (256, 640, "graded"), (200, 200, "graded"), (1000, 17, "graded"),
                                       (17, 1000, "rand"), (33, 1, "rand"), (1, 7, "rand"),
-                                      (128, 320, "lowrank")])
+                                      (128, 320, "lowrank"), (4096, 64, "graded"), (64, 4096, "rand"),
+                                      (2048, 100, "lowrank"), (4096, 128, "rand")])
 @pytest.mark.parametrize("side", ["L", "R"])
 def test_svd_side(ttlib, m, n, kind, side):
     """One-sided SVD used by the truncations: singular values to LAPACK's

@@ -642,6 +642,107 @@ void complete_columns(int m, int k0, int k, double* U, int64_t ldu, hipStream_t 
   TTF_HIP(hipGetLastError());
 }
 
+// ---- Cholesky QR of nearly orthonormal columns
+// G (q x q, ld q) = R^T R by right-looking Cholesky in LDS and the inverse of
+// the upper factor: Rinv (q x q, ld q, upper).  LDS holds R in the upper
+// triangle of a (q+1) x (q+1) array and column j of R^-1 in its row j+1 (the
+// strictly lower part), so both fit at once.  *info = k+1 when the k-th
+// pivot is not positive.  One work-group, q <= kCholMax.
+constexpr int kCholMax = 128;
+
+__global__ __launch_bounds__(256) void chol_inv_kernel(int q, const double* __restrict__ G,
+                                                       double* __restrict__ Rinv, int* __restrict__ info) {
+  extern __shared__ double L[];
+  __shared__ int fail;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int ld = q + 1;
+  for (int e = tid; e < q * q; e += nt) {
+    const int i = e % q, j = e / q;
+    if (i <= j) L[i * ld + j] = 0.5 * (G[i + (int64_t)q * j] + G[j + (int64_t)q * i]);
+  }
+  if (tid == 0) fail = 0;
+  for (int k = 0; k < q; ++k) {
+    __syncthreads();
+    if (tid == 0) {
+      const double d = L[k * ld + k];
+      if (d > 0.0) L[k * ld + k] = sqrt(d);
+      else fail = k + 1;
+    }
+    __syncthreads();
+    if (fail) break;
+    const double rkk = L[k * ld + k];
+    for (int j = k + 1 + tid; j < q; j += nt) L[k * ld + j] /= rkk;
+    __syncthreads();
+    const int w = q - k - 1;
+    for (int e = tid; e < w * w; e += nt) {
+      const int i = k + 1 + e % w, j = k + 1 + e / w;
+      if (i <= j) L[i * ld + j] -= L[k * ld + i] * L[k * ld + j];
+    }
+  }
+  __syncthreads();
+  if (fail) {
+    if (tid == 0) *info = fail;
+    return;
+  }
+  // column j of X = R^-1 by back substitution, kept in row j+1 of L
+  for (int j = tid; j < q; j += nt) {
+    double* x = L + (j + 1) * ld;
+    x[j] = 1.0 / L[j * ld + j];
+    for (int i = j - 1; i >= 0; --i) {
+      double acc = 0.0;
+      for (int l = i + 1; l <= j; ++l) acc += L[i * ld + l] * x[l];
+      x[i] = -acc / L[i * ld + i];
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < q * q; e += nt) {
+    const int i = e % q, j = e / q;
+    Rinv[e] = (i <= j) ? L[(j + 1) * ld + i] : 0.0;
+  }
+  if (tid == 0) *info = 0;
+}
+
+__global__ void scale_cols_kernel(int m, int k, double* __restrict__ U, int64_t ldu, const double* __restrict__ sc) {
+  const int64_t total = (int64_t)m * k;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % m), c = (int)(e / m);
+    U[i + ldu * c] *= sc[c];
+  }
+}
+
+// Two passes of Cholesky QR on the p x k columns of U (ld ldu), which must
+// be nearly orthonormal already; returns false (U partly updated) if a
+// Cholesky factorisation breaks down.
+bool cholqr2(int p, int k, double* U, int64_t ldu, hipStream_t st) {
+  if (k <= 0) return true;
+  TTF_REQUIRE(k <= kCholMax, "cholqr2: %d columns exceed %d", k, kCholMax);
+  static bool attr_set = false;
+  if (!attr_set) {
+    TTF_HIP(hipFuncSetAttribute((const void*)chol_inv_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(sizeof(double) * (kCholMax + 1) * (kCholMax + 1))));
+    attr_set = true;
+  }
+  double* G = (double*)arena(kArenaSvdLong, sizeof(double) * ((size_t)2 * k * k + (size_t)p * k) + 64);
+  double* Ri = G + (size_t)k * k;
+  double* T = Ri + (size_t)k * k;
+  int* info = (int*)(T + (size_t)p * k);
+  const size_t lds = sizeof(double) * (size_t)(k + 1) * (k + 1);
+  for (int pass = 0; pass < 2; ++pass) {
+    if (k <= 64) tsgram(p, k, k, U, ldu, U, ldu, G, k, st);
+    else gemm_launch(colmajor<double>('T', 'N', k, k, p, 1.0, U, ldu, U, ldu, 0.0, G, k), st);
+    hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(256), lds, st, k, G, Ri, info);
+    TTF_HIP(hipGetLastError());
+    int hinfo = 0;
+    TTF_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, st));
+    TTF_HIP(hipStreamSynchronize(st));
+    if (hinfo != 0) return false;
+    gemm_launch(colmajor<double>('N', 'N', p, k, k, 1.0, U, ldu, Ri, k, 0.0, T, p), st);
+    TTF_HIP(hipMemcpy2DAsync(U, sizeof(double) * ldu, T, sizeof(double) * p, sizeof(double) * p, k,
+                             hipMemcpyDeviceToDevice, st));
+  }
+  return true;
+}
+
 // Hestenes sweeps per pair and round (TTF_SVD_INNER overrides; default 1).
 static int svd_inner_sweeps() {
   static int v = [] {
@@ -859,6 +960,26 @@ int svd_vectors(int m, int n, const double* A, int64_t lda, bool want_left, doub
   TTF_REQUIRE(q <= 4096, "svd_vectors: min(m,n)=%d exceeds 4096", q);
   const bool short_side = want_left ? !tall : tall;  // vectors of dimension q
   const bool precond = short_side && (p | 1) <= kSvdLdsDoubles;
+  if (!short_side && q <= kCholMax && (p | 1) <= kSvdLdsDoubles && q < p) {
+    // vectors of the long side from the short side's: with T = A (tall) or
+    // A^T (wide) and T^T T = Vs S^2 Vs^T, the long-side vectors are
+    // T Vs S^-1 for the nonzero values, orthonormalised again by Cholesky QR
+    // (in column order, so the leading columns span the dominant subspace)
+    double* Vs = (double*)arena(kArenaSvdLong2, sizeof(double) * ((size_t)q * q + (size_t)q));
+    double* sc = Vs + (size_t)q * q;
+    const int sw = svd_vectors(m, n, A, lda, !want_left, S, Vs, q, hs, kpos, st);
+    if (tall) gemm_launch(colmajor<double>('N', 'N', m, q, n, 1.0, A, lda, Vs, q, 0.0, O, ldo), st);
+    else gemm_launch(colmajor<double>('T', 'N', n, q, m, 1.0, A, lda, Vs, q, 0.0, O, ldo), st);
+    std::vector<double> hsc(q, 0.0);
+    for (int c = 0; c < *kpos; ++c) hsc[c] = 1.0 / hs[c];
+    TTF_HIP(hipMemcpyAsync(sc, hsc.data(), sizeof(double) * q, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(scale_cols_kernel, dim3((unsigned)std::min<int64_t>(ceil_div((int64_t)p * q, 256), 8192)),
+                       dim3(256), 0, st, p, q, O, ldo, sc);
+    TTF_HIP(hipGetLastError());
+    if (cholqr2(p, *kpos, O, ldo, st)) return sw;
+    // breakdown (not expected for these nearly orthonormal columns): the
+    // one-sided Jacobi on the long columns below
+  }
   if (short_side && !precond) {
     // very long rows: the plain SVD, then the wanted factor
     double* Uf = (double*)arena(kArenaDriver, sizeof(double) * ((size_t)m * q + (size_t)q * n));

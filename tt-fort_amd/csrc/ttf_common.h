@@ -80,6 +80,8 @@ enum Arena {
   kArenaMlp2,        // learned preconditioner: gradients, loss partials, indices
   kArenaMlp3,        // learned preconditioner: batch targets
   kArenaZ,           // flexible GMRES: preconditioned directions
+  kArenaSvdLong,     // long-side singular vectors: Cholesky QR workspace
+  kArenaSvdLong2,    // long-side singular vectors: short-side vectors
   kArenaCount
 };
 void* arena(Arena a, size_t bytes);
