@@ -26,26 +26,37 @@ __device__ inline void jacobi_pair(int r, int k, int nn, int& p, int& q) {
   q = max(a, b);
 }
 
-// Rotation (c, s) that annihilates A[p][q] (J = [[c, s], [-s, c]] on rows
-// and columns p, q), or the identity when the pair is already decoupled:
-// |a_pq| at or below the caller's rounding floor tol_abs or below
-// eps * sqrt(|a_pp a_qq|).  t = tan(theta) is the smaller root of
-// t^2 + 2 cot(2 theta) t - 1 = 0, written as
-// t = sign(d) 2 a_pq / (|d| + sqrt(d^2 + 4 a_pq^2)) with d = a_qq - a_pp.
-template <int LD>
-__device__ inline bool jacobi_rotation(const double (*A)[LD], int p, int q, int n, double tol_abs,
-                                       double& c, double& s) {
-  c = 1.0;
-  s = 0.0;
-  if (q >= n) return false;
-  const double eps = 2.220446049250313e-16;
-  const double apq = A[p][q], app = A[p][p], aqq = A[q][q];
-  if (!(fabs(apq) > tol_abs && apq * apq > eps * eps * fabs(app * aqq))) return false;
+// Rotation parameters (J = [[c, s], [-s, c]] on rows and columns p, q) that
+// annihilate a_pq of the 2x2 block [[app, apq], [apq, aqq]] when `go`, else
+// the identity.  t = tan(theta) is the smaller root of
+// t^2 + 2 cot(2 theta) t - 1 = 0, t = sign(d) 2 a_pq / (|d| + sqrt(d^2 + 4 a_pq^2))
+// with d = a_qq - a_pp, evaluated from the hardware reciprocal (square
+// root) estimates refined by Newton steps: t to about 2^-46 (it only sets
+// how well a_pq is annihilated), c = (1 + t^2)^(-1/2) by two steps so that
+// c^2 + s^2 = 1 to rounding (the rotation stays orthogonal).  Branch free,
+// so the rotations of two pairs interleave.
+__device__ __forceinline__ void jacobi_cs(double apq, double app, double aqq, bool go, double& c, double& s) {
   const double d = aqq - app;
-  const double t = copysign(1.0, d) * (2.0 * apq) / (fabs(d) + sqrt(d * d + 4.0 * apq * apq));
-  c = rsqrt(1.0 + t * t);
-  s = t * c;
-  return true;
+  const double q = go ? d * d + 4.0 * apq * apq : 1.0;
+  double rq = __builtin_amdgcn_rsq(q);
+  rq = rq * (1.5 - 0.5 * q * rq * rq);
+  const double den = go ? fabs(d) + q * rq : 1.0;
+  double ri = __builtin_amdgcn_rcp(den);
+  ri = ri * (2.0 - den * ri);
+  const double t = go ? (d >= 0.0 ? 2.0 : -2.0) * apq * ri : 0.0;
+  const double x = 1.0 + t * t;
+  double cc = __builtin_amdgcn_rsq(x);
+  cc = cc * (1.5 - 0.5 * x * cc * cc);
+  cc = cc * (1.5 - 0.5 * x * cc * cc);
+  c = cc;
+  s = t * cc;
+}
+
+// Whether pair (p, q) of the current matrix is rotated: |a_pq| above the
+// caller's rounding floor tol_abs and above eps * sqrt(|a_pp a_qq|).
+__device__ __forceinline__ bool jacobi_go(double apq, double app, double aqq, double tol_abs) {
+  const double eps = 2.220446049250313e-16;
+  return fabs(apq) > tol_abs && apq * apq > eps * eps * fabs(app * aqq);
 }
 
 // Diagonalises the symmetric n x n matrix held in A (row stride LD, n <= 64)
@@ -54,10 +65,11 @@ __device__ inline bool jacobi_rotation(const double (*A)[LD], int p, int q, int 
 // other, so one barrier per round suffices.  Every thread owns 2x2 blocks
 // (rows of pair kp x columns of pair kq) and derives both rotations from
 // the round's input itself, so no phase publishes rotations; the thread also
-// rotates rows kp and kp + nn/2 of V for pair kq.  Annihilated entries are
-// set to zero exactly.  The loop ends after a sweep without rotation or
-// after max_sweeps.  `flags` is 3 ints of LDS.  All threads of the block
-// call it; the returned pointer (A or B) holds the diagonalised matrix.
+// rotates rows kp and kp + nn/2 of V for pair kq.  The 2x2 blocks are
+// updated in full (annihilated entries keep their rounding-level values).
+// The loop ends after a sweep without rotation or after max_sweeps.
+// `flags` is 3 ints of LDS.  All threads of the block call it; the returned
+// pointer (A or B) holds the diagonalised matrix.
 template <int LD>
 __device__ double (*jacobi_sym(double (*A)[LD], double (*B)[LD], double (*V)[LD], int n, int max_sweeps,
                                double tol_abs, int* flags))[LD] {
@@ -67,6 +79,7 @@ __device__ double (*jacobi_sym(double (*A)[LD], double (*B)[LD], double (*V)[LD]
   if (n <= 1) return A;
   // thread -> (kp, kq0); threads with kq0 >= kqstep would repeat columns
   const int kp = tid % half, kq0 = tid / half, kqstep = max(nt / half, 1);
+  const bool active = kq0 < kqstep && kq0 < half;
   double(*cur)[LD] = A;
   double(*nxt)[LD] = B;
   if (tid < 3) flags[tid] = 0;
@@ -76,37 +89,31 @@ __device__ double (*jacobi_sym(double (*A)[LD], double (*B)[LD], double (*V)[LD]
     // flags[(sweep+1) % 3] was last read before the previous sweep's barriers
     if (tid == 0) flags[(sweep + 1) % 3] = 0;
     for (int r = 0; r < nn - 1; ++r) {
-      if (kq0 < kqstep && kq0 < half) {
+      if (active) {
         int p, q;
         jacobi_pair(r, kp, nn, p, q);
-        double c1, s1;
-        const bool rot1 = jacobi_rotation<LD>(cur, p, q, n, tol_abs, c1, s1);
-        if (rot1) *rotated = 1;
         const bool hq = q < n;
+        const double apq = hq ? cur[p][q] : 0.0, app = cur[p][p], aqq = hq ? cur[q][q] : 0.0;
+        const bool go1 = hq && jacobi_go(apq, app, aqq, tol_abs);
+        if (go1) *rotated = 1;
         for (int kq = kq0; kq < half; kq += kqstep) {
           int u, v;
           jacobi_pair(r, kq, nn, u, v);
-          double c2, s2;
-          jacobi_rotation<LD>(cur, u, v, n, tol_abs, c2, s2);
           const bool hv = v < n;
+          const double auv = hv ? cur[u][v] : 0.0, auu = cur[u][u], avv = hv ? cur[v][v] : 0.0;
           double apu = cur[p][u], apv = hv ? cur[p][v] : 0.0;
           double aqu = hq ? cur[q][u] : 0.0, aqv = (hq && hv) ? cur[q][v] : 0.0;
+          double c1, s1, c2, s2;
+          jacobi_cs(apq, app, aqq, go1, c1, s1);
+          jacobi_cs(auv, auu, avv, hv && jacobi_go(auv, auu, avv, tol_abs), c2, s2);
           // columns u, v
           const double t0 = c2 * apu - s2 * apv, t1 = s2 * apu + c2 * apv;
           const double t2 = c2 * aqu - s2 * aqv, t3 = s2 * aqu + c2 * aqv;
           // rows p, q
-          apu = c1 * t0 - s1 * t2;
-          aqu = s1 * t0 + c1 * t2;
-          apv = c1 * t1 - s1 * t3;
-          aqv = s1 * t1 + c1 * t3;
-          if (kq == kp && rot1) {
-            apv = 0.0;
-            aqu = 0.0;
-          }
-          nxt[p][u] = apu;
-          if (hv) nxt[p][v] = apv;
-          if (hq) nxt[q][u] = aqu;
-          if (hq && hv) nxt[q][v] = aqv;
+          nxt[p][u] = c1 * t0 - s1 * t2;
+          if (hv) nxt[p][v] = c1 * t1 - s1 * t3;
+          if (hq) nxt[q][u] = s1 * t0 + c1 * t2;
+          if (hq && hv) nxt[q][v] = s1 * t1 + c1 * t3;
           // rows kp and kp + half of V, columns of pair kq
           if (s2 != 0.0) {
             for (int i = kp; i < n; i += half) {
