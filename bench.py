@@ -311,6 +311,7 @@ def run_amen(args):
         if learned.load(args.model) is None:
             raise SystemExit("bench.py: --prec m needs trained parameters at %s "
                              "(python tt-fort_amd/learned.py)" % args.model)
+    ttfort.set_local_precision(32 if args.mixed else 64)
     d, n, tol, rmax = 6, 64, 1e-8, 64
     crA, ra, nn, info = mpo.diffusion_7pt(d, n, seed=0, coef_rank=3)
     crY, ry = mpo.ones_tt(d, n)
@@ -331,12 +332,14 @@ def run_amen(args):
         sweeps += st["sweeps"]
     wall = time.perf_counter() - t0
     resid = ttops.residual(crA, ra, nn, res, rx_out, crY, ry)
-    return dict(metric="sweep time + local-matvec TFLOP/s (FP64 AMEn solve, 6D variable-coefficient diffusion)",
+    prec_name = "mixed fp32-matvec/FP64-refinement" if args.mixed else "FP64"
+    return dict(metric="sweep time + local-matvec TFLOP/s (%s AMEn solve, 6D variable-coefficient diffusion)"
+                       % prec_name,
                 value=round(flops / secs / 1e12, 4), unit="TFLOP/s", n_gpus=1, steps=args.steps,
                 warmup=args.warmup, ms_per_step=round(1e3 * wall / args.steps, 1), higher_is_better=True,
-                scaling="weak", vs_baseline=None, dtype="f64", data="synthetic",
+                scaling="weak", vs_baseline=None, dtype="f32+f64" if args.mixed else "f64", data="synthetic",
                 config=dict(workload="tt_amen_solve 6D diffusion n=64 coef rank 3 (operator rank 6) tol 1e-8 "
-                                     "rmax 64 prec %s" % args.prec,
+                                     "rmax 64 prec %s%s" % (args.prec, " mixed" if args.mixed else ""),
                             d=d, n=n, tol=tol, rmax=rmax, prec=args.prec, sweeps_per_solve=sweeps / args.steps,
                             ms_per_sweep=round(1e3 * secs / max(sweeps, 1), 1), max_rank=st["max_rank"],
                             matvecs=st["total_matvecs"], coef_maxerr=info["coef_maxerr"],
@@ -353,6 +356,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="eigb", choices=["eigb", "matvec", "ksl", "amen"])
     ap.add_argument("--prec", default="c", help="AMEn local preconditioner: n, c, l, r, m (learned)")
+    ap.add_argument("--mixed", action="store_true",
+                    help="AMEn: fp32 Krylov matvecs with FP64 refinement in the local GMRES")
     ap.add_argument("--model", default=os.path.join(ROOT, "tt-fort_amd", "models", "mlp_precond_n64.npy"),
                     help="parameters of the learned preconditioner (--prec m)")
     ap.add_argument("--eigb-d", type=int, default=64)

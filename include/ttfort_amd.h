@@ -58,6 +58,14 @@ void ttf_clear_error(void);
 /* Library version string. */
 const char* ttf_version(void);
 
+/* Precision of the matvecs inside the iterative local solvers (the local
+ * GMRES of ttf_tt_amen_solve): 64 (default, FP64 throughout) or 32 (Krylov
+ * directions multiplied on the fp32 matrix cores with fp32 accumulation,
+ * every restart residual rhs - B x and the solver's own residual checks in
+ * FP64, i.e. FP64 iterative refinement).  Not in the reference. */
+int ttf_set_local_precision(int bits);
+int ttf_get_local_precision(void);
+
 /* Device memory helpers (used by tests and benchmarks; any hipMalloc'ed
  * pointer is accepted by the native entry points). */
 int ttf_malloc(void** ptr, size_t bytes);

@@ -147,3 +147,27 @@ def test_amen_diffusion(ttlib, d, n, prec):
     assert np.linalg.norm(x - x_ref) <= 1e-6 * np.linalg.norm(x_ref), (np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref), st)
     assert ttops.residual(crA, ra, nn, res, rx_out, crY, ry) <= 1e-6
     assert st["flops"] > 0 and st["last_res_max"] < eps
+
+
+@pytest.mark.parametrize("prec", ["c", "n"])
+def test_amen_mixed_precision(ttlib, prec):
+    """Local GMRES with fp32 Krylov matvecs and FP64 restart residuals reaches
+    the FP64 solver's tolerance and solution."""
+    import ttfort
+    import ttops
+    d, n = 4, 12
+    crA, ra, nn, _ = mpo.diffusion_7pt(d, n, seed=8, coef_rank=3)
+    crY, ry = mpo.ones_tt(d, n)
+    crX, rx = mpo.random_tt(d, n, 2, seed=3)
+    eps = 1e-8
+    A = O.mpo_to_dense(_cores(crA, ra, nn, nn), ra, nn, nn)
+    x_ref = np.linalg.solve(A, np.ones(n ** d))
+    try:
+        ttfort.set_local_precision(32)
+        res, rx_out, st = ttfort.tt_amen_solve(nn, ra, crA, ry, crY, rx, crX, eps, kickrank=4, prec=prec, nswp=20)
+    finally:
+        ttfort.set_local_precision(64)
+    x = O.tt_full(_cores(res, rx_out, nn), nn, rx_out)
+    assert st["last_res_max"] < eps
+    assert np.linalg.norm(x - x_ref) <= 1e-6 * np.linalg.norm(x_ref)
+    assert ttops.residual(crA, ra, nn, res, rx_out, crY, ry) <= 1e-6

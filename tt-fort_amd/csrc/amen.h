@@ -41,8 +41,10 @@ struct GmresStats {
 // as tt_adapt_als.f90:776-781 does) by restarted GMRES(nrestart), at most
 // niters cycles, stopping when the residual estimate falls below tol times
 // the initial residual norm (tt_linalg.f90:797-803).  sol starts at zero.
+// With `mix`, the Krylov directions are multiplied by the fp32 operator and
+// each restart residual by the FP64 one (iterative refinement).
 GmresStats gmres_solve(const LocalOp& op, const JacPrec& P, const double* rhs, double* sol, int nrestart,
-                       double tol, int niters, hipStream_t st);
+                       double tol, int niters, hipStream_t st, const MixedOp* mix = nullptr);
 
 // Flexible GMRES (right preconditioning by a possibly nonlinear map M, with
 // the preconditioned directions kept: x = sum_j y_j M(v_j)).  Solves
@@ -50,7 +52,7 @@ GmresStats gmres_solve(const LocalOp& op, const JacPrec& P, const double* rhs, d
 // gmres_solve; x is the solution itself.
 typedef std::function<void(const double* v, double* z)> PrecFn;
 GmresStats gmres_flex(const LocalOp& op, const PrecFn& M, const double* rhs, double* x, int nrestart, double tol,
-                      int niters, hipStream_t st);
+                      int niters, hipStream_t st, const MixedOp* mix = nullptr);
 
 // Pivoted incomplete Cholesky of the Gram matrix of the n "items" of dA
 // (columns of the m x n dA for trans 'N', rows of the n x m dA for 'T'),

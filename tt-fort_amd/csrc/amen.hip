@@ -243,6 +243,12 @@ extern "C" int ttf_tt_amen_solve(int d, const int* n, const int* m, const int* r
       GmresStats gs;
       if (gtol < 1.0) {
         const int nrs = (int)std::min<int64_t>(nrestart, N);
+        MixedOp mixop;
+        const MixedOp* mix = nullptr;
+        if (local_matvec_bits() == 32) {
+          mixop = make_mixed(op, st);
+          mix = &mixop;
+        }
         if (ptype == 'm') {
           // learned block preconditioner in flexible GMRES; plain GMRES takes
           // over from its iterate when it misses the tolerance
@@ -253,7 +259,7 @@ extern "C" int ttf_tt_amen_solve(int d, const int* n, const int* m, const int* r
           PrecFn M = [&, p1, Ai, p2, ra1, ra2, r1, nn_i, r2](const double* v, double* z) {
             mlp_precond(r1, nn_i, r2, ra1, ra2, p1, Ai, p2, v, z, st);
           };
-          gs = gmres_flex(op, M, res, dx, nrs, gtol, niters, st);
+          gs = gmres_flex(op, M, res, dx, nrs, gtol, niters, st, mix);
           mv_total += gs.matvecs;
           flops_total += bfun3_flops(op.s) * gs.matvecs;
           ++mlp_solves;
@@ -267,7 +273,8 @@ extern "C" int ttf_tt_amen_solve(int d, const int* n, const int* m, const int* r
             const double n0 = dev_norm(N, res, scr, st), n2 = dev_norm(N, candy, scr, st);
             int extra = 1;
             if (n2 > 0.0) {
-              GmresStats g2 = gmres_solve(op, JacPrec{}, candy, cand, nrs, std::min(1.0, gtol * n0 / n2), niters, st);
+              GmresStats g2 = gmres_solve(op, JacPrec{}, candy, cand, nrs, std::min(1.0, gtol * n0 / n2), niters, st,
+                                          mix);
               hipLaunchKernelGGL(axpy_kernel, dim3(grid_for(N)), dim3(256), 0, st, N, 1.0, cand, dx);
               extra += g2.matvecs;
               gs.relres = g2.relres * n2 / n0;
@@ -283,7 +290,7 @@ extern "C" int ttf_tt_amen_solve(int d, const int* n, const int* m, const int* r
             jac_gen(ptype, r1, nn_i, r2, ra[i], ra[i + 1], phiA[i].p, dA + pa[i], phiA[i + 1].p, jacs, st);
             P = JacPrec{ptype, r1, nn_i, r2, jacs, ra[i], ra[i + 1], phiA[i].p, dA + pa[i], phiA[i + 1].p};
           }
-          gs = gmres_solve(op, P, res, sol, nrs, gtol, niters, st);
+          gs = gmres_solve(op, P, res, sol, nrs, gtol, niters, st, mix);
           mv_total += gs.matvecs;
           flops_total += bfun3_flops(op.s) * gs.matvecs;
           if (ptype != 'n') jac_apply(ptype, r1, nn_i, r2, P.jacs, sol, dx, st);

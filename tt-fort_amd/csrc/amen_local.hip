@@ -265,8 +265,51 @@ void jac_apply(char type, int rx1, int n, int rx2, const double* jacs, const dou
 }
 
 // ---------------------------------------------------------------- GMRES
+namespace {
+__global__ void d2f_kernel(int64_t n, const double* __restrict__ x, float* __restrict__ y) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    y[e] = (float)x[e];
+}
+__global__ void f2d_kernel(int64_t n, const float* __restrict__ x, double* __restrict__ y) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    y[e] = (double)x[e];
+}
+void to_float(int64_t n, const double* x, float* y, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(d2f_kernel, dim3(grid_for(n)), dim3(256), 0, st, n, x, y);
+  TTF_HIP(hipGetLastError());
+}
+}  // namespace
+
+MixedOp make_mixed(const LocalOp& op, hipStream_t st) {
+  const Shape& s = op.s;
+  const int64_t n1 = (int64_t)s.ry1 * s.rx1 * s.ra1, na = (int64_t)s.ra1 * s.n * s.m * s.ra2;
+  const int64_t n2 = (int64_t)s.rx2 * s.ra2 * s.ry2;
+  const int64_t nx = (int64_t)s.rx1 * s.m * s.rx2, ny = (int64_t)s.ry1 * s.n * s.ry2;
+  float* base = (float*)arena(kArenaMixed, sizeof(float) * (size_t)(n1 + na + n2 + nx + ny + 64));
+  MixedOp mo;
+  mo.s = s;
+  mo.phi1 = base;
+  mo.A = mo.phi1 + n1;
+  mo.phi2 = mo.A + na;
+  mo.x = mo.phi2 + n2;
+  mo.y = mo.x + nx;
+  to_float(n1, op.phi1, mo.phi1, st);
+  to_float(na, op.A, mo.A, st);
+  to_float(n2, op.phi2, mo.phi2, st);
+  return mo;
+}
+
+void MixedOp::apply(const double* xin, double* yout, hipStream_t st) const {
+  const int64_t nx = (int64_t)s.rx1 * s.m * s.rx2, ny = (int64_t)s.ry1 * s.n * s.ry2;
+  to_float(nx, xin, x, st);
+  bfun3_chain<float>(s, 1, phi1, A, phi2, x, nx, y, ny, st);
+  hipLaunchKernelGGL(f2d_kernel, dim3(grid_for(ny)), dim3(256), 0, st, ny, y, yout);
+  TTF_HIP(hipGetLastError());
+}
+
 GmresStats gmres_solve(const LocalOp& op, const JacPrec& P, const double* rhs, double* sol, int nrestart,
-                       double tol, int niters, hipStream_t st) {
+                       double tol, int niters, hipStream_t st, const MixedOp* mix) {
   GmresStats gs;
   const int64_t N = op.N();
   const int m = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)nrestart, N, (int64_t)63}));
@@ -279,7 +322,9 @@ GmresStats gmres_solve(const LocalOp& op, const JacPrec& P, const double* rhs, d
   const int gN = grid_for(N);
   std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), y(m), hh(2 * 64 + 8);
   double nrmrhs = 0.0, curres = 1.0;
-  auto apply_BP = [&](const double* v, double* out) {
+  // B P v; Krylov directions use the fp32 operator when given, restart
+  // residuals (exact) always the FP64 one
+  auto apply_BP = [&](const double* v, double* out, bool exact) {
     const double* in = v;
     if (P.type != 'n') {
       jac_apply(P.type, P.rx1, P.n, P.rx2, P.jacs, v, t, st);
@@ -287,7 +332,8 @@ GmresStats gmres_solve(const LocalOp& op, const JacPrec& P, const double* rhs, d
         mlp_collect_pair(P.rx1, P.n, P.rx2, P.ra1, P.ra2, P.phi1, P.A, P.phi2, v, t, st);
       in = t;
     }
-    op.apply(1, in, N, out, N, st);
+    if (mix && !exact) mix->apply(in, out, st);
+    else op.apply(1, in, N, out, N, st);
     ++gs.matvecs;
   };
   for (int it = 0; it < niters; ++it) {
@@ -296,7 +342,7 @@ GmresStats gmres_solve(const LocalOp& op, const JacPrec& P, const double* rhs, d
     if (it == 0) {
       TTF_HIP(hipMemcpyAsync(V, rhs, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
     } else {
-      apply_BP(sol, V);
+      apply_BP(sol, V, true);
       hipLaunchKernelGGL(axpby_kernel, dim3(gN), dim3(256), 0, st, N, 1.0, rhs, -1.0, V);
       TTF_HIP(hipGetLastError());
     }
@@ -311,7 +357,7 @@ GmresStats gmres_solve(const LocalOp& op, const JacPrec& P, const double* rhs, d
     g[0] = beta;
     int jj = 0;
     for (int j = 0; j < m; ++j) {
-      apply_BP(V + (size_t)N * j, u);
+      apply_BP(V + (size_t)N * j, u, false);
       const int k = j + 1;
       // classical Gram-Schmidt, twice, against V(:, 0..j)
       for (int pass = 0; pass < 2; ++pass) {
@@ -364,7 +410,7 @@ GmresStats gmres_solve(const LocalOp& op, const JacPrec& P, const double* rhs, d
 }
 
 GmresStats gmres_flex(const LocalOp& op, const PrecFn& M, const double* rhs, double* x, int nrestart, double tol,
-                      int niters, hipStream_t st) {
+                      int niters, hipStream_t st, const MixedOp* mix) {
   GmresStats gs;
   const int64_t N = op.N();
   const int m = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)nrestart, N, (int64_t)63}));
@@ -400,7 +446,8 @@ GmresStats gmres_flex(const LocalOp& op, const PrecFn& M, const double* rhs, dou
     for (int j = 0; j < m; ++j) {
       double* zj = Z + (size_t)N * j;
       M(V + (size_t)N * j, zj);
-      op.apply(1, zj, N, u, N, st);
+      if (mix) mix->apply(zj, u, st);
+      else op.apply(1, zj, N, u, N, st);
       ++gs.matvecs;
       const int k = j + 1;
       for (int pass = 0; pass < 2; ++pass) {

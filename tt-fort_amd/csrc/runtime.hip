@@ -4,6 +4,7 @@
 
 #include <cstdarg>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 
 namespace ttf {
@@ -17,7 +18,10 @@ hipStream_t g_stream = nullptr;
 hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
 struct ArenaSlot { void* p = nullptr; size_t bytes = 0; };
 ArenaSlot g_arena[kArenaCount];
+std::atomic<int> g_local_bits{64};
 }  // namespace
+
+int local_matvec_bits() { return g_local_bits.load(); }
 
 void set_error(const char* fmt, ...) {
   va_list ap;
@@ -106,6 +110,15 @@ int ttf_finalize(void) {
     g_init = false;
   });
 }
+
+int ttf_set_local_precision(int bits) {
+  return guarded([&] {
+    TTF_REQUIRE(bits == 64 || bits == 32, "ttf_set_local_precision: %d bits (64 or 32)", bits);
+    g_local_bits.store(bits);
+  });
+}
+
+int ttf_get_local_precision(void) { return g_local_bits.load(); }
 
 const char* ttf_last_error(void) { return g_err; }
 void ttf_clear_error(void) { g_err[0] = 0; }

@@ -18,6 +18,23 @@ struct LocalOp {
   }
 };
 
+// FP32 copy of a local operator for the mixed-precision path: the vector
+// is rounded to float, the three-stage chain runs on the fp32 matrix cores
+// with fp32 accumulation, and the result is widened back to double.  The
+// iterative solvers that use it keep their bases, projections and restart
+// residuals in FP64 (the restart residual rhs - B x is the FP64 refinement
+// step).  Buffers live in a dedicated arena: one MixedOp at a time.
+struct MixedOp {
+  Shape s;
+  float* phi1 = nullptr;
+  float* A = nullptr;
+  float* phi2 = nullptr;
+  float* x = nullptr;
+  float* y = nullptr;
+  void apply(const double* xin, double* yout, hipStream_t st) const;
+};
+MixedOp make_mixed(const LocalOp& op, hipStream_t st);
+
 struct EigStats {
   int iters = 0;
   int matvecs = 0;

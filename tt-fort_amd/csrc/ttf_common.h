@@ -61,6 +61,9 @@ int guarded(Fn&& body) {
 // outputs: contraction intermediates, solver bases).  Arenas grow with
 // hipMalloc and are released by ttf_finalize().
 hipStream_t stream();
+// Precision of the matvecs inside the iterative local solvers: 64 (default)
+// or 32 (fp32 matrix cores, FP64 refinement); set by ttf_set_local_precision.
+int local_matvec_bits();
 void ensure_init();
 
 enum Arena {
@@ -82,6 +85,7 @@ enum Arena {
   kArenaZ,           // flexible GMRES: preconditioned directions
   kArenaSvdLong,     // long-side singular vectors: Cholesky QR workspace
   kArenaSvdLong2,    // long-side singular vectors: short-side vectors
+  kArenaMixed,       // fp32 operator copy and vector staging of the mixed path
   kArenaCount
 };
 void* arena(Arena a, size_t bytes);

@@ -56,6 +56,8 @@ def _sig(name, res, *args):
 
 _S8 = [_c_int] * 8
 _sig("ttf_init", _c_int, _c_int)
+_sig("ttf_set_local_precision", _c_int, _c_int)
+_sig("ttf_get_local_precision", _c_int)
 _sig("ttf_finalize", _c_int)
 _sig("ttf_last_error", C.c_char_p)
 _sig("ttf_clear_error", None)
@@ -670,3 +672,9 @@ def mlp_collected(host=False):
         check(lib.ttf_memcpy_d2h(X.ctypes.data, xp.value, X.nbytes), lib)
         check(lib.ttf_memcpy_d2h(T.ctypes.data, tp.value, T.nbytes), lib)
     return X, T, c.value
+
+
+def set_local_precision(bits):
+    """Precision of the matvecs inside the local iterative solvers (64 or 32)."""
+    lib = load()
+    check(lib.ttf_set_local_precision(bits), lib)
