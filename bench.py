@@ -349,12 +349,56 @@ def run_amen(args):
                             phi_s=round(st["phi_seconds"], 3), wall_s=round(wall, 2)))
 
 
+def run_batch(args):
+    """Many-instance eigb: independent open XXZ chains, d ~ U{16..48},
+    Delta ~ U(0.5, 1.5), J ~ U(0.8, 1.2), max rank r ~ U{32..96} from
+    PCG64(seed 5), 3 lowest eigenpairs each, tol 1e-8, random rank-4 start;
+    all chains solved concurrently by ttf_tt_eigb_batch.  One step = the
+    whole batch."""
+    import ttfort
+    import mpo
+    lib = ttfort.load()
+    ttfort.check(lib.ttf_init(0), lib)
+    rng = np.random.default_rng(5)
+    probs = []
+    for c in range(args.chains):
+        d = int(rng.integers(16, 49))
+        delta = float(rng.uniform(0.5, 1.5))
+        J = float(rng.uniform(0.8, 1.2))
+        r = int(rng.integers(32, 97))
+        crA, ra, n = mpo.xxz(d, J=J, delta=delta)
+        crY, ry = mpo.random_block_tt(d, n, 4, 3, seed=1000 + c)
+        probs.append(dict(n=n, ra=ra, crA=crA, crY0=crY, ry=ry, eps=1e-8, rmax=r))
+    for _ in range(args.warmup):
+        ttfort.tt_eigb_batch(probs[:min(8, len(probs))], 3, nswp=20, nworkers=args.workers)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = ttfort.tt_eigb_batch(probs, 3, nswp=20, nworkers=args.workers)
+    wall = (time.perf_counter() - t0) / args.steps
+    flops = sum(sum(o[3]["sweep_flops"]) for o in out)
+    chain_s = sum(o[3]["seconds"] for o in out)
+    sweeps = [o[3]["sweeps"] for o in out]
+    return dict(metric="batch time + local-matvec TFLOP/s (FP64 tt_eigb on %d independent XXZ chains)"
+                       % args.chains,
+                value=round(flops / wall / 1e12, 4), unit="TFLOP/s", n_gpus=1, steps=args.steps,
+                warmup=args.warmup, ms_per_step=round(1e3 * wall, 1), higher_is_better=True, scaling="weak",
+                vs_baseline=None, dtype="f64", data="synthetic",
+                config=dict(workload="tt_eigb batch: %d XXZ chains d~U{16..48} r~U{32..96} B=3 tol 1e-8"
+                                     % args.chains,
+                            chains=args.chains, workers=args.workers, B=3, tol=1e-8,
+                            sum_chain_seconds=round(chain_s, 2), concurrency=round(chain_s / wall, 2),
+                            mean_sweeps=float(np.mean(sweeps)), max_rank=max(o[3]["max_rank"] for o in out),
+                            matvecs=int(sum(o[3]["total_matvecs"] for o in out)), wall_s=round(wall, 2)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="eigb", choices=["eigb", "matvec", "ksl", "amen"])
+    ap.add_argument("--workload", default="eigb", choices=["eigb", "matvec", "ksl", "amen", "batch"])
+    ap.add_argument("--chains", type=int, default=256, help="batch: number of chains")
+    ap.add_argument("--workers", type=int, default=16, help="batch: concurrent worker streams")
     ap.add_argument("--prec", default="c", help="AMEn local preconditioner: n, c, l, r, m (learned)")
     ap.add_argument("--mixed", action="store_true",
                     help="AMEn: fp32 Krylov matvecs with FP64 refinement in the local GMRES")
@@ -370,12 +414,14 @@ def main():
         print("bench.py: this build measures one GPU only (the local-solver path has no "
               "multi-GPU mode); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    default_steps = {"eigb": (2, 2), "matvec": (200, 20), "ksl": (10, 1), "amen": (1, 1)}[args.workload]
+    default_steps = {"eigb": (2, 2), "matvec": (200, 20), "ksl": (10, 1), "amen": (1, 1),
+                     "batch": (1, 0)}[args.workload]
     if args.steps is None:
         args.steps = default_steps[0]
     if args.warmup is None:
         args.warmup = default_steps[1]
-    out = {"matvec": run_matvec, "eigb": run_eigb, "ksl": run_ksl, "amen": run_amen}[args.workload](args)
+    out = {"matvec": run_matvec, "eigb": run_eigb, "ksl": run_ksl, "amen": run_amen,
+           "batch": run_batch}[args.workload](args)
     print(json.dumps(out), flush=True)
 
 

@@ -217,6 +217,24 @@ int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, const double* 
                 int kickrank, int nswp, int max_full_size, int verb, double* result,
                 int64_t result_cap, int64_t* result_size, ttf_eigb_stats* stats);
 
+/* Many independent tt_eigb problems solved concurrently on one GPU (no
+ * counterpart in the reference, which is not re-entrant: tt_eigb.f90:6-8).
+ * Chain c has d[c] modes; its mode sizes (n = m) are n_all[sum_{c'<c} d],
+ * its operator and block-TT ranks ra_all / ry_all[sum_{c'<c} (d+1)] (ry in
+ * and out, last rank B), its operator and initial cores at
+ * crA_all + crA_off[c] and crY0_all + crY0_off[c], tolerance eps[c] and
+ * rank cap rmax[c]; all chains share B and nswp.  Results: lambda_all
+ * (B per chain), cores at result_all + result_off[c] (capacity
+ * result_off[c+1] - result_off[c] doubles; sizes in result_size[c] when
+ * given), per-chain stats when given, per-chain status (0 = solved).
+ * nworkers host threads (<= 0: 8), each with its own stream and scratch,
+ * take the chains largest first (d rmax^3).  Host pointers. */
+int ttf_tt_eigb_batch(int nchains, const int* d, const int* n_all, const int* ra_all, const double* crA_all,
+                      const int64_t* crA_off, const double* crY0_all, const int64_t* crY0_off, int* ry_all,
+                      const double* eps, const int* rmax, int B, int nswp, int nworkers, double* lambda_all,
+                      double* result_all, const int64_t* result_off, int64_t* result_size, ttf_eigb_stats* stats,
+                      int* status);
+
 /* ------------------------------------------------- AMEn building blocks */
 /* Block-Jacobi preconditioner of one local step (tt_linalg.f90:363-495,
  * djac_gen bind(c)): ptype 'c' rx1*rx2 inverted n x n blocks (layout

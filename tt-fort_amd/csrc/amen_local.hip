@@ -11,6 +11,7 @@
 #include "mlp.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <vector>
 
@@ -226,7 +227,7 @@ void jac_gen(char type, int rx1, int n, int rx2, int ra1, int ra2, const double*
   int* info = (int*)arena(kArenaSmall, sizeof(int) * (size_t)nblk);
   TTF_HIP(hipMemsetAsync(info, 0, sizeof(int) * (size_t)nblk, st));
   const size_t lds = sizeof(double) * (size_t)bs * (bs + 1);
-  static bool attr = false;
+  static std::atomic<bool> attr{false};   // concurrent first calls only repeat the call
   if (!attr) {
     TTF_HIP(hipFuncSetAttribute((const void*)batched_inverse_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)(sizeof(double) * kInvMaxN * (kInvMaxN + 1))));

@@ -77,7 +77,7 @@ extern "C" int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, con
     std::vector<int64_t> pa(d + 1, 0);
     for (int k = 0; k < d; ++k) pa[k + 1] = pa[k] + (int64_t)ra[k] * n[k] * m[k] * ra[k + 1];
     double* dA = nullptr;
-    TTF_HIP(hipMalloc(&dA, sizeof(double) * std::max<int64_t>(pa[d], 1)));
+    TTF_HIP(hipMallocAsync((void**)&dA, sizeof(double) * std::max<int64_t>(pa[d], 1), st));
     TTF_HIP(hipMemcpyAsync(dA, crA, sizeof(double) * pa[d], hipMemcpyHostToDevice, st));
     int R = rmax;
     for (int k = 0; k <= d; ++k) R = std::max(R, ry[k]);
@@ -88,20 +88,20 @@ extern "C" int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, con
     const size_t core_cap = (size_t)R * nmax * R * B;
     const size_t phi_cap = (size_t)R * ramax * R;
     std::vector<double*> core(d), phi(d + 1);
-    for (int k = 0; k < d; ++k) TTF_HIP(hipMalloc(&core[k], sizeof(double) * core_cap));
-    for (int k = 0; k <= d; ++k) TTF_HIP(hipMalloc(&phi[k], sizeof(double) * phi_cap));
+    for (int k = 0; k < d; ++k) TTF_HIP(hipMallocAsync((void**)&core[k], sizeof(double) * core_cap, st));
+    for (int k = 0; k <= d; ++k) TTF_HIP(hipMallocAsync((void**)&phi[k], sizeof(double) * phi_cap, st));
     double* tmp = nullptr;
-    TTF_HIP(hipMalloc(&tmp, sizeof(double) * core_cap));
+    TTF_HIP(hipMallocAsync((void**)&tmp, sizeof(double) * core_cap, st));
     // SVD outputs (U: mu x k, S: k, VT: k x nv) for the largest unfolding
     // every unfolding is at most (max(B, n) R) x (max(B, n) R)
     const size_t wide = (size_t)std::max(B, nmax) * R;
     const size_t svd_cap = wide * wide;
     double* U = nullptr;
-    TTF_HIP(hipMalloc(&U, sizeof(double) * svd_cap));
+    TTF_HIP(hipMallocAsync((void**)&U, sizeof(double) * svd_cap, st));
     double* VT = nullptr;
-    TTF_HIP(hipMalloc(&VT, sizeof(double) * svd_cap));
+    TTF_HIP(hipMallocAsync((void**)&VT, sizeof(double) * svd_cap, st));
     double* S = nullptr;
-    TTF_HIP(hipMalloc(&S, sizeof(double) * (size_t)R * std::max(B, nmax) * 4));
+    TTF_HIP(hipMallocAsync((void**)&S, sizeof(double) * (size_t)R * std::max(B, nmax) * 4, st));
     {
       int64_t off = 0;
       for (int k = 0; k < d; ++k) {
@@ -274,13 +274,15 @@ extern "C" int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, con
       }
     }
     TTF_HIP(hipStreamSynchronize(st));
-    for (auto p : core) TTF_HIP(hipFree(p));
-    for (auto p : phi) TTF_HIP(hipFree(p));
-    TTF_HIP(hipFree(tmp));
-    TTF_HIP(hipFree(U));
-    TTF_HIP(hipFree(VT));
-    TTF_HIP(hipFree(S));
-    TTF_HIP(hipFree(dA));
+    // stream-ordered frees: no device-wide synchronisation, so concurrent
+    // solves on other streams keep running
+    for (auto p : core) TTF_HIP(hipFreeAsync(p, st));
+    for (auto p : phi) TTF_HIP(hipFreeAsync(p, st));
+    TTF_HIP(hipFreeAsync(tmp, st));
+    TTF_HIP(hipFreeAsync(U, st));
+    TTF_HIP(hipFreeAsync(VT, st));
+    TTF_HIP(hipFreeAsync(S, st));
+    TTF_HIP(hipFreeAsync(dA, st));
     if (stats) {
       stats->sweeps = last_swp > 0 ? last_swp : nswp0;
       stats->total_matvecs = total_mv;

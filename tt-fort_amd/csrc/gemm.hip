@@ -2,6 +2,7 @@
 // small-panel kernel for products with a short K and a narrow shared B.
 #include "gemm_mfma.h"
 
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -90,7 +91,7 @@ struct KeyHash {
 namespace {
 std::mutex g_tune_mu;
 std::unordered_map<Key, Cfg, KeyHash> g_tuned;
-int g_autotune = -1;   // -1: read TTF_AUTOTUNE on first use
+std::atomic<int> g_autotune{-1};   // -1: read TTF_AUTOTUNE on first use
 long g_tune_count = 0;
 bool g_table_loaded = false;
 }  // namespace
@@ -134,11 +135,11 @@ static void record_tuned(const Key& k, const Cfg& c) {
 }
 
 static bool autotune_enabled() {
-  if (g_autotune < 0) {
+  if (g_autotune.load() < 0) {
     const char* e = std::getenv("TTF_AUTOTUNE");
-    g_autotune = (e && e[0] == '0') ? 0 : 1;
+    g_autotune.store((e && e[0] == '0') ? 0 : 1);
   }
-  return g_autotune == 1;
+  return g_autotune.load() == 1;
 }
 
 static Cfg heuristic(int M, int N, int K, int nbatch) {

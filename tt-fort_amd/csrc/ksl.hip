@@ -18,6 +18,7 @@
 #include "solvers.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -631,7 +632,7 @@ void zqr(int rows, int cols, const zc* A, zc* Q, zc* R, hipStream_t st) {
   const int rnew = std::min(rows, cols);
   TTF_REQUIRE((int64_t)rows * cols <= kQrLds, "zqr: %d x %d block exceeds %d entries", rows, cols, kQrLds);
   TTF_REQUIRE(rnew <= 64, "zqr: %d columns exceed 64", rnew);
-  static bool attr = false;
+  static std::atomic<bool> attr{false};   // concurrent first calls only repeat the call
   if (!attr) {
     TTF_HIP(hipFuncSetAttribute((const void*)zqr_gs_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)(sizeof(zc) * kQrLds)));

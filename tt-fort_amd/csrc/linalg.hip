@@ -6,6 +6,7 @@
 #include "jacobi.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -716,7 +717,7 @@ __global__ void scale_cols_kernel(int m, int k, double* __restrict__ U, int64_t 
 bool cholqr2(int p, int k, double* U, int64_t ldu, hipStream_t st) {
   if (k <= 0) return true;
   TTF_REQUIRE(k <= kCholMax, "cholqr2: %d columns exceed %d", k, kCholMax);
-  static bool attr_set = false;
+  static std::atomic<bool> attr_set{false};   // concurrent first calls only repeat the call
   if (!attr_set) {
     TTF_HIP(hipFuncSetAttribute((const void*)chol_inv_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)(sizeof(double) * (kCholMax + 1) * (kCholMax + 1))));
@@ -800,7 +801,7 @@ JacobiResult jacobi_columns(int mw, int nw, double* X, double* V, hipStream_t st
   const int nr = (int)ceil_div(mw, 16);
   auto kern = nr <= 4 ? svd_pair_kernel<4> : nr <= 8 ? svd_pair_kernel<8>
             : nr <= 16 ? svd_pair_kernel<16> : nr <= 32 ? svd_pair_kernel<32> : svd_pair_kernel<0>;
-  static bool attr_set = false;
+  static std::atomic<bool> attr_set{false};   // concurrent first calls only repeat the call
   if (!attr_set) {
     for (auto f : {svd_pair_kernel<4>, svd_pair_kernel<8>, svd_pair_kernel<16>, svd_pair_kernel<32>,
                    svd_pair_kernel<0>})
@@ -861,7 +862,7 @@ void qr_r_factor(int p, int q, double* W, hipStream_t st) {
   const int lp = p | 1;
   const int pcmax = std::min(kQrPanel, kSvdLdsDoubles / lp);
   TTF_REQUIRE(pcmax >= 1, "qr: %d rows exceed the panel LDS budget", p);
-  static bool attr_set = false;
+  static std::atomic<bool> attr_set{false};   // concurrent first calls only repeat the call
   if (!attr_set) {
     TTF_HIP(hipFuncSetAttribute((const void*)qr_panel_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)(sizeof(double) * kSvdLdsDoubles)));

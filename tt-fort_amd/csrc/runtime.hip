@@ -19,6 +19,14 @@ hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
 struct ArenaSlot { void* p = nullptr; size_t bytes = 0; };
 ArenaSlot g_arena[kArenaCount];
 std::atomic<int> g_local_bits{64};
+
+// A worker's own stream and arenas (ScopedContext); null on threads that use
+// the library-wide state.
+struct Context {
+  hipStream_t stream = nullptr;
+  ArenaSlot arena[kArenaCount];
+};
+thread_local Context* t_ctx = nullptr;
 }  // namespace
 
 int local_matvec_bits() { return g_local_bits.load(); }
@@ -57,17 +65,18 @@ void ensure_init() {
 }
 
 hipStream_t stream() {
+  if (t_ctx) return t_ctx->stream;
   ensure_init();
   return g_stream;
 }
 
 void* arena(Arena a, size_t bytes) {
   ensure_init();
-  ArenaSlot& s = g_arena[a];
+  ArenaSlot& s = t_ctx ? t_ctx->arena[a] : g_arena[a];
   if (bytes == 0) bytes = 256;
   if (s.bytes < bytes) {
     if (s.p) {
-      TTF_HIP(hipStreamSynchronize(g_stream));
+      TTF_HIP(hipStreamSynchronize(stream()));
       TTF_HIP(hipFree(s.p));
       s.p = nullptr;
       s.bytes = 0;
@@ -78,6 +87,30 @@ void* arena(Arena a, size_t bytes) {
     s.bytes = nb;
   }
   return s.p;
+}
+
+ScopedContext::ScopedContext() {
+  ensure_init();
+  TTF_REQUIRE(t_ctx == nullptr, "nested worker contexts");
+  TTF_HIP(hipSetDevice(g_device));
+  Context* c = new Context();
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    TTF_REQUIRE(false, "could not create a worker stream");
+  }
+  t_ctx = c;
+  impl_ = c;
+}
+
+ScopedContext::~ScopedContext() {
+  Context* c = static_cast<Context*>(impl_);
+  if (!c) return;
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& a : c->arena)
+    if (a.p) (void)hipFree(a.p);
+  (void)hipStreamDestroy(c->stream);
+  if (t_ctx == c) t_ctx = nullptr;
+  delete c;
 }
 
 }  // namespace ttf

@@ -61,6 +61,21 @@ int guarded(Fn&& body) {
 // outputs: contraction intermediates, solver bases).  Arenas grow with
 // hipMalloc and are released by ttf_finalize().
 hipStream_t stream();
+
+// While alive, gives the calling host thread its own stream and scratch
+// arenas (on the library's device): every library call made by that thread
+// then runs on them, so that several threads can drive independent solves
+// concurrently.  Destruction waits for the stream and frees the arenas.
+class ScopedContext {
+ public:
+  ScopedContext();
+  ~ScopedContext();
+  ScopedContext(const ScopedContext&) = delete;
+  ScopedContext& operator=(const ScopedContext&) = delete;
+
+ private:
+  void* impl_ = nullptr;
+};
 // Precision of the matvecs inside the iterative local solvers: 64 (default)
 // or 32 (fp32 matrix cores, FP64 refinement); set by ttf_set_local_precision.
 int local_matvec_bits();

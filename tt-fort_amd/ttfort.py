@@ -142,6 +142,9 @@ class KslStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+_I64P = C.POINTER(_c_i64)
+_sig("ttf_tt_eigb_batch", _c_int, _c_int, _IP, _IP, _IP, _DP, _I64P, _DP, _I64P, _IP, _DP, _IP, _c_int, _c_int,
+     _c_int, _DP, _DP, _I64P, _I64P, C.POINTER(EigbStats), _IP)
 _sig("ttf_zexpm", _c_int, _c_int, C.c_double, _vp, _vp)
 _sig("ttf_zexp_bfun3", _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, C.c_double, _c_int,
      C.c_double, _DP, _IP)
@@ -678,3 +681,41 @@ def set_local_precision(bits):
     """Precision of the matvecs inside the local iterative solvers (64 or 32)."""
     lib = load()
     check(lib.ttf_set_local_precision(bits), lib)
+
+
+def tt_eigb_batch(problems, B, nswp=20, nworkers=8):
+    """Solve many tt_eigb problems concurrently.  problems: list of dicts with
+    n, ra, crA, crY0, ry, eps, rmax.  Returns a list of (lambda, cores, ry,
+    stats) in the same order."""
+    lib = load()
+    nc = len(problems)
+    d = np.array([len(p["n"]) for p in problems], dtype=np.int32)
+    n_all = np.concatenate([np.asarray(p["n"], dtype=np.int32) for p in problems])
+    ra_all = np.concatenate([np.asarray(p["ra"], dtype=np.int32) for p in problems])
+    ry_all = np.concatenate([np.asarray(p["ry"], dtype=np.int32) for p in problems]).copy()
+    crA = [np.asarray(p["crA"], dtype=np.float64) for p in problems]
+    crY = [np.asarray(p["crY0"], dtype=np.float64) for p in problems]
+    crA_off = np.concatenate([[0], np.cumsum([a.size for a in crA])]).astype(np.int64)
+    crY_off = np.concatenate([[0], np.cumsum([a.size for a in crY])]).astype(np.int64)
+    caps = [int(sum(max(p["rmax"], max(p["ry"])) ** 2 * int(nn) for nn in p["n"]) * B + 16) for p in problems]
+    res_off = np.concatenate([[0], np.cumsum(caps)]).astype(np.int64)
+    eps = np.array([p["eps"] for p in problems], dtype=np.float64)
+    rmax = np.array([p["rmax"] for p in problems], dtype=np.int32)
+    lam = np.zeros(nc * B)
+    res = np.zeros(int(res_off[-1]))
+    sizes = np.zeros(nc, dtype=np.int64)
+    stats = (EigbStats * nc)()
+    status = np.zeros(nc, dtype=np.int32)
+    crA_all = np.concatenate(crA)
+    crY_all = np.concatenate(crY)
+    check(lib.ttf_tt_eigb_batch(nc, _ip(d), _ip(n_all), _ip(ra_all), _dp(crA_all), crA_off.ctypes.data_as(_I64P),
+                                _dp(crY_all), crY_off.ctypes.data_as(_I64P), _ip(ry_all), _dp(eps), _ip(rmax), B,
+                                nswp, nworkers, _dp(lam), _dp(res), res_off.ctypes.data_as(_I64P),
+                                sizes.ctypes.data_as(_I64P), stats, _ip(status)), lib)
+    out, orr = [], 0
+    for c in range(nc):
+        ry = ry_all[orr:orr + d[c] + 1].copy()
+        orr += d[c] + 1
+        out.append((lam[c * B:(c + 1) * B].copy(), res[res_off[c]:res_off[c] + sizes[c]].copy(), ry,
+                    stats[c].as_dict() if hasattr(stats[c], "as_dict") else None))
+    return out
