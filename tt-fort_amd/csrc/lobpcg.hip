@@ -67,14 +67,16 @@ __global__ __launch_bounds__(256) void svqb_kernel(int k, const double* __restri
   }
 }
 
-// Rayleigh-Ritz in the small space.  H (m x m) = S^T A S; mask (m) marks
-// valid basis columns; columns [0, nx) are the current Ritz vectors X.
-// Writes theta (bs), CZ (m x 2bs: Ritz coefficients C, then P-direction
-// coefficients Z), maskP (bs) and amax = max |valid Ritz value|.
+// Rayleigh-Ritz in the small space.  H (m x m) = S^T A S for the basis
+// S = [X P W] (nx, np, np columns, m = nx + 2 np); columns [0, nx) are the
+// current Ritz vectors X, the validity of the P and W columns is given by
+// maskP_in and maskW.  Writes theta (bs), CZ (m x 2bs: Ritz coefficients C,
+// then P-direction coefficients Z), maskP (bs, validity of the new P) and
+// amax = max |valid Ritz value|.
 __global__ __launch_bounds__(256) void rr_kernel(int m, int bs, int nx, const double* __restrict__ H,
-                                                 const int* __restrict__ mask, double* __restrict__ theta,
-                                                 double* __restrict__ CZ, int* __restrict__ maskP,
-                                                 double* __restrict__ amax) {
+                                                 const int* __restrict__ maskP_in, const int* __restrict__ maskW,
+                                                 double* __restrict__ theta, double* __restrict__ CZ,
+                                                 int* __restrict__ maskP, double* __restrict__ amax) {
   __shared__ double A[kMaxM][kMaxM + 1], V[kMaxM][kMaxM + 1], Y[kMaxM][kMaxM + 1];
   __shared__ double Z[kMaxM][kMaxM / 2 + 1];
   __shared__ double w[kMaxM], dg[kMaxM], red[16];
@@ -86,10 +88,15 @@ __global__ __launch_bounds__(256) void rr_kernel(int m, int bs, int nx, const do
   const double big = 1e6 * (1.0 + mx);
   // the rounding floor of the rotations is set by the valid block only: the
   // penalty rows are exactly decoupled and never rotate
+  // basis columns [X (nx) | P (np) | W (np)]: X always valid, P and W by mask
+  const int np = (m - nx) / 2;
+  auto colok = [&](int i) {
+    return i < nx ? 1 : (i < nx + np ? maskP_in[i - nx] : maskW[i - nx - np]);
+  };
   double f2 = 0.0;
   for (int e = tid; e < m * m; e += blockDim.x) {
     int i = e % m, j = e / m;
-    bool ok = mask[i] && mask[j];
+    bool ok = colok(i) && colok(j);
     double h = 0.5 * (H[i + m * j] + H[j + m * i]);
     A[i][j] = ok ? h : ((i == j) ? big : 0.0);
     V[i][j] = (i == j) ? 1.0 : 0.0;
@@ -259,14 +266,13 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
   TTF_REQUIRE(3 * bs <= kMaxM && 3 * bs <= N, "block_eig: block %d too large for N=%d", bs, N);
   const int64_t ld = N;
   const size_t colsz = (size_t)N;
-  double* base = (double*)arena(kArenaSolver, sizeof(double) * colsz * (3 + 3 + 2 + 2 + 1) * bs);
-  double* S = base;                       // [X W P]   N x 3bs
-  double* AS = S + colsz * 3 * bs;        // [AX AW AP]
-  double* Sn = AS + colsz * 3 * bs;       // [Xn Pn]   N x 2bs
-  double* ASn = Sn + colsz * 2 * bs;
-  double* Wt = ASn + colsz * 2 * bs;      // N x bs
-  double* X = S, *W = S + colsz * bs, *P = S + colsz * 2 * bs;
-  double* AX = AS, *AW = AS + colsz * bs, *AP = AS + colsz * 2 * bs;
+  // two basis sets (ping-pong), each S = [X P W] (N x 3bs) followed by
+  // AS = [AX AP AW]; the Rayleigh-Ritz update writes the next set's [X P]
+  // and [AX AP] in one batched product; Wr stages the residual block
+  const size_t set_sz = colsz * 6 * bs;
+  double* base = (double*)arena(kArenaSolver, sizeof(double) * (2 * set_sz + colsz * bs));
+  double* Wr = base + 2 * set_sz;
+  auto Sset = [&](int k) { return base + (size_t)k * set_sz; };
   // small device buffers
   const int M2 = kMaxM * kMaxM;
   double* sm = (double*)arena(kArenaSmall, sizeof(double) * (4 * M2 + 4 * kMaxM) + sizeof(int) * 4 * kMaxM);
@@ -277,101 +283,108 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
   double* theta = CZ + M2;
   double* nrm = theta + kMaxM;
   double* amax = nrm + kMaxM;
-  int* imask = (int*)(amax + 2 * kMaxM);   // m entries: validity of S columns
-  int* maskW = imask + kMaxM;
-  int* maskP = maskW + kMaxM;
+  int* maskW = (int*)(amax + 2 * kMaxM);
+  int* maskPb[2] = {maskW + kMaxM, maskW + 2 * kMaxM};   // ping-pong validity of P
+  TTF_HIP(hipMemsetAsync(maskPb[0], 0, sizeof(int) * 2 * kMaxM, st));
+  for (int k = 0; k < 2; ++k) {   // P and AP start at zero (and masked out)
+    TTF_HIP(hipMemsetAsync(Sset(k) + colsz * bs, 0, sizeof(double) * colsz * bs, st));
+    TTF_HIP(hipMemsetAsync(Sset(k) + colsz * 4 * bs, 0, sizeof(double) * colsz * bs, st));
+  }
 
-  // initial block: guess + guard vectors, SVQB, A X, Rayleigh-Ritz on X
-  copy_cols(Xout, N, X, ld, N, nev, st);
-  hipLaunchKernelGGL(random_cols, dim3(grid_for((int64_t)N * (bs - nev))), dim3(256), 0, st, N, nev,
-                     bs, X, ld, 12345u + (unsigned)N);
-  TTF_HIP(hipGetLastError());
-  TTF_HIP(hipMemsetAsync(P, 0, sizeof(double) * colsz * bs, st));
-  TTF_HIP(hipMemsetAsync(AP, 0, sizeof(double) * colsz * bs, st));
-  for (int attempt = 0; attempt < 3; ++attempt) {
-    tsgram(N, bs, bs, X, ld, X, ld, Gm, bs, st);
-    hipLaunchKernelGGL(svqb_kernel, dim3(1), dim3(256), 0, st, bs, Gm, Tm, maskW);
-    mul_small(N, bs, bs, X, ld, Tm, bs, Wt, ld, 0.0, st);
-    copy_cols(Wt, ld, X, ld, N, bs, st);
-    std::vector<int> hm(bs);
-    TTF_HIP(hipMemcpyAsync(hm.data(), maskW, sizeof(int) * bs, hipMemcpyDeviceToHost, st));
-    TTF_HIP(hipStreamSynchronize(st));
-    int bad = -1;
-    for (int i = 0; i < bs; ++i) if (!hm[i]) { bad = i; break; }
-    if (bad < 0) break;
-    // a rank-deficient start: replace the dropped directions by random ones
-    for (int i = 0; i < bs; ++i)
-      if (!hm[i]) {
-        hipLaunchKernelGGL(random_cols, dim3(grid_for(N)), dim3(256), 0, st, N, i, i + 1, X, ld,
-                           777u * (attempt + 1) + i);
-      }
-  }
-  op.apply(bs, X, ld, AX, ld, st);
-  int matvecs = bs;
+  // initial block in set 0: guess + guard vectors, SVQB, A X, Rayleigh-Ritz
+  // on X into set 1
   {
-    std::vector<int> ones(kMaxM, 1);
-    TTF_HIP(hipMemcpyAsync(imask, ones.data(), sizeof(int) * kMaxM, hipMemcpyHostToDevice, st));
+    double* X = Sset(0);
+    double* AX = X + colsz * 3 * bs;
+    copy_cols(Xout, N, Wr, ld, N, nev, st);
+    hipLaunchKernelGGL(random_cols, dim3(grid_for((int64_t)N * (bs - nev))), dim3(256), 0, st, N, nev,
+                       bs, Wr, ld, 12345u + (unsigned)N);
+    TTF_HIP(hipGetLastError());
+    for (int attempt = 0; attempt < 3; ++attempt) {
+      tsgram(N, bs, bs, Wr, ld, Wr, ld, Gm, bs, st);
+      hipLaunchKernelGGL(svqb_kernel, dim3(1), dim3(256), 0, st, bs, Gm, Tm, maskW);
+      mul_small(N, bs, bs, Wr, ld, Tm, bs, X, ld, 0.0, st);
+      std::vector<int> hm(bs);
+      TTF_HIP(hipMemcpyAsync(hm.data(), maskW, sizeof(int) * bs, hipMemcpyDeviceToHost, st));
+      TTF_HIP(hipStreamSynchronize(st));
+      bool bad = false;
+      for (int i = 0; i < bs; ++i) bad = bad || !hm[i];
+      if (!bad) break;
+      // a rank-deficient start: replace the dropped directions by random ones
+      copy_cols(X, ld, Wr, ld, N, bs, st);
+      for (int i = 0; i < bs; ++i)
+        if (!hm[i]) {
+          hipLaunchKernelGGL(random_cols, dim3(grid_for(N)), dim3(256), 0, st, N, i, i + 1, Wr, ld,
+                             777u * (attempt + 1) + i);
+        }
+    }
+    op.apply(bs, X, ld, AX, ld, st);
     tsgram(N, bs, bs, X, ld, AX, ld, Hm, bs, st);
-    hipLaunchKernelGGL(rr_kernel, dim3(1), dim3(256), 0, st, bs, bs, bs, Hm, imask, theta, CZ, maskP,
-                       amax);
-    mul_small(N, bs, bs, X, ld, CZ, bs, Sn, ld, 0.0, st);
-    mul_small(N, bs, bs, AX, ld, CZ, bs, ASn, ld, 0.0, st);
-    copy_cols(Sn, ld, X, ld, N, bs, st);
-    copy_cols(ASn, ld, AX, ld, N, bs, st);
+    hipLaunchKernelGGL(rr_kernel, dim3(1), dim3(256), 0, st, bs, bs, bs, Hm, maskPb[0], maskW, theta, CZ,
+                       maskPb[1], amax);
+    double* Xn = Sset(1);
+    mul_small(N, bs, bs, X, ld, CZ, bs, Xn, ld, 0.0, st);
+    mul_small(N, bs, bs, AX, ld, CZ, bs, Xn + colsz * 3 * bs, ld, 0.0, st);
+    TTF_HIP(hipMemsetAsync(maskPb[1], 0, sizeof(int) * kMaxM, st));
   }
-  bool have_p = false;
+  int matvecs = bs;
+  int cur = 1;
   double anorm = 0.0;
   std::vector<double> hbuf(2 * kMaxM + 1);
+  // the residual norms are read back (a host synchronisation) at the first
+  // iterations and then every kSyncEvery: in between, iterations are queued
+  // without waiting, and at most kSyncEvery - 1 run past convergence
+  constexpr int kSyncEvery = 4;
   for (int it = 0;; ++it) {
+    double* S = Sset(cur);
+    double* X = S, *P = S + colsz * bs, *W = S + colsz * 2 * bs;
+    double* AS = S + colsz * 3 * bs;
+    double* AX = AS, *AW = AS + colsz * 2 * bs;
     hipLaunchKernelGGL(residual_kernel, dim3(grid_for((int64_t)N * bs)), dim3(256), 0, st, N, bs, X,
-                       AX, ld, theta, W);
-    hipLaunchKernelGGL(colnorms_kernel, dim3(nev), dim3(256), 0, st, N, W, ld, nrm);
-    TTF_HIP(hipMemcpyAsync(hbuf.data(), theta, sizeof(double) * (2 * kMaxM + 1), hipMemcpyDeviceToHost, st));
-    TTF_HIP(hipStreamSynchronize(st));
-    anorm = std::max(anorm, hbuf[2 * kMaxM]);   // amax follows nrm in the buffer
-    double res2 = 0.0, rmax = 0.0;
-    for (int i = 0; i < nev; ++i) {
-      res2 += hbuf[kMaxM + i] * hbuf[kMaxM + i];
-      rmax = std::max(rmax, hbuf[kMaxM + i]);
-    }
-    stats.iters = it;
-    stats.res = std::sqrt(res2);
-    if (rmax <= tol * std::max(anorm, 1e-300) || matvecs + bs > max_matvecs) break;
-    // W <- W - [X P] ([X P]^T W), twice
-    const int nb = have_p ? 3 : 2;   // basis blocks after this step: S = [X W (P)]
-    for (int pass = 0; pass < 2; ++pass) {
-      tsgram(N, bs, bs, X, ld, W, ld, Gm, bs, st);
-      GemmOp<double> g = colmajor<double>('N', 'N', N, bs, bs, -1.0, X, ld, Gm, bs, 1.0, W, ld);
-      gemm_launch(g, st);
-      if (have_p) {
-        tsgram(N, bs, bs, P, ld, W, ld, Gm, bs, st);
-        GemmOp<double> h = colmajor<double>('N', 'N', N, bs, bs, -1.0, P, ld, Gm, bs, 1.0, W, ld);
-        gemm_launch(h, st);
+                       AX, ld, theta, Wr);
+    hipLaunchKernelGGL(colnorms_kernel, dim3(nev), dim3(256), 0, st, N, Wr, ld, nrm);
+    const bool at_cap = matvecs + bs > max_matvecs;
+    if (it < 3 || it % kSyncEvery == 0 || at_cap) {
+      TTF_HIP(hipMemcpyAsync(hbuf.data(), theta, sizeof(double) * (2 * kMaxM + 1), hipMemcpyDeviceToHost, st));
+      TTF_HIP(hipStreamSynchronize(st));
+      anorm = std::max(anorm, hbuf[2 * kMaxM]);   // amax follows nrm in the buffer
+      double res2 = 0.0, rmax = 0.0;
+      for (int i = 0; i < nev; ++i) {
+        res2 += hbuf[kMaxM + i] * hbuf[kMaxM + i];
+        rmax = std::max(rmax, hbuf[kMaxM + i]);
       }
+      stats.iters = it;
+      stats.res = std::sqrt(res2);
+      if (rmax <= tol * std::max(anorm, 1e-300) || at_cap) break;
     }
-    // SVQB on W
-    tsgram(N, bs, bs, W, ld, W, ld, Gm, bs, st);
+    // Wr <- Wr - [X P] ([X P]^T Wr), twice (P is zero before the first update)
+    for (int pass = 0; pass < 2; ++pass) {
+      tsgram(N, 2 * bs, bs, X, ld, Wr, ld, Gm, 2 * bs, st);
+      gemm_launch(colmajor<double>('N', 'N', N, bs, 2 * bs, -1.0, X, ld, Gm, 2 * bs, 1.0, Wr, ld), st);
+    }
+    // SVQB: W = Wr T, then A W
+    tsgram(N, bs, bs, Wr, ld, Wr, ld, Gm, bs, st);
     hipLaunchKernelGGL(svqb_kernel, dim3(1), dim3(256), 0, st, bs, Gm, Tm, maskW);
-    mul_small(N, bs, bs, W, ld, Tm, bs, Wt, ld, 0.0, st);
-    copy_cols(Wt, ld, W, ld, N, bs, st);
+    mul_small(N, bs, bs, Wr, ld, Tm, bs, W, ld, 0.0, st);
     op.apply(bs, W, ld, AW, ld, st);
     matvecs += bs;
-    // validity mask of S = [X W P]
-    TTF_HIP(hipMemcpyAsync(imask + bs, maskW, sizeof(int) * bs, hipMemcpyDeviceToDevice, st));
-    if (have_p) TTF_HIP(hipMemcpyAsync(imask + 2 * bs, maskP, sizeof(int) * bs, hipMemcpyDeviceToDevice, st));
-    const int m = nb * bs;
+    // Rayleigh-Ritz on S = [X P W]
+    const int m = 3 * bs;
     tsgram(N, m, m, S, ld, AS, ld, Hm, m, st);
-    hipLaunchKernelGGL(rr_kernel, dim3(1), dim3(256), 0, st, m, bs, bs, Hm, imask, theta, CZ, maskP,
-                       amax);
-    mul_small(N, m, 2 * bs, S, ld, CZ, m, Sn, ld, 0.0, st);
-    mul_small(N, m, 2 * bs, AS, ld, CZ, m, ASn, ld, 0.0, st);
-    copy_cols(Sn, ld, X, ld, N, bs, st);
-    copy_cols(Sn + colsz * bs, ld, P, ld, N, bs, st);
-    copy_cols(ASn, ld, AX, ld, N, bs, st);
-    copy_cols(ASn + colsz * bs, ld, AP, ld, N, bs, st);
-    have_p = true;
+    hipLaunchKernelGGL(rr_kernel, dim3(1), dim3(256), 0, st, m, bs, bs, Hm, maskPb[cur & 1], maskW, theta, CZ,
+                       maskPb[(cur + 1) & 1], amax);
+    // next set: [X P] = S [C Z] and [AX AP] = AS [C Z], one batched product
+    double* Sn = Sset(1 - cur);
+    GemmOp<double> g = colmajor<double>('N', 'N', N, 2 * bs, m, 1.0, S, ld, CZ, m, 0.0, Sn, ld);
+    g.nb0 = 2;
+    g.nbatch = 2;
+    g.sAb0 = (int64_t)colsz * 3 * bs;   // S -> AS
+    g.sBb0 = 0;
+    g.sCb0 = (int64_t)colsz * 3 * bs;   // next S -> next AS
+    gemm_launch(g, st);
+    cur = 1 - cur;
   }
-  copy_cols(X, ld, Xout, N, N, nev, st);
+  copy_cols(Sset(cur), ld, Xout, N, N, nev, st);
   for (int i = 0; i < nev; ++i) lambda[i] = hbuf[i];
   stats.matvecs = matvecs;
   stats.anorm = anorm;
