@@ -292,11 +292,16 @@ __device__ inline double hestenes_pair(double* ap, double* aq, int mw, int l16, 
   }
   *s_out = 0.0;
   if (!(al > small2 && be > small2)) return 0.0;
-  const double rel = fabs(ga) / sqrt(al * be);
+  // coupling |ga| / sqrt(al be) from the refined reciprocal square root
+  const double ab = al * be;
+  double ri = __builtin_amdgcn_rsq(ab);
+  ri = ri * (1.5 - 0.5 * ab * ri * ri);
+  const double rel = fabs(ga) * ri;
   if (rel <= tol) return rel;
-  const double zeta = (be - al) / (2.0 * ga);
-  const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-  const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+  // the rotation that orthogonalises the two columns is the Jacobi rotation
+  // of their 2x2 Gram matrix [[al, ga], [ga, be]]
+  double c, s;
+  jacobi_cs(ga, al, be, true, c, s);
   if constexpr (NR > 0) {
 #pragma unroll
     for (int t2 = 0; t2 < NR; ++t2) {
@@ -744,6 +749,18 @@ bool cholqr2(int p, int k, double* U, int64_t ldu, hipStream_t st) {
   return true;
 }
 
+// Columns per block pair of the one-sided Jacobi (TTF_SVD_PAIR overrides,
+// 4..32; default 32): smaller pairs give more work-groups per round and
+// shorter inner sweeps, but more rounds.
+static int svd_pair_columns() {
+  static int v = [] {
+    const char* e = std::getenv("TTF_SVD_PAIR");
+    int x = e ? std::atoi(e) : kSvdMaxP;
+    return std::min(std::max(x, 4), (int)kSvdMaxP);
+  }();
+  return v;
+}
+
 // Hestenes sweeps per pair and round (TTF_SVD_INNER overrides; default 1).
 static int svd_inner_sweeps() {
   static int v = [] {
@@ -788,7 +805,7 @@ JacobiResult jacobi_columns(int mw, int nw, double* X, double* V, hipStream_t st
   res.small2 = mw * eps * eps * fro2;
   // block size: the pair's 2b columns (odd stride mw | 1) must fit the LDS budget
   const int ldp = mw | 1;
-  int two_b = std::min<int>(kSvdMaxP, kSvdLdsDoubles / ldp);
+  int two_b = std::min<int>({kSvdMaxP, kSvdLdsDoubles / ldp, svd_pair_columns()});
   two_b -= (two_b & 1);
   TTF_REQUIRE(two_b >= 2, "jacobi: %d rows exceed the pair LDS budget", mw);
   // at least two non-empty blocks, so that every column pair is coupled
