@@ -48,7 +48,7 @@ __device__ __forceinline__ void jacobi_cs(double apq, double app, double aqq, bo
   double cc = __builtin_amdgcn_rsq(x);
   cc = cc * (1.5 - 0.5 * x * cc * cc);
   cc = cc * (1.5 - 0.5 * x * cc * cc);
-  c = cc;
+  c = go ? cc : 1.0;
   s = t * cc;
 }
 
@@ -59,17 +59,28 @@ __device__ __forceinline__ bool jacobi_go(double apq, double app, double aqq, do
   return fabs(apq) > tol_abs && apq * apq > eps * eps * fabs(app * aqq);
 }
 
+// Rotations with |sin| at most this are "small": a sweep whose rotations
+// are all small leaves off-diagonal entries of order n * kJacobiSmall^2 *
+// ||A|| (quadratic convergence), below the rounding floor for n <= 64, so
+// the next sweep would not change the result and is skipped.
+constexpr double kJacobiSmall = 1e-9;
+
 // Diagonalises the symmetric n x n matrix held in A (row stride LD, n <= 64)
-// and accumulates the rotations into V (V <- V J).  B is a second n x n
-// buffer: each round of the circle schedule reads one buffer and writes the
-// other, so one barrier per round suffices.  Every thread owns 2x2 blocks
-// (rows of pair kp x columns of pair kq) and derives both rotations from
-// the round's input itself, so no phase publishes rotations; the thread also
-// rotates rows kp and kp + nn/2 of V for pair kq.  The 2x2 blocks are
+// and accumulates the rotations into V (V <- V J).  B is a second buffer:
+// each round of the circle schedule reads one buffer and writes the other,
+// so one barrier per round suffices.  Odd n is padded with a zero row and
+// column n (in A, B and V; the buffers hold at least n + 1 rows), so every
+// pair is real and no load is conditional; rotations involving the pad are
+// the identity.  Thread t owns the 2x2 block (rows of pair kp) x (columns of
+// pair kq), t = kp + half * kq, derives both rotations from the round's
+// input itself (no phase publishes rotations) and also rotates rows kp and
+// kp + half of V in the columns of pair kq.  All of a task's loads are
+// issued before its arithmetic so that one LDS round trip and the two
+// interleaved rotation computations make up the round.  The 2x2 blocks are
 // updated in full (annihilated entries keep their rounding-level values).
-// The loop ends after a sweep without rotation or after max_sweeps.
-// `flags` is 3 ints of LDS.  All threads of the block call it; the returned
-// pointer (A or B) holds the diagonalised matrix.
+// The loop ends after a sweep with no rotation above kJacobiSmall or after
+// max_sweeps.  `flags` is 3 ints of LDS.  All threads of the block call it;
+// the returned pointer (A or B) holds the diagonalised matrix.
 template <int LD>
 __device__ double (*jacobi_sym(double (*A)[LD], double (*B)[LD], double (*V)[LD], int n, int max_sweeps,
                                double tol_abs, int* flags))[LD] {
@@ -77,59 +88,62 @@ __device__ double (*jacobi_sym(double (*A)[LD], double (*B)[LD], double (*V)[LD]
   const int nn = n + (n & 1);
   const int half = nn / 2;
   if (n <= 1) return A;
-  // thread -> (kp, kq0); threads with kq0 >= kqstep would repeat columns
-  const int kp = tid % half, kq0 = tid / half, kqstep = max(nt / half, 1);
-  const bool active = kq0 < kqstep && kq0 < half;
+  if (n & 1) {
+    for (int j = tid; j <= n; j += nt) {
+      A[n][j] = 0.0;
+      A[j][n] = 0.0;
+      B[n][j] = 0.0;
+      B[j][n] = 0.0;
+      V[n][j] = 0.0;
+      V[j][n] = 0.0;
+    }
+  }
+  const int ntask = half * half;
   double(*cur)[LD] = A;
   double(*nxt)[LD] = B;
   if (tid < 3) flags[tid] = 0;
   __syncthreads();
   for (int sweep = 0; sweep < max_sweeps; ++sweep) {
-    int* rotated = flags + sweep % 3;
+    int* moved = flags + sweep % 3;
     // flags[(sweep+1) % 3] was last read before the previous sweep's barriers
     if (tid == 0) flags[(sweep + 1) % 3] = 0;
     for (int r = 0; r < nn - 1; ++r) {
-      if (active) {
-        int p, q;
+      for (int t = tid; t < ntask; t += nt) {
+        const int kp = t % half, kq = t / half;
+        int p, q, u, v;
         jacobi_pair(r, kp, nn, p, q);
-        const bool hq = q < n;
-        const double apq = hq ? cur[p][q] : 0.0, app = cur[p][p], aqq = hq ? cur[q][q] : 0.0;
-        const bool go1 = hq && jacobi_go(apq, app, aqq, tol_abs);
-        if (go1) *rotated = 1;
-        for (int kq = kq0; kq < half; kq += kqstep) {
-          int u, v;
-          jacobi_pair(r, kq, nn, u, v);
-          const bool hv = v < n;
-          const double auv = hv ? cur[u][v] : 0.0, auu = cur[u][u], avv = hv ? cur[v][v] : 0.0;
-          double apu = cur[p][u], apv = hv ? cur[p][v] : 0.0;
-          double aqu = hq ? cur[q][u] : 0.0, aqv = (hq && hv) ? cur[q][v] : 0.0;
-          double c1, s1, c2, s2;
-          jacobi_cs(apq, app, aqq, go1, c1, s1);
-          jacobi_cs(auv, auu, avv, hv && jacobi_go(auv, auu, avv, tol_abs), c2, s2);
-          // columns u, v
-          const double t0 = c2 * apu - s2 * apv, t1 = s2 * apu + c2 * apv;
-          const double t2 = c2 * aqu - s2 * aqv, t3 = s2 * aqu + c2 * aqv;
-          // rows p, q
-          nxt[p][u] = c1 * t0 - s1 * t2;
-          if (hv) nxt[p][v] = c1 * t1 - s1 * t3;
-          if (hq) nxt[q][u] = s1 * t0 + c1 * t2;
-          if (hq && hv) nxt[q][v] = s1 * t1 + c1 * t3;
-          // rows kp and kp + half of V, columns of pair kq
-          if (s2 != 0.0) {
-            for (int i = kp; i < n; i += half) {
-              const double x = V[i][u], y = V[i][v];
-              V[i][u] = c2 * x - s2 * y;
-              V[i][v] = s2 * x + c2 * y;
-            }
-          }
-        }
+        jacobi_pair(r, kq, nn, u, v);
+        const double app = cur[p][p], aqq = cur[q][q], apq = cur[p][q];
+        const double auu = cur[u][u], avv = cur[v][v], auv = cur[u][v];
+        const double apu = cur[p][u], apv = cur[p][v], aqu = cur[q][u], aqv = cur[q][v];
+        const int i1 = kp + half;
+        const double x0 = V[kp][u], y0 = V[kp][v], x1 = V[i1][u], y1 = V[i1][v];
+        const bool go1 = jacobi_go(apq, app, aqq, tol_abs);
+        const bool go2 = jacobi_go(auv, auu, avv, tol_abs);
+        double c1, s1, c2, s2;
+        jacobi_cs(apq, app, aqq, go1, c1, s1);
+        jacobi_cs(auv, auu, avv, go2, c2, s2);
+        if (kq == 0 && fabs(s1) > kJacobiSmall) *moved = 1;
+        // columns u, v
+        const double t0 = c2 * apu - s2 * apv, t1 = s2 * apu + c2 * apv;
+        const double t2 = c2 * aqu - s2 * aqv, t3 = s2 * aqu + c2 * aqv;
+        // rows p, q
+        nxt[p][u] = c1 * t0 - s1 * t2;
+        nxt[p][v] = c1 * t1 - s1 * t3;
+        nxt[q][u] = s1 * t0 + c1 * t2;
+        nxt[q][v] = s1 * t1 + c1 * t3;
+        // rows kp and kp + half of V, columns of pair kq
+        V[kp][u] = c2 * x0 - s2 * y0;
+        V[kp][v] = s2 * x0 + c2 * y0;
+        V[i1][u] = c2 * x1 - s2 * y1;
+        V[i1][v] = s2 * x1 + c2 * y1;
       }
       __syncthreads();
-      double(*t)[LD] = cur;
+      double(*tt)[LD] = cur;
       cur = nxt;
-      nxt = t;
+      nxt = tt;
     }
-    if (!*rotated) break;
+    if (!*moved) break;
   }
   return cur;
 }
