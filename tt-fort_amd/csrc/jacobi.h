@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "wave.h"
+
 namespace ttf {
 
 constexpr int kJacobiMaxN = 64;   // largest order (rows are indexed by lanes)
@@ -151,8 +153,7 @@ __device__ double (*jacobi_sym(double (*A)[LD], double (*B)[LD], double (*V)[LD]
 // Block-wide sum of v over all threads (blockDim.x a multiple of 64, at most
 // 1024); `red` holds one double per wave.  Every thread receives the sum.
 __device__ inline double block_sum(double v, double* red) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  v = wave_sum64(v);
   const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
   __syncthreads();
   if ((threadIdx.x & 63) == 0) red[w] = v;

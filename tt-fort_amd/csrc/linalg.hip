@@ -4,6 +4,7 @@
 #include "linalg.h"
 #include "gemm_mfma.h"
 #include "jacobi.h"
+#include "wave.h"
 
 #include <algorithm>
 #include <atomic>
@@ -257,9 +258,10 @@ __device__ inline void lds_to_cols(const double* src, int ld, int rows, int n, d
 
 // A column pair's 2x2 Hestenes step inside the pair block: returns the
 // measured coupling (0 for a column at the noise floor) and rotates the two
-// LDS columns when it exceeds tol.  16 threads (one row segment each) share
-// a pair; with NR > 0 each thread keeps its NR elements per column in
-// registers between the dot products and the update (mw <= 16 NR).
+// LDS columns when it exceeds tol.  The 16 lanes of one DPP row (one row
+// segment each) share a pair and reduce the dot products in registers; with
+// NR > 0 each thread keeps its NR elements per column in registers between
+// the dot products and the update (mw <= 16 NR).
 template <int NR>
 __device__ inline double hestenes_pair(double* ap, double* aq, int mw, int l16, double small2, double tol,
                                        double* c_out, double* s_out) {
@@ -284,12 +286,9 @@ __device__ inline double hestenes_pair(double* ap, double* aq, int mw, int l16, 
       al += x * x; be += y * y; ga += x * y;
     }
   }
-#pragma unroll
-  for (int o = 8; o > 0; o >>= 1) {
-    al += __shfl_xor(al, o, 16);
-    be += __shfl_xor(be, o, 16);
-    ga += __shfl_xor(ga, o, 16);
-  }
+  al = row_sum16(al);
+  be = row_sum16(be);
+  ga = row_sum16(ga);
   *s_out = 0.0;
   if (!(al > small2 && be > small2)) return 0.0;
   // coupling |ga| / sqrt(al be) from the refined reciprocal square root
@@ -477,8 +476,7 @@ __global__ __launch_bounds__(256) void qr_panel_kernel(int rows, int pc, double*
         double* y = P + c * lp;
         double w = (l16 == 0) ? y[j] : 0.0;
         for (int i = j + 1 + l16; i < rows; i += 16) w += x[i] * y[i];
-#pragma unroll
-        for (int o = 8; o > 0; o >>= 1) w += __shfl_xor(w, o, 16);
+        w = row_sum16(w);
         w *= t;
         if (l16 == 0) y[j] -= w;
         for (int i = j + 1 + l16; i < rows; i += 16) y[i] -= w * x[i];

@@ -18,6 +18,7 @@
 // matrix so they can never produce a spurious Ritz value.
 #include "solvers.h"
 #include "jacobi.h"
+#include "wave.h"
 
 #include <cmath>
 #include <vector>
@@ -152,21 +153,17 @@ __global__ __launch_bounds__(256) void rr_kernel(int m, int bs, int nx, const do
     __syncthreads();
   }
   // modified Gram-Schmidt on the columns of Z (twice) by the first wave,
-  // lane r owning row r (m <= 64): dot products are wave reductions through
-  // lane shuffles and each lane touches only its own row, so the wave needs
-  // no barrier.
+  // lane r owning row r (m <= 64): dot products are wave sums in registers
+  // (DPP rows, then lane reads) and each lane touches only its own row, so
+  // the wave needs no barrier.
   if (tid < 64) {
     for (int pass = 0; pass < 2; ++pass) {
       for (int c = 0; c < bs; ++c) {
         for (int j = 0; j < c; ++j) {
-          double s = (tid < m) ? Z[tid][j] * Z[tid][c] : 0.0;
-#pragma unroll
-          for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+          const double s = wave_sum64((tid < m) ? Z[tid][j] * Z[tid][c] : 0.0);
           if (tid < m) Z[tid][c] -= s * Z[tid][j];
         }
-        double s = (tid < m) ? Z[tid][c] * Z[tid][c] : 0.0;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        const double s = wave_sum64((tid < m) ? Z[tid][c] * Z[tid][c] : 0.0);
         double nrm = sqrt(s);
         double inv = nrm > 1e-10 ? 1.0 / nrm : 0.0;
         if (tid < m) Z[tid][c] *= inv;
