@@ -19,6 +19,11 @@ namespace ttf {
 // ------------------------------------------------------------------ tsgram
 constexpr int kTsgRows = 64;
 
+// G (p x q) = X^T Y for tall X (N x p), Y (N x q) in two launches: every
+// work-group of tsgram_partial sums row chunks of 64 into its partial p x q
+// block, and tsgram_reduce adds the partials in work-group order (a fixed
+// order, so the result is reproducible), `sub` threads per output over
+// interleaved partials with their sub-sums added in order.
 __global__ __launch_bounds__(256) void tsgram_partial(int N, int p, int q, const double* __restrict__ X,
                                                       int64_t ldx, const double* __restrict__ Y,
                                                       int64_t ldy, double* __restrict__ part) {
@@ -59,15 +64,26 @@ __global__ __launch_bounds__(256) void tsgram_partial(int N, int p, int q, const
   }
 }
 
-__global__ void tsgram_reduce(int p, int q, int nparts, const double* __restrict__ part,
-                              double* __restrict__ G, int ldg) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  int pq = p * q;
-  if (e >= pq) return;
+constexpr int kTsgSub = 4;   // threads per output in the reduction
+
+__global__ __launch_bounds__(256) void tsgram_reduce(int p, int q, int nparts, const double* __restrict__ part,
+                                                     double* __restrict__ G, int ldg) {
+  __shared__ double red[256];
+  const int pq = p * q, tid = threadIdx.x;
+  const int e = blockIdx.x * (256 / kTsgSub) + tid / kTsgSub, k = tid % kTsgSub;
   double s = 0.0;
+  if (e < pq) {
 #pragma unroll 16
-  for (int w = 0; w < nparts; ++w) s += part[(int64_t)w * pq + e];
-  G[(e % p) + (int64_t)ldg * (e / p)] = s;
+    for (int w = k; w < nparts; w += kTsgSub) s += part[(int64_t)w * pq + e];
+  }
+  red[tid] = s;
+  __syncthreads();
+  if (k == 0 && e < pq) {
+    double t = red[tid];
+#pragma unroll
+    for (int j = 1; j < kTsgSub; ++j) t += red[tid + j];
+    G[(e % p) + (int64_t)ldg * (e / p)] = t;
+  }
 }
 
 void tsgram(int N, int p, int q, const double* X, int64_t ldx, const double* Y, int64_t ldy,
@@ -78,7 +94,7 @@ void tsgram(int N, int p, int q, const double* X, int64_t ldx, const double* Y, 
   double* part = (double*)arena(kArenaGram, sizeof(double) * (size_t)nparts * p * q);
   hipLaunchKernelGGL(tsgram_partial, dim3(nparts), dim3(256), 0, st, N, p, q, X, ldx, Y, ldy, part);
   TTF_HIP(hipGetLastError());
-  hipLaunchKernelGGL(tsgram_reduce, dim3((unsigned)ceil_div(p * q, 256)), dim3(256), 0, st, p, q,
+  hipLaunchKernelGGL(tsgram_reduce, dim3((unsigned)ceil_div(p * q, 256 / kTsgSub)), dim3(256), 0, st, p, q,
                      nparts, part, G, ldg);
   TTF_HIP(hipGetLastError());
 }
@@ -826,6 +842,11 @@ JacobiResult jacobi_columns(int mw, int nw, double* X, double* V, hipStream_t st
   }
   const int inner = svd_inner_sweeps();
   unsigned long long hoff = 0;
+  // Convergence is quadratic at the end: after a sweep whose largest
+  // coupling (measured before its rotations) was at most kQuadStop, the
+  // couplings left are of order nw * kQuadStop^2, below tol, so the sweep
+  // that would only confirm it is not run.
+  const double kQuadStop = std::min(1e-9, 0.1 * std::sqrt(tol / nw));
   // cyclic one-sided Jacobi needs about 40 sweeps on unpreconditioned
   // columns with a 1e-13 graded spectrum (10 after the QR preconditioning)
   for (res.sweeps = 1; res.sweeps <= 60; ++res.sweeps) {
@@ -839,7 +860,7 @@ JacobiResult jacobi_columns(int mw, int nw, double* X, double* V, hipStream_t st
     TTF_HIP(hipStreamSynchronize(st));
     double off;
     std::memcpy(&off, &hoff, sizeof(off));
-    if (off <= tol) break;
+    if (off <= tol || off <= kQuadStop) break;
   }
   return res;
 }

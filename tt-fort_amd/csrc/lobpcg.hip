@@ -178,34 +178,40 @@ __global__ __launch_bounds__(256) void rr_kernel(int m, int bs, int nx, const do
   }
 }
 
-// W = AX - X diag(theta)
-__global__ void residual_kernel(int N, int bs, const double* __restrict__ X,
-                                const double* __restrict__ AX, int64_t ld,
-                                const double* __restrict__ theta, double* __restrict__ W) {
-  int64_t total = (int64_t)N * bs;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    int c = (int)(e / N);
-    int64_t i = e % N;
-    W[i + ld * c] = AX[i + ld * c] - theta[c] * X[i + ld * c];
+// W = AX - X diag(theta); each work-group takes a range of rows and stores
+// its per-column sums of squares to part (nparts x bs) ...
+__global__ __launch_bounds__(256) void residual_kernel(int N, int bs, const double* __restrict__ X,
+                                                       const double* __restrict__ AX, int64_t ld,
+                                                       const double* __restrict__ theta, double* __restrict__ W,
+                                                       double* __restrict__ part) {
+  __shared__ double red[4][kMaxM];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int per = (N + gridDim.x - 1) / gridDim.x;
+  const int r0 = blockIdx.x * per, r1 = min(N, r0 + per);
+  for (int c = 0; c < bs; ++c) {
+    const double th = theta[c];
+    double s = 0.0;
+    for (int i = r0 + tid; i < r1; i += 256) {
+      const double w = AX[i + ld * c] - th * X[i + ld * c];
+      W[i + ld * c] = w;
+      s += w * w;
+    }
+    s = wave_sum64(s);
+    if (lane == 0) red[wv][c] = s;
   }
-}
-
-__global__ void colnorms_kernel(int N, const double* __restrict__ W, int64_t ld, double* out) {
-  __shared__ double red[256];
-  const double* w = W + ld * blockIdx.x;
-  double s = 0.0;
-  for (int i = threadIdx.x; i < N; i += blockDim.x) s += w[i] * w[i];
-  red[threadIdx.x] = s;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[blockIdx.x] = sqrt(red[0]);
+  if (tid < bs) part[(int64_t)blockIdx.x * bs + tid] = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
 }
 
-// deterministic pseudo-random fill of columns [c0, c1) (values in [-0.5, 0.5))
+// ... and the residual norms nrm[c] = sqrt(sum of the partials in order)
+__global__ void residual_norms(int bs, int nparts, const double* __restrict__ part, double* __restrict__ nrm) {
+  const int c = threadIdx.x;
+  if (c >= bs) return;
+  double t = 0.0;
+  for (int w = 0; w < nparts; ++w) t += part[(int64_t)w * bs + c];
+  nrm[c] = sqrt(t);
+}
+
 __global__ void random_cols(int N, int c0, int c1, double* X, int64_t ld, unsigned seed) {
   int64_t total = (int64_t)N * (c1 - c0);
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
@@ -332,14 +338,20 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
   // iterations and then every kSyncEvery: in between, iterations are queued
   // without waiting, and at most kSyncEvery - 1 run past convergence
   constexpr int kSyncEvery = 4;
+  // the residual's partial sums share the Gram arena, which tsgram may grow
+  // (and so move) between iterations: the pointer is fetched at each launch
+  const int res_grid = (int)std::min<int64_t>(256, ceil_div(N, 256));
+  const size_t res_bytes = sizeof(double) * (size_t)res_grid * bs;
   for (int it = 0;; ++it) {
     double* S = Sset(cur);
     double* X = S, *P = S + colsz * bs, *W = S + colsz * 2 * bs;
     double* AS = S + colsz * 3 * bs;
     double* AX = AS, *AW = AS + colsz * 2 * bs;
-    hipLaunchKernelGGL(residual_kernel, dim3(grid_for((int64_t)N * bs)), dim3(256), 0, st, N, bs, X,
-                       AX, ld, theta, Wr);
-    hipLaunchKernelGGL(colnorms_kernel, dim3(nev), dim3(256), 0, st, N, Wr, ld, nrm);
+    double* res_part = (double*)arena(kArenaGram, res_bytes);
+    hipLaunchKernelGGL(residual_kernel, dim3(res_grid), dim3(256), 0, st, N, bs, X, AX, ld, theta, Wr,
+                       res_part);
+    hipLaunchKernelGGL(residual_norms, dim3(1), dim3(64), 0, st, bs, res_grid, res_part, nrm);
+    TTF_HIP(hipGetLastError());
     const bool at_cap = matvecs + bs > max_matvecs;
     if (it < 3 || it % kSyncEvery == 0 || at_cap) {
       TTF_HIP(hipMemcpyAsync(hbuf.data(), theta, sizeof(double) * (2 * kMaxM + 1), hipMemcpyDeviceToHost, st));
