@@ -369,6 +369,8 @@ def run_batch(args):
         crA, ra, n = mpo.xxz(d, J=J, delta=delta)
         crY, ry = mpo.random_block_tt(d, n, 4, 3, seed=1000 + c)
         probs.append(dict(n=n, ra=ra, crA=crA, crY0=crY, ry=ry, eps=1e-8, rmax=r))
+    if args.verb:
+        os.environ["TTF_PROGRESS"] = "1"   # a stderr line per finished chain
     for _ in range(args.warmup):
         ttfort.tt_eigb_batch(probs[:min(8, len(probs))], 3, nswp=20, nworkers=args.workers)
     t0 = time.perf_counter()

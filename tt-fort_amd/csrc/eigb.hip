@@ -18,6 +18,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <vector>
 
@@ -54,6 +55,17 @@ struct Timer {
 
 using namespace ttf;
 
+// Guard vectors of the local block eigensolver (block size B + guard):
+// TTF_EIG_GUARD overrides the default 3 (0..16).
+static int eig_guard_vectors() {
+  static const int v = [] {
+    const char* e = std::getenv("TTF_EIG_GUARD");
+    const int x = e ? std::atoi(e) : 3;
+    return std::min(std::max(x, 0), 16);
+  }();
+  return v;
+}
+
 extern "C" int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, const double* crA,
                            const double* crY0, int* ry, double eps, int rmax, double* lambda, int B,
                            int kickrank, int nswp, int max_full_size, int verb, double* result,
@@ -70,7 +82,8 @@ extern "C" int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, con
     const double eps2 = eps / std::sqrt((double)d);
     const double resid_damp = 10.0;      // tt_eigb.f90:154
     const int max_matvecs = 500;         // tt_eigb.f90:152
-    const int guard = 3;
+    // LOBPCG guard vectors beyond the B wanted (TTF_EIG_GUARD overrides)
+    const int guard = eig_guard_vectors();
     const int nswp0 = nswp > 0 ? nswp : 20;
     const int mfs = max_full_size > 0 ? max_full_size : 2000;
     // operator cores on the device

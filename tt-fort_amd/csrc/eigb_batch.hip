@@ -10,6 +10,9 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -44,7 +47,10 @@ extern "C" int ttf_tt_eigb_batch(int nchains, const int* d, const int* n_all, co
     auto cost = [&](int c) { return (double)d[c] * rmax[c] * rmax[c] * rmax[c]; };
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
     const int nw = std::max(1, std::min(nworkers > 0 ? nworkers : 8, nchains));
-    std::atomic<int> next{0};
+    // TTF_PROGRESS set: one line on stderr per finished chain
+    const bool progress = std::getenv("TTF_PROGRESS") != nullptr;
+    const auto t0 = std::chrono::steady_clock::now();
+    std::atomic<int> next{0}, done{0};
     std::vector<std::string> errs(nchains);
     auto work = [&]() {
       try {
@@ -63,6 +69,12 @@ extern "C" int ttf_tt_eigb_batch(int nchains, const int* d, const int* n_all, co
           if (status[c] != 0) {
             errs[c] = ttf_last_error();
             ttf_clear_error();
+          }
+          const int nd = done.fetch_add(1) + 1;
+          if (progress) {
+            const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            std::fprintf(stderr, "ttf_tt_eigb_batch: chain %d (d=%d, rmax=%d) %s, %d of %d done, %.1f s\n", c,
+                         d[c], rmax[c], status[c] == 0 ? "solved" : "failed", nd, nchains, sec);
           }
         }
       } catch (...) {

@@ -21,9 +21,10 @@ constexpr int kTsgRows = 64;
 
 // G (p x q) = X^T Y for tall X (N x p), Y (N x q) in two launches: every
 // work-group of tsgram_partial sums row chunks of 64 into its partial p x q
-// block, and tsgram_reduce adds the partials in work-group order (a fixed
-// order, so the result is reproducible), `sub` threads per output over
-// interleaved partials with their sub-sums added in order.
+// block, and tsgram_reduce adds the partials in a fixed order (so the
+// result is reproducible): the 16 lanes of a DPP row take interleaved
+// partials of one output, one batch of loads each, and a row sum combines
+// them.
 __global__ __launch_bounds__(256) void tsgram_partial(int N, int p, int q, const double* __restrict__ X,
                                                       int64_t ldx, const double* __restrict__ Y,
                                                       int64_t ldy, double* __restrict__ part) {
@@ -64,26 +65,19 @@ __global__ __launch_bounds__(256) void tsgram_partial(int N, int p, int q, const
   }
 }
 
-constexpr int kTsgSub = 4;   // threads per output in the reduction
+constexpr int kTsgSub = 16;   // lanes per output in the reduction (one DPP row)
 
 __global__ __launch_bounds__(256) void tsgram_reduce(int p, int q, int nparts, const double* __restrict__ part,
                                                      double* __restrict__ G, int ldg) {
-  __shared__ double red[256];
   const int pq = p * q, tid = threadIdx.x;
   const int e = blockIdx.x * (256 / kTsgSub) + tid / kTsgSub, k = tid % kTsgSub;
   double s = 0.0;
   if (e < pq) {
-#pragma unroll 16
+#pragma unroll 4
     for (int w = k; w < nparts; w += kTsgSub) s += part[(int64_t)w * pq + e];
   }
-  red[tid] = s;
-  __syncthreads();
-  if (k == 0 && e < pq) {
-    double t = red[tid];
-#pragma unroll
-    for (int j = 1; j < kTsgSub; ++j) t += red[tid + j];
-    G[(e % p) + (int64_t)ldg * (e / p)] = t;
-  }
+  s = row_sum16(s);
+  if (k == 0 && e < pq) G[(e % p) + (int64_t)ldg * (e / p)] = s;
 }
 
 void tsgram(int N, int p, int q, const double* X, int64_t ldx, const double* Y, int64_t ldy,

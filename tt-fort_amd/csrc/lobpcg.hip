@@ -203,13 +203,18 @@ __global__ __launch_bounds__(256) void residual_kernel(int N, int bs, const doub
   if (tid < bs) part[(int64_t)blockIdx.x * bs + tid] = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
 }
 
-// ... and the residual norms nrm[c] = sqrt(sum of the partials in order)
-__global__ void residual_norms(int bs, int nparts, const double* __restrict__ part, double* __restrict__ nrm) {
-  const int c = threadIdx.x;
-  if (c >= bs) return;
-  double t = 0.0;
-  for (int w = 0; w < nparts; ++w) t += part[(int64_t)w * bs + c];
-  nrm[c] = sqrt(t);
+// ... and the residual norms nrm[c] = sqrt(sum of the partials): the 16
+// lanes of a DPP row take interleaved partials of column c and a row sum
+// combines them (a fixed order).
+__global__ __launch_bounds__(256) void residual_norms(int bs, int nparts, const double* __restrict__ part,
+                                                      double* __restrict__ nrm) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int c = t / 16, k = t % 16;
+  double s = 0.0;
+  if (c < bs)
+    for (int w = k; w < nparts; w += 16) s += part[(int64_t)w * bs + c];
+  s = row_sum16(s);
+  if (k == 0 && c < bs) nrm[c] = sqrt(s);
 }
 
 __global__ void random_cols(int N, int c0, int c1, double* X, int64_t ld, unsigned seed) {
@@ -350,7 +355,8 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
     double* res_part = (double*)arena(kArenaGram, res_bytes);
     hipLaunchKernelGGL(residual_kernel, dim3(res_grid), dim3(256), 0, st, N, bs, X, AX, ld, theta, Wr,
                        res_part);
-    hipLaunchKernelGGL(residual_norms, dim3(1), dim3(64), 0, st, bs, res_grid, res_part, nrm);
+    hipLaunchKernelGGL(residual_norms, dim3((unsigned)ceil_div(16 * bs, 256)), dim3(256), 0, st, bs, res_grid,
+                       res_part, nrm);
     TTF_HIP(hipGetLastError());
     const bool at_cap = matvecs + bs > max_matvecs;
     if (it < 3 || it % kSyncEvery == 0 || at_cap) {
