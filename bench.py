@@ -355,6 +355,10 @@ def run_batch(args):
     PCG64(seed 5), 3 lowest eigenpairs each, tol 1e-8, random rank-4 start;
     all chains solved concurrently by ttf_tt_eigb_batch.  One step = the
     whole batch."""
+    if args.hw_queues:
+        # hardware queues the HIP runtime gives this process (read when it
+        # initialises): the worker streams share them
+        os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
     import ttfort
     import mpo
     lib = ttfort.load()
@@ -387,7 +391,8 @@ def run_batch(args):
                 vs_baseline=None, dtype="f64", data="synthetic",
                 config=dict(workload="tt_eigb batch: %d XXZ chains d~U{16..48} r~U{32..96} B=3 tol 1e-8"
                                      % args.chains,
-                            chains=args.chains, workers=args.workers, B=3, tol=1e-8,
+                            chains=args.chains, workers=args.workers,
+                            hw_queues=int(os.environ.get("GPU_MAX_HW_QUEUES", "4")), B=3, tol=1e-8,
                             sum_chain_seconds=round(chain_s, 2), concurrency=round(chain_s / wall, 2),
                             mean_sweeps=float(np.mean(sweeps)), max_rank=max(o[3]["max_rank"] for o in out),
                             matvecs=int(sum(o[3]["total_matvecs"] for o in out)), wall_s=round(wall, 2)))
@@ -401,6 +406,8 @@ def main():
     ap.add_argument("--workload", default="eigb", choices=["eigb", "matvec", "ksl", "amen", "batch"])
     ap.add_argument("--chains", type=int, default=256, help="batch: number of chains")
     ap.add_argument("--workers", type=int, default=16, help="batch: concurrent worker streams")
+    ap.add_argument("--hw-queues", type=int, default=None,
+                    help="batch: GPU_MAX_HW_QUEUES for this process (4..32; default: the runtime's)")
     ap.add_argument("--prec", default="c", help="AMEn local preconditioner: n, c, l, r, m (learned)")
     ap.add_argument("--mixed", action="store_true",
                     help="AMEn: fp32 Krylov matvecs with FP64 refinement in the local GMRES")
