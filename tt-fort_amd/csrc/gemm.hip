@@ -155,23 +155,33 @@ static Cfg heuristic(int M, int N, int K, int nbatch) {
   return c;
 }
 
+template <typename T, bool CT>
+static void launch_tiles(const GemmOp<T>& op, const Cfg& c, dim3 grid, hipStream_t st) {
+  if (c.bm == 64 && c.bk == 16)
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, 64, 64, 16, 4, CT>), grid, dim3(256), 0, st, op);
+  else if (c.bm == 64 && c.bk == 32)
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, 64, 64, 32, 4, CT>), grid, dim3(256), 0, st, op);
+  else if (c.bm == 64)
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, 64, 64, 64, 4, CT>), grid, dim3(256), 0, st, op);
+  else if (c.bk == 16)
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, 32, 32, 16, 1, CT>), grid, dim3(64), 0, st, op);
+  else if (c.bk == 32)
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, 32, 32, 32, 1, CT>), grid, dim3(64), 0, st, op);
+  else
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, 32, 32, 64, 1, CT>), grid, dim3(64), 0, st, op);
+}
+
 template <typename T>
 static void launch_cfg(GemmOp<T> op, const Cfg& c, hipStream_t st) {
   op.ksplit = c.ks;
   int64_t tiles = ceil_div(op.M, c.bm) * ceil_div(op.N, c.bm);
   dim3 grid((unsigned)tiles, (unsigned)op.nbatch, (unsigned)op.ksplit);
-  if (c.bm == 64 && c.bk == 16)
-    hipLaunchKernelGGL((gemm_mfma_kernel<T, 64, 64, 16>), grid, dim3(256), 0, st, op);
-  else if (c.bm == 64 && c.bk == 32)
-    hipLaunchKernelGGL((gemm_mfma_kernel<T, 64, 64, 32>), grid, dim3(256), 0, st, op);
-  else if (c.bm == 64)
-    hipLaunchKernelGGL((gemm_mfma_kernel<T, 64, 64, 64>), grid, dim3(256), 0, st, op);
-  else if (c.bk == 16)
-    hipLaunchKernelGGL((gemm_mfma_kernel<T, 32, 32, 16, 1>), grid, dim3(64), 0, st, op);
-  else if (c.bk == 32)
-    hipLaunchKernelGGL((gemm_mfma_kernel<T, 32, 32, 32, 1>), grid, dim3(64), 0, st, op);
+  // column-contiguous outputs (and the column-major split-K slices) take the
+  // swapped-fragment variant, whose lanes store consecutive rows
+  if (op.ksplit > 1 || op.sCi == 1)
+    launch_tiles<T, true>(op, c, grid, st);
   else
-    hipLaunchKernelGGL((gemm_mfma_kernel<T, 32, 32, 64, 1>), grid, dim3(64), 0, st, op);
+    launch_tiles<T, false>(op, c, grid, st);
   TTF_HIP(hipGetLastError());
   if (op.ksplit > 1) {
     int64_t total = (int64_t)op.M * op.N * op.nbatch;

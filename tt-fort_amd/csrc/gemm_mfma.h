@@ -79,7 +79,11 @@ __device__ __forceinline__ int xcd_remap(int wg, int total) {
 // lets a single wave own the whole tile.  Every wave keeps at least four
 // independent 16x16 accumulator chains (a 32x32 wave tile), since dependent
 // f64 MFMAs issue back to back only when several chains interleave.
-template <typename T, int BM, int BN, int BK, int NW = 4>
+// CT (column-contiguous output, op.sCi == 1): the fragments are multiplied
+// in swapped order, B^T A^T, so that each accumulator holds a tile of C^T:
+// lane l then owns row l%16 of the 16x16 block and consecutive lanes store
+// consecutive rows (contiguous in memory) instead of consecutive columns.
+template <typename T, int BM, int BN, int BK, int NW = 4, bool CT = false>
 __global__ __launch_bounds__(64 * NW) void gemm_mfma_kernel(GemmOp<T> op) {
   constexpr int NT = 64 * NW;
   constexpr int PAD = 2;
@@ -171,12 +175,13 @@ __global__ __launch_bounds__(64 * NW) void gemm_mfma_kernel(GemmOp<T> op) {
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < FN; ++j) acc[i][j] = mfma16(af[i], bf[j], acc[i][j]);
+        for (int j = 0; j < FN; ++j)
+          acc[i][j] = CT ? mfma16(bf[j], af[i], acc[i][j]) : mfma16(af[i], bf[j], acc[i][j]);
     }
   }
 
   // Epilogue: lane l, register r holds row mfma_row<T>(l, r), column l%16
-  // of each 16x16 block.
+  // of each 16x16 block (CT: row l%16, column mfma_row<T>(l, r)).
   if (op.ksplit == 1) {
     T* __restrict__ C = op.C + b0 * op.sCb0 + b1 * op.sCb1;
 #pragma unroll
@@ -185,8 +190,10 @@ __global__ __launch_bounds__(64 * NW) void gemm_mfma_kernel(GemmOp<T> op) {
       for (int j = 0; j < FN; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          int gi = tm * BM + wm * WM + i * 16 + mfma_row<T>(lane, r);
-          int gj = tn * BN + wn * WN + j * 16 + (lane & 15);
+          const int fr = CT ? (lane & 15) : mfma_row<T>(lane, r);
+          const int fc = CT ? mfma_row<T>(lane, r) : (lane & 15);
+          int gi = tm * BM + wm * WM + i * 16 + fr;
+          int gj = tn * BN + wn * WN + j * 16 + fc;
           if (gi < op.M && gj < op.N) {
             T* c = C + gi * op.sCi + gj * op.sCj;
             T v = op.alpha * acc[i][j][r];
@@ -202,8 +209,10 @@ __global__ __launch_bounds__(64 * NW) void gemm_mfma_kernel(GemmOp<T> op) {
       for (int j = 0; j < FN; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          int gi = tm * BM + wm * WM + i * 16 + mfma_row<T>(lane, r);
-          int gj = tn * BN + wn * WN + j * 16 + (lane & 15);
+          const int fr = CT ? (lane & 15) : mfma_row<T>(lane, r);
+          const int fc = CT ? mfma_row<T>(lane, r) : (lane & 15);
+          int gi = tm * BM + wm * WM + i * 16 + fr;
+          int gj = tn * BN + wn * WN + j * 16 + fc;
           if (gi < op.M && gj < op.N) W[gi + (int64_t)gj * op.M] = acc[i][j][r];
         }
   }
