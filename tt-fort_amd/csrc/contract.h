@@ -54,5 +54,9 @@ void phi2_right_chain(int rx1, int rx2, int ry1, int n, int ry2, const T* phi2, 
 
 void fullmat_launch(int r1, int nn, int r2, int ra1, int ra2, const double* phi1, const double* A,
                     const double* phi2, double* B, hipStream_t st);
+// Diagonal of the same local matrix: D[(c1, i, c2)] = sum over (a1, a2) of
+// phi1[c1, c1, a1] A[a1, i, i, a2] phi2[c2, a2, c2] (N = r1 nn r2 values).
+void localdiag_launch(int r1, int nn, int r2, int ra1, int ra2, const double* phi1, const double* A,
+                      const double* phi2, double* D, hipStream_t st);
 
 }  // namespace ttf

@@ -315,6 +315,32 @@ __global__ void fullmat_kernel(int r1, int nn, int r2, int ra1, int ra2, const d
   }
 }
 
+__global__ void localdiag_kernel(int r1, int nn, int r2, int ra1, int ra2, const double* __restrict__ phi1,
+                                 const double* __restrict__ A, const double* __restrict__ phi2,
+                                 double* __restrict__ D) {
+  const int64_t N = (int64_t)r1 * nn * r2;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < N; t += (int64_t)gridDim.x * blockDim.x) {
+    const int c1 = (int)(t % r1), i = (int)((t / r1) % nn), c2 = (int)(t / ((int64_t)r1 * nn));
+    double s = 0.0;
+    for (int a1 = 0; a1 < ra1; ++a1) {
+      const double p1 = phi1[c1 + (int64_t)r1 * (c1 + (int64_t)r1 * a1)];
+      for (int a2 = 0; a2 < ra2; ++a2)
+        s += p1 * A[a1 + ra1 * (i + (int64_t)nn * (i + (int64_t)nn * a2))] *
+             phi2[c2 + (int64_t)r2 * (a2 + (int64_t)ra2 * c2)];
+    }
+    D[t] = s;
+  }
+}
+
+void localdiag_launch(int r1, int nn, int r2, int ra1, int ra2, const double* phi1, const double* A,
+                      const double* phi2, double* D, hipStream_t st) {
+  const int64_t N = (int64_t)r1 * nn * r2;
+  const int blocks = (int)std::min<int64_t>(ceil_div(N, 256), 4096);
+  hipLaunchKernelGGL(localdiag_kernel, dim3(std::max(blocks, 1)), dim3(256), 0, st, r1, nn, r2, ra1, ra2, phi1,
+                     A, phi2, D);
+  TTF_HIP(hipGetLastError());
+}
+
 void fullmat_launch(int r1, int nn, int r2, int ra1, int ra2, const double* phi1, const double* A,
                     const double* phi2, double* B, hipStream_t st) {
   const int64_t N = (int64_t)r1 * nn * r2;

@@ -20,7 +20,9 @@
 #include "jacobi.h"
 #include "wave.h"
 
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 namespace ttf {
@@ -180,20 +182,32 @@ __global__ __launch_bounds__(256) void rr_kernel(int m, int bs, int nx, const do
 
 // W = AX - X diag(theta); each work-group takes a range of rows and stores
 // its per-column sums of squares to part (nparts x bs) ...
+// With D (the local matrix's diagonal), W holds the preconditioned residual
+// (D - theta_c)^-1 r_c (Davidson's diagonal preconditioner; |D - theta_c|
+// clamped below at delta (1 + |theta_c|)), while the sums of squares, and so
+// the convergence test, are of the plain residual.
 __global__ __launch_bounds__(256) void residual_kernel(int N, int bs, const double* __restrict__ X,
                                                        const double* __restrict__ AX, int64_t ld,
                                                        const double* __restrict__ theta, double* __restrict__ W,
-                                                       double* __restrict__ part) {
+                                                       double* __restrict__ part, const double* __restrict__ D,
+                                                       double delta) {
   __shared__ double red[4][kMaxM];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int per = (N + gridDim.x - 1) / gridDim.x;
   const int r0 = blockIdx.x * per, r1 = min(N, r0 + per);
   for (int c = 0; c < bs; ++c) {
     const double th = theta[c];
+    const double floor_c = delta * (1.0 + fabs(th));
     double s = 0.0;
     for (int i = r0 + tid; i < r1; i += 256) {
       const double w = AX[i + ld * c] - th * X[i + ld * c];
-      W[i + ld * c] = w;
+      if (D) {
+        double den = D[i] - th;
+        if (fabs(den) < floor_c) den = den >= 0.0 ? floor_c : -floor_c;
+        W[i + ld * c] = w / den;
+      } else {
+        W[i + ld * c] = w;
+      }
       s += w * w;
     }
     s = wave_sum64(s);
@@ -246,6 +260,18 @@ void mul_small(int N, int p, int q, const double* S, int64_t lds, const double* 
 }
 
 }  // namespace
+
+// Davidson diagonal preconditioning of the LOBPCG search directions, with
+// the relative clamp delta = 1e-2 (TTF_EIG_PREC overrides; 0 leaves the
+// directions unpreconditioned).  It changes the search directions only: the
+// stopping rule and the Rayleigh-Ritz values are those of the plain method.
+static double eig_prec_delta() {
+  static const double v = [] {
+    const char* e = std::getenv("TTF_EIG_PREC");
+    return e ? std::max(0.0, std::atof(e)) : 1e-2;
+  }();
+  return v;
+}
 
 EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* lambda, double tol,
                    int max_matvecs, int dense_max, hipStream_t st) {
@@ -345,6 +371,13 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
   constexpr int kSyncEvery = 4;
   // the residual's partial sums share the Gram arena, which tsgram may grow
   // (and so move) between iterations: the pointer is fetched at each launch
+  // the optional diagonal preconditioner (TTF_EIG_PREC = clamp delta > 0)
+  const double prec_delta = eig_prec_delta();
+  double* Dg = nullptr;
+  if (prec_delta > 0.0) {
+    Dg = (double*)arena(kArenaSolver2, sizeof(double) * (size_t)N);
+    localdiag_launch(op.s.ry1, op.s.n, op.s.ry2, op.s.ra1, op.s.ra2, op.phi1, op.A, op.phi2, Dg, st);
+  }
   const int res_grid = (int)std::min<int64_t>(256, ceil_div(N, 256));
   const size_t res_bytes = sizeof(double) * (size_t)res_grid * bs;
   for (int it = 0;; ++it) {
@@ -354,7 +387,7 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
     double* AX = AS, *AW = AS + colsz * 2 * bs;
     double* res_part = (double*)arena(kArenaGram, res_bytes);
     hipLaunchKernelGGL(residual_kernel, dim3(res_grid), dim3(256), 0, st, N, bs, X, AX, ld, theta, Wr,
-                       res_part);
+                       res_part, Dg, prec_delta);
     hipLaunchKernelGGL(residual_norms, dim3((unsigned)ceil_div(16 * bs, 256)), dim3(256), 0, st, bs, res_grid,
                        res_part, nrm);
     TTF_HIP(hipGetLastError());
