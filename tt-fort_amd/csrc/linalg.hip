@@ -448,8 +448,13 @@ __global__ __launch_bounds__(256) void svd_pair_kernel(int mw, int nw, int bsz, 
 constexpr int kQrPanel = kSvdMaxP;
 
 __global__ __launch_bounds__(256) void qr_panel_kernel(int rows, int pc, double* __restrict__ X, int64_t ld,
-                                                       double* __restrict__ Vb, double* __restrict__ Yb) {
+                                                       double* __restrict__ Vb, double* __restrict__ Yb,
+                                                       int64_t sX, int64_t sVY) {
   extern __shared__ double P[];   // panel column c at P + c * lp
+  // blockIdx.y: the matrix of a batch (offsets sX into X, sVY into Vb, Yb)
+  X += blockIdx.y * sX;
+  Vb += blockIdx.y * sVY;
+  Yb += blockIdx.y * sVY;
   __shared__ double T[kQrPanel][kQrPanel + 1];
   __shared__ double S[kQrPanel][kQrPanel + 1];
   __shared__ double tau[kQrPanel], red[16];
@@ -886,36 +891,131 @@ int sorted_norms(int mw, int k, const double* X, double small2, std::vector<doub
   return kpos;
 }
 
-// R factor of the QR factorisation of W (p x q, ld p, p >= q, overwritten):
-// R is left in the upper triangle of W(0:q, 0:q).
-void qr_r_factor(int p, int q, double* W, hipStream_t st) {
+// R factors of nb matrices W_b = W + b sW (p x q, ld p, p >= q, overwritten):
+// R_b is left in the upper triangle of W_b(0:q, 0:q).  Panels of pc <= 32
+// columns (as many as fit the LDS budget with p rows) in one launch for the
+// whole batch, trailing columns updated by two batched GEMMs.
+void qr_r_factor_batched(int p, int q, double* W, int nb, int64_t sW, hipStream_t st) {
   const int lp = p | 1;
   const int pcmax = std::min(kQrPanel, kSvdLdsDoubles / lp);
   TTF_REQUIRE(pcmax >= 1, "qr: %d rows exceed the panel LDS budget", p);
+  TTF_REQUIRE(nb >= 1 && nb <= 65535, "qr: batch of %d", nb);
   static std::atomic<bool> attr_set{false};   // concurrent first calls only repeat the call
   if (!attr_set) {
     TTF_HIP(hipFuncSetAttribute((const void*)qr_panel_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)(sizeof(double) * kSvdLdsDoubles)));
     attr_set = true;
   }
-  double* Vb = (double*)arena(kArenaTrunc3, sizeof(double) * ((size_t)2 * p * kQrPanel + (size_t)kQrPanel * q));
-  double* Yb = Vb + (size_t)p * kQrPanel;
-  double* Z = Yb + (size_t)p * kQrPanel;
+  const int64_t sVY = (int64_t)p * kQrPanel, sZ = (int64_t)kQrPanel * q;
+  double* Vb = (double*)arena(kArenaTrunc3, sizeof(double) * (size_t)nb * (2 * sVY + sZ));
+  double* Yb = Vb + (size_t)nb * sVY;
+  double* Z = Yb + (size_t)nb * sVY;
   for (int k0 = 0; k0 < q; k0 += pcmax) {
     const int rows = p - k0, pc = std::min(pcmax, q - k0);
     const size_t lds = sizeof(double) * (size_t)pc * (rows | 1);
     double* Xp = W + k0 + (int64_t)p * k0;
-    hipLaunchKernelGGL(qr_panel_kernel, dim3(1), dim3(256), lds, st, rows, pc, Xp, (int64_t)p, Vb, Yb);
+    hipLaunchKernelGGL(qr_panel_kernel, dim3(1, nb), dim3(256), lds, st, rows, pc, Xp, (int64_t)p, Vb, Yb, sW,
+                       sVY);
     TTF_HIP(hipGetLastError());
     const int nrest = q - k0 - pc;
     if (nrest > 0) {
       double* Cm = W + k0 + (int64_t)p * (k0 + pc);
       GemmOp<double> g1 = colmajor<double>('T', 'N', pc, nrest, rows, 1.0, Vb, rows, Cm, p, 0.0, Z, pc);
+      g1.nb0 = g1.nbatch = nb;
+      g1.sAb0 = sVY; g1.sBb0 = sW; g1.sCb0 = sZ;
       gemm_launch(g1, st);
       GemmOp<double> g2 = colmajor<double>('N', 'N', rows, nrest, pc, -1.0, Yb, rows, Z, pc, 1.0, Cm, p);
+      g2.nb0 = g2.nbatch = nb;
+      g2.sAb0 = sVY; g2.sBb0 = sZ; g2.sCb0 = sW;
       gemm_launch(g2, st);
     }
   }
+}
+
+// Row chunks of X (rows x q, ld ldx) into nb consecutive h x q matrices
+// (ld h), the last one padded with zero rows.
+__global__ void tsqr_chunks_kernel(int rows, int q, const double* __restrict__ X, int64_t ldx, int h, int nb,
+                                   double* __restrict__ C) {
+  const int64_t total = (int64_t)nb * h * q;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % h), j = (int)((e / h) % q), b = (int)(e / ((int64_t)h * q));
+    const int r = b * h + i;
+    C[e] = r < rows ? X[r + ldx * j] : 0.0;
+  }
+}
+
+// The chunks' R factors (upper q x q of each h x q chunk) stacked into S
+// ((nb q) x q, ld nb q), zeros below their diagonals.
+__global__ void tsqr_stack_kernel(int q, int h, int nb, const double* __restrict__ C, double* __restrict__ S) {
+  const int64_t ls = (int64_t)nb * q, total = ls * q;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e % ls), j = (int)(e / ls), b = r / q, i = r % q;
+    S[e] = i <= j ? C[(int64_t)b * h * q + i + (int64_t)h * j] : 0.0;
+  }
+}
+
+// Upper q x q triangle of S (ld lds) into W (ld ldw).
+__global__ void tsqr_copy_r_kernel(int q, const double* __restrict__ S, int64_t lds, double* __restrict__ W,
+                                   int64_t ldw) {
+  const int64_t total = (int64_t)q * q;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % q), j = (int)(e / q);
+    if (i <= j) W[i + ldw * j] = S[i + lds * j];
+  }
+}
+
+// R factor of the QR factorisation of W (p x q, ld p, p >= q, overwritten):
+// R is left in the upper triangle of W(0:q, 0:q).  Matrices whose direct
+// factorisation would take more than four panels (rows too long for wide
+// LDS panels), with q <= 128, take the tall-skinny tree: row chunks of
+// h = min(4q, 256) >= 2q rows are factorised as one batch, their R factors
+// stacked (at most half as many rows) and the stack factorised again, until
+// it is at most 2h rows.  Each level is Householder QR (backward stable);
+// the R obtained differs from the direct one only by the signs of its rows.
+void qr_r_factor(int p, int q, double* W, hipStream_t st) {
+  const int h = std::max(q, std::min(4 * q, 256));
+  const int pc_direct = std::max(1, std::min(kQrPanel, kSvdLdsDoubles / (p | 1)));
+  if (q > 128 || p <= 2 * h || ceil_div(q, pc_direct) <= 4) {
+    qr_r_factor_batched(p, q, W, 1, 0, st);
+    return;
+  }
+  // workspace for all levels: chunks (<= (rows + h) q) and stacks (<= rows q / 2)
+  size_t need = 0;
+  for (int64_t rows = p; rows > 2 * h;) {
+    const int64_t nb = ceil_div(rows, h);
+    need += (size_t)(nb * h * q + nb * q * q);
+    rows = nb * q;
+  }
+  double* ws = (double*)arena(kArenaTsqr, sizeof(double) * need);
+  const double* cur = W;
+  int64_t rows = p, ldc = p;
+  while (rows > 2 * h) {
+    const int nb = (int)ceil_div(rows, h);
+    double* C = ws;
+    ws += (size_t)nb * h * q;
+    const int64_t tot = (int64_t)nb * h * q;
+    hipLaunchKernelGGL(tsqr_chunks_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(tot, 256), 8192)), dim3(256),
+                       0, st, (int)rows, q, cur, ldc, h, nb, C);
+    TTF_HIP(hipGetLastError());
+    qr_r_factor_batched(h, q, C, nb, (int64_t)h * q, st);
+    double* S = ws;
+    ws += (size_t)nb * q * q;
+    const int64_t tots = (int64_t)nb * q * q;
+    hipLaunchKernelGGL(tsqr_stack_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(tots, 256), 8192)), dim3(256),
+                       0, st, q, h, nb, C, S);
+    TTF_HIP(hipGetLastError());
+    cur = S;
+    rows = (int64_t)nb * q;
+    ldc = rows;
+  }
+  double* last = const_cast<double*>(cur);
+  qr_r_factor_batched((int)rows, q, last, 1, 0, st);
+  hipLaunchKernelGGL(tsqr_copy_r_kernel, dim3((unsigned)std::min<int64_t>(ceil_div((int64_t)q * q, 256), 4096)),
+                     dim3(256), 0, st, q, last, ldc, W, (int64_t)p);
+  TTF_HIP(hipGetLastError());
 }
 
 }  // namespace

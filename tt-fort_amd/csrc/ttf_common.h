@@ -101,6 +101,7 @@ enum Arena {
   kArenaSvdLong,     // long-side singular vectors: Cholesky QR workspace
   kArenaSvdLong2,    // long-side singular vectors: short-side vectors
   kArenaMixed,       // fp32 operator copy and vector staging of the mixed path
+  kArenaTsqr,        // tall-skinny QR: row chunks and stacked R factors
   kArenaCount
 };
 void* arena(Arena a, size_t bytes);
