@@ -16,6 +16,7 @@
 //     completion of rank-deficient blocks;
 //   * the S-step (zsfun_matvec, tt_ksl.f90:130-136) by two complex GEMMs.
 #include "solvers.h"
+#include "wave.h"
 
 #include <algorithm>
 #include <atomic>
@@ -151,16 +152,24 @@ __global__ __launch_bounds__(256) void zqr_gs_kernel(int rows, int cols, int rne
     for (int w = 0; w < (nt >> 6); ++w) t += red[w];
     return t;
   };
+  // a -= Q Q^H a over the first b basis columns, twice.  The b <= 64 dot
+  // products take a quad of lanes each (every lane a quarter of the rows,
+  // then a quad sum in registers), so the 256 threads share them.
   auto orthogonalise = [&](zc* a, int b) {
     for (int pass = 0; pass < 2; ++pass) {
-      for (int i = tid; i < b; i += nt) {
-        const zc* qi = L + (size_t)i * rows;
+      {
+        const int i = tid >> 2, part = tid & 3;
         double re = 0.0, im = 0.0;
-        for (int r = 0; r < rows; ++r) {
-          re += qi[r].x * a[r].x + qi[r].y * a[r].y;
-          im += qi[r].x * a[r].y - qi[r].y * a[r].x;
+        if (i < b) {
+          const zc* qi = L + (size_t)i * rows;
+          for (int r = part; r < rows; r += 4) {
+            re += qi[r].x * a[r].x + qi[r].y * a[r].y;
+            im += qi[r].x * a[r].y - qi[r].y * a[r].x;
+          }
         }
-        h[i] = make_hipDoubleComplex(re, im);
+        re = quad_sum(re);
+        im = quad_sum(im);
+        if (i < b && part == 0) h[i] = make_hipDoubleComplex(re, im);
       }
       __syncthreads();
       for (int r = tid; r < rows; r += nt) {

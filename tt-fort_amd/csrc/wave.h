@@ -32,6 +32,14 @@ __device__ __forceinline__ double row_sum16(double x) {
   return x;
 }
 
+// Sum over the 4 lanes of the caller's quad (lanes 4k .. 4k + 3, all
+// active); all four end with the same value.
+__device__ __forceinline__ double quad_sum(double x) {
+  x += dpp_f64<kDppXor1>(x);
+  x += dpp_f64<kDppXor2>(x);
+  return x;
+}
+
 __device__ __forceinline__ double readlane_f64(double x, int lane) {
   const long long v = __double_as_longlong(x);
   const int lo = __builtin_amdgcn_readlane((int)v, lane);
