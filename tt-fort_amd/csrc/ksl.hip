@@ -95,6 +95,15 @@ __global__ void zscale_kernel(int64_t n, zc a, const zc* __restrict__ x, zc* __r
     y[e] = zmul(a, x[e]);
 }
 
+// u /= sqrt(nrm2[0].x), or u = 0 when that norm is at most tol (Arnoldi
+// breakdown): the normalisation of a Krylov step without a host round trip.
+__global__ void zscale_by_norm_kernel(int64_t n, const zc* __restrict__ nrm2, double tol, zc* __restrict__ u) {
+  const double h = sqrt(fmax(nrm2[0].x, 0.0));
+  const double inv = h > tol ? 1.0 / h : 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    u[i] = make_hipDoubleComplex(u[i].x * inv, u[i].y * inv);
+}
+
 void zscale(int64_t n, zc a, const zc* x, zc* y, hipStream_t st) {
   hipLaunchKernelGGL(zscale_kernel, dim3(grid_for(n)), dim3(256), 0, st, n, a, x, y);
   TTF_HIP(hipGetLastError());
@@ -496,7 +505,11 @@ ExpInfo zgexpv(int64_t n, int m, double t, const zc* v, zc* w, double tol, doubl
   const int mh = m + 2;
   TTF_REQUIRE(mh <= kExpmMax, "zexp_mv: Krylov order %d exceeds %d", m, kExpmMax - 2);
   const double sgn = t >= 0 ? 1.0 : -1.0, t_out = std::fabs(t);
-  zc* V = (zc*)arena(kArenaSolver, sizeof(zc) * (size_t)n * (m + 2));
+  // Krylov basis (m + 2 vectors), then per Arnoldi step its dot products
+  // (two passes of 64) and squared norm, read back every kArnoldiSync steps
+  constexpr int kArnoldiSync = 8;
+  zc* V = (zc*)arena(kArenaSolver, sizeof(zc) * ((size_t)n * (m + 2) + (size_t)129 * m));
+  zc* Hs = V + (size_t)n * (m + 2);
   zc* sm = (zc*)arena(kArenaSmall, sizeof(zc) * (size_t)(3 * kExpmMax * kExpmMax + 256));
   zc* Hd = sm;
   zc* Ed = Hd + kExpmMax * kExpmMax;
@@ -521,32 +534,50 @@ ExpInfo zgexpv(int64_t n, int m, double t, const zc* v, zc* w, double tol, doubl
     int k1 = 2, mbrkdwn = m;
     double avnorm = 0.0;
     bool brk = false;
-    for (int j = 1; j <= m; ++j) {
+    // Arnoldi steps queued without waiting: each step's coefficients stay on
+    // the device and the normalisation is done there (a vector below the
+    // breakdown tolerance becomes zero, so steps queued past a breakdown
+    // only produce zeros); the host reads the columns every kArnoldiSync
+    // steps and takes them in order, stopping at the first breakdown as the
+    // step-by-step loop does.
+    int j_done = 0;
+    for (int j = 1; j <= m && !brk; ++j) {
       zc* u = V + (size_t)n * j;
       op(V + (size_t)n * (j - 1), u);
       ++info.nmult;
+      zc* hs = Hs + (size_t)129 * (j - 1);
       // Gram-Schmidt twice against V(:, 0..j-1)
       for (int pass = 0; pass < 2; ++pass) {
-        zc* h = hb + 64 * pass;
+        zc* h = hs + 64 * pass;
         zmdot(n, j, V, n, u, h, st);
         zgemm_cm('N', 'N', (int)n, 1, j, make_hipDoubleComplex(-1.0, 0.0), V, n, h, j,
                  make_hipDoubleComplex(1.0, 0.0), u, n, st);
       }
-      zmdot(n, 1, u, n, u, hb + 128, st);
-      TTF_HIP(hipMemcpyAsync(hh.data(), hb, sizeof(zc) * 129, hipMemcpyDeviceToHost, st));
+      zmdot(n, 1, u, n, u, hs + 128, st);
+      hipLaunchKernelGGL(zscale_by_norm_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(n, 256), 4096)), dim3(256),
+                         0, st, n, hs + 128, break_tol, u);
+      TTF_HIP(hipGetLastError());
+      if (j % kArnoldiSync != 0 && j != m) continue;
+      const int cnt = j - j_done;
+      hh.resize((size_t)129 * cnt);
+      TTF_HIP(hipMemcpyAsync(hh.data(), Hs + (size_t)129 * j_done, sizeof(zc) * 129 * cnt, hipMemcpyDeviceToHost,
+                             st));
       TTF_HIP(hipStreamSynchronize(st));
-      for (int i = 0; i < j; ++i)
-        H[(size_t)(j - 1) * mh + i] = make_hipDoubleComplex(hh[i].x + hh[64 + i].x, hh[i].y + hh[64 + i].y);
-      const double hj1j = std::sqrt(std::max(hh[128].x, 0.0));
-      if (hj1j <= break_tol) {   // happy breakdown
-        k1 = 0;
-        mbrkdwn = j;
-        t_step = t_out - t_now;
-        brk = true;
-        break;
+      for (int jj = j_done + 1; jj <= j; ++jj) {
+        const zc* c = hh.data() + (size_t)129 * (jj - 1 - j_done);
+        for (int i = 0; i < jj; ++i)
+          H[(size_t)(jj - 1) * mh + i] = make_hipDoubleComplex(c[i].x + c[64 + i].x, c[i].y + c[64 + i].y);
+        const double hj1j = std::sqrt(std::max(c[128].x, 0.0));
+        if (hj1j <= break_tol) {   // happy breakdown
+          k1 = 0;
+          mbrkdwn = jj;
+          t_step = t_out - t_now;
+          brk = true;
+          break;
+        }
+        H[(size_t)(jj - 1) * mh + jj] = make_hipDoubleComplex(hj1j, 0.0);
       }
-      H[(size_t)(j - 1) * mh + j] = make_hipDoubleComplex(hj1j, 0.0);
-      zscale(n, make_hipDoubleComplex(1.0 / hj1j, 0.0), u, u, st);
+      j_done = j;
     }
     if (!brk) {
       zc* u = V + (size_t)n * (m + 1);
