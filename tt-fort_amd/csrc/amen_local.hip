@@ -26,6 +26,16 @@ __global__ void scale_copy_kernel(int64_t n, double alpha, const double* __restr
     y[e] = alpha * x[e];
 }
 
+// out = u / sqrt(nrm2[0]), or 0 when that norm is 0: the normalisation of
+// an Arnoldi step without a host round trip.
+__global__ void scale_by_norm_kernel(int64_t N, const double* __restrict__ nrm2, const double* __restrict__ u,
+                                     double* __restrict__ out) {
+  const double h = sqrt(fmax(nrm2[0], 0.0));
+  const double inv = h > 0.0 ? 1.0 / h : 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = u[i] * inv;
+}
+
 // y = a x + b y
 __global__ void axpby_kernel(int64_t n, double a, const double* __restrict__ x, double b, double* y) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
@@ -314,10 +324,13 @@ GmresStats gmres_solve(const LocalOp& op, const JacPrec& P, const double* rhs, d
   GmresStats gs;
   const int64_t N = op.N();
   const int m = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)nrestart, N, (int64_t)63}));
-  // device workspace: V (N x (m+1)) | u | t
-  double* V = (double*)arena(kArenaSolver, sizeof(double) * (size_t)N * (m + 3));
+  // device workspace: V (N x (m+1)) | u | t | per Arnoldi step its dot
+  // products (two passes of 64) and squared norm, read every kGmresSync steps
+  constexpr int kGmresSync = 4;
+  double* V = (double*)arena(kArenaSolver, sizeof(double) * ((size_t)N * (m + 3) + (size_t)129 * m));
   double* u = V + (size_t)N * (m + 1);
   double* t = u + N;
+  double* Hs = t + N;
   double* hb = (double*)arena(kArenaSmall, sizeof(double) * (2 * 64 + 8));
   TTF_HIP(hipMemsetAsync(sol, 0, sizeof(double) * N, st));
   const int gN = grid_for(N);
@@ -356,42 +369,63 @@ GmresStats gmres_solve(const LocalOp& op, const JacPrec& P, const double* rhs, d
     hipLaunchKernelGGL(scale_copy_kernel, dim3(gN), dim3(256), 0, st, N, 1.0 / beta, V, V);
     std::fill(g.begin(), g.end(), 0.0);
     g[0] = beta;
-    int jj = 0;
-    for (int j = 0; j < m; ++j) {
+    // Arnoldi steps are queued without waiting: each step's coefficients
+    // stay on the device and the next basis vector is normalised there; the
+    // host reads the coefficients every kGmresSync steps and applies the
+    // Givens rotations in order, stopping at the first step that meets the
+    // tolerance (steps queued past it are not used).
+    int jj = 0, j_done = 0;
+    bool stop = false;
+    for (int j = 0; j < m && !stop; ++j) {
       apply_BP(V + (size_t)N * j, u, false);
       const int k = j + 1;
+      double* hs = Hs + (size_t)129 * j;
       // classical Gram-Schmidt, twice, against V(:, 0..j)
       for (int pass = 0; pass < 2; ++pass) {
-        double* h = hb + 64 * pass;
+        double* h = hs + 64 * pass;
         tsgram((int)N, k, 1, V, N, u, N, h, k, st);
         GemmOp<double> gm = colmajor<double>('N', 'N', (int)N, 1, k, -1.0, V, N, h, k, 1.0, u, N);
         gemm_launch(gm, st);
       }
-      tsgram((int)N, 1, 1, u, N, u, N, hb + 128, 1, st);
-      TTF_HIP(hipMemcpyAsync(hh.data(), hb, sizeof(double) * 129, hipMemcpyDeviceToHost, st));
-      TTF_HIP(hipStreamSynchronize(st));
-      double* col = &H[(size_t)(m + 1) * j];
-      for (int i = 0; i < k; ++i) col[i] = hh[i] + hh[64 + i];
-      const double hn = std::sqrt(std::max(hh[128], 0.0));
-      col[k] = hn;
-      for (int i = 0; i < j; ++i) {   // previous rotations
-        const double a = col[i], b = col[i + 1];
-        col[i] = cs[i] * a + sn[i] * b;
-        col[i + 1] = -sn[i] * a + cs[i] * b;
-      }
-      const double r = std::hypot(col[j], col[k]);
-      cs[j] = r > 0.0 ? col[j] / r : 1.0;
-      sn[j] = r > 0.0 ? col[k] / r : 0.0;
-      col[j] = r;
-      col[k] = 0.0;
-      g[k] = -sn[j] * g[j];
-      g[j] = cs[j] * g[j];
-      curres = std::fabs(g[k]) / nrmrhs;
-      jj = k;
-      if (curres < tol || hn == 0.0) break;
-      if (j + 1 < m)
-        hipLaunchKernelGGL(scale_copy_kernel, dim3(gN), dim3(256), 0, st, N, 1.0 / hn, u,
+      tsgram((int)N, 1, 1, u, N, u, N, hs + 128, 1, st);
+      if (j + 1 < m) {
+        hipLaunchKernelGGL(scale_by_norm_kernel, dim3(gN), dim3(256), 0, st, N, hs + 128, u,
                            V + (size_t)N * (j + 1));
+        TTF_HIP(hipGetLastError());
+      }
+      if ((j + 1) % kGmresSync != 0 && j + 1 < m) continue;
+      const int cnt = j + 1 - j_done;
+      hh.resize((size_t)129 * cnt);
+      TTF_HIP(hipMemcpyAsync(hh.data(), Hs + (size_t)129 * j_done, sizeof(double) * 129 * cnt,
+                             hipMemcpyDeviceToHost, st));
+      TTF_HIP(hipStreamSynchronize(st));
+      for (int jc = j_done; jc <= j; ++jc) {
+        const double* c = hh.data() + (size_t)129 * (jc - j_done);
+        const int kc = jc + 1;
+        double* col = &H[(size_t)(m + 1) * jc];
+        for (int i = 0; i < kc; ++i) col[i] = c[i] + c[64 + i];
+        const double hn = std::sqrt(std::max(c[128], 0.0));
+        col[kc] = hn;
+        for (int i = 0; i < jc; ++i) {   // previous rotations
+          const double a = col[i], b = col[i + 1];
+          col[i] = cs[i] * a + sn[i] * b;
+          col[i + 1] = -sn[i] * a + cs[i] * b;
+        }
+        const double r = std::hypot(col[jc], col[kc]);
+        cs[jc] = r > 0.0 ? col[jc] / r : 1.0;
+        sn[jc] = r > 0.0 ? col[kc] / r : 0.0;
+        col[jc] = r;
+        col[kc] = 0.0;
+        g[kc] = -sn[jc] * g[jc];
+        g[jc] = cs[jc] * g[jc];
+        curres = std::fabs(g[kc]) / nrmrhs;
+        jj = kc;
+        if (curres < tol || hn == 0.0) {
+          stop = true;
+          break;
+        }
+      }
+      j_done = j + 1;
     }
     gs.j = jj;
     // y = R^-1 g (upper triangular jj x jj), sol += V(:, 0..jj-1) y
