@@ -8,6 +8,7 @@
 //   * the half contractions of the local operator used to build the kick
 //     block (tt_linalg.f90:210-256, dbfun3_right / dbfun3_left).
 #include "amen.h"
+#include "arnoldi_gs.h"
 #include "mlp.h"
 
 #include <algorithm>
@@ -380,18 +381,23 @@ GmresStats gmres_solve(const LocalOp& op, const JacPrec& P, const double* rhs, d
       apply_BP(V + (size_t)N * j, u, false);
       const int k = j + 1;
       double* hs = Hs + (size_t)129 * j;
-      // classical Gram-Schmidt, twice, against V(:, 0..j)
-      for (int pass = 0; pass < 2; ++pass) {
-        double* h = hs + 64 * pass;
-        tsgram((int)N, k, 1, V, N, u, N, h, k, st);
-        GemmOp<double> gm = colmajor<double>('N', 'N', (int)N, 1, k, -1.0, V, N, h, k, 1.0, u, N);
-        gemm_launch(gm, st);
-      }
-      tsgram((int)N, 1, 1, u, N, u, N, hs + 128, 1, st);
-      if (j + 1 < m) {
-        hipLaunchKernelGGL(scale_by_norm_kernel, dim3(gN), dim3(256), 0, st, N, hs + 128, u,
-                           V + (size_t)N * (j + 1));
-        TTF_HIP(hipGetLastError());
+      // classical Gram-Schmidt, twice, against V(:, 0..j), and the next
+      // basis vector u / |u|
+      if (gs::fits(N, k)) {
+        gs::step<double>(N, k, V, N, u, hs, 0.0, j + 1 < m ? V + (size_t)N * (j + 1) : nullptr, st);
+      } else {
+        for (int pass = 0; pass < 2; ++pass) {
+          double* h = hs + 64 * pass;
+          tsgram((int)N, k, 1, V, N, u, N, h, k, st);
+          GemmOp<double> gm = colmajor<double>('N', 'N', (int)N, 1, k, -1.0, V, N, h, k, 1.0, u, N);
+          gemm_launch(gm, st);
+        }
+        tsgram((int)N, 1, 1, u, N, u, N, hs + 128, 1, st);
+        if (j + 1 < m) {
+          hipLaunchKernelGGL(scale_by_norm_kernel, dim3(gN), dim3(256), 0, st, N, hs + 128, u,
+                             V + (size_t)N * (j + 1));
+          TTF_HIP(hipGetLastError());
+        }
       }
       if ((j + 1) % kGmresSync != 0 && j + 1 < m) continue;
       const int cnt = j + 1 - j_done;
@@ -485,13 +491,19 @@ GmresStats gmres_flex(const LocalOp& op, const PrecFn& M, const double* rhs, dou
       else op.apply(1, zj, N, u, N, st);
       ++gs.matvecs;
       const int k = j + 1;
-      for (int pass = 0; pass < 2; ++pass) {
-        double* h = hb + 64 * pass;
-        tsgram((int)N, k, 1, V, N, u, N, h, k, st);
-        GemmOp<double> gm = colmajor<double>('N', 'N', (int)N, 1, k, -1.0, V, N, h, k, 1.0, u, N);
-        gemm_launch(gm, st);
+      // the fused path also writes the next basis vector u / |u|
+      const bool fused = gs::fits(N, k);
+      if (fused) {
+        gs::step<double>(N, k, V, N, u, hb, 0.0, j + 1 < m ? V + (size_t)N * (j + 1) : nullptr, st);
+      } else {
+        for (int pass = 0; pass < 2; ++pass) {
+          double* h = hb + 64 * pass;
+          tsgram((int)N, k, 1, V, N, u, N, h, k, st);
+          GemmOp<double> gm = colmajor<double>('N', 'N', (int)N, 1, k, -1.0, V, N, h, k, 1.0, u, N);
+          gemm_launch(gm, st);
+        }
+        tsgram((int)N, 1, 1, u, N, u, N, hb + 128, 1, st);
       }
-      tsgram((int)N, 1, 1, u, N, u, N, hb + 128, 1, st);
       TTF_HIP(hipMemcpyAsync(hh.data(), hb, sizeof(double) * 129, hipMemcpyDeviceToHost, st));
       TTF_HIP(hipStreamSynchronize(st));
       double* col = &H[(size_t)(m + 1) * j];
@@ -513,7 +525,7 @@ GmresStats gmres_flex(const LocalOp& op, const PrecFn& M, const double* rhs, dou
       curres = std::fabs(g[k]) / nrmrhs;
       jj = k;
       if (!(curres >= tol) || hn == 0.0) break;   // also stops on a NaN from the preconditioner
-      if (j + 1 < m)
+      if (j + 1 < m && !fused)
         hipLaunchKernelGGL(scale_copy_kernel, dim3(gN), dim3(256), 0, st, N, 1.0 / hn, u, V + (size_t)N * (j + 1));
     }
     gs.j = jj;

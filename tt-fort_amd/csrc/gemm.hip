@@ -295,8 +295,12 @@ void zgemm_launch(ZGemmOp op, hipStream_t st) {
   Cfg c = heuristic(op.M, op.N, op.K, op.nbatch);
   int64_t tiles = ceil_div(op.M, c.bm) * ceil_div(op.N, c.bm);
   dim3 grid((unsigned)tiles, (unsigned)op.nbatch, 1);
+  // 32x32 tiles come with few work-groups, each one's K loop latency-bound
+  // on its global loads: a 32-deep K chunk halves the trips
   if (c.bm == 64)
     hipLaunchKernelGGL((zgemm_mfma_kernel<64, 64>), grid, dim3(256), 0, st, op);
+  else if (op.K > 16)
+    hipLaunchKernelGGL((zgemm_mfma_kernel<32, 32, 32>), grid, dim3(256), 0, st, op);
   else
     hipLaunchKernelGGL((zgemm_mfma_kernel<32, 32>), grid, dim3(256), 0, st, op);
   TTF_HIP(hipGetLastError());

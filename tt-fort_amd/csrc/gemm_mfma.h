@@ -257,9 +257,9 @@ struct ZGemmOp {
   hipDoubleComplex alpha, beta;
 };
 
-template <int BM, int BN>
+template <int BM, int BN, int BK = 16>
 __global__ __launch_bounds__(256) void zgemm_mfma_kernel(ZGemmOp op) {
-  constexpr int BK = 16;
+  static_assert(BM * BK % 256 == 0 && BN * BK % 256 == 0 && BK % 4 == 0, "zgemm tile");
   constexpr int PAD = 2;
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int FM = WM / 16, FN = WN / 16;

@@ -16,6 +16,7 @@
 //     completion of rank-deficient blocks;
 //   * the S-step (zsfun_matvec, tt_ksl.f90:130-136) by two complex GEMMs.
 #include "solvers.h"
+#include "arnoldi_gs.h"
 #include "wave.h"
 
 #include <algorithm>
@@ -546,17 +547,21 @@ ExpInfo zgexpv(int64_t n, int m, double t, const zc* v, zc* w, double tol, doubl
       op(V + (size_t)n * (j - 1), u);
       ++info.nmult;
       zc* hs = Hs + (size_t)129 * (j - 1);
-      // Gram-Schmidt twice against V(:, 0..j-1)
-      for (int pass = 0; pass < 2; ++pass) {
-        zc* h = hs + 64 * pass;
-        zmdot(n, j, V, n, u, h, st);
-        zgemm_cm('N', 'N', (int)n, 1, j, make_hipDoubleComplex(-1.0, 0.0), V, n, h, j,
-                 make_hipDoubleComplex(1.0, 0.0), u, n, st);
+      // Gram-Schmidt twice against V(:, 0..j-1), then u / |u|
+      if (gs::fits(n, j)) {
+        gs::step<zc>(n, j, V, n, u, hs, break_tol, u, st);
+      } else {
+        for (int pass = 0; pass < 2; ++pass) {
+          zc* h = hs + 64 * pass;
+          zmdot(n, j, V, n, u, h, st);
+          zgemm_cm('N', 'N', (int)n, 1, j, make_hipDoubleComplex(-1.0, 0.0), V, n, h, j,
+                   make_hipDoubleComplex(1.0, 0.0), u, n, st);
+        }
+        zmdot(n, 1, u, n, u, hs + 128, st);
+        hipLaunchKernelGGL(zscale_by_norm_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(n, 256), 4096)),
+                           dim3(256), 0, st, n, hs + 128, break_tol, u);
+        TTF_HIP(hipGetLastError());
       }
-      zmdot(n, 1, u, n, u, hs + 128, st);
-      hipLaunchKernelGGL(zscale_by_norm_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(n, 256), 4096)), dim3(256),
-                         0, st, n, hs + 128, break_tol, u);
-      TTF_HIP(hipGetLastError());
       if (j % kArnoldiSync != 0 && j != m) continue;
       const int cnt = j - j_done;
       hh.resize((size_t)129 * cnt);
