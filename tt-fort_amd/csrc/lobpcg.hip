@@ -9,8 +9,9 @@
 //
 // Per iteration: residuals W = AX - X diag(theta); convergence test
 // ||r_i|| <= tol * anorm on the host (the only host synchronisation);
-// W orthogonalised twice against [X P] and orthonormalised by SVQB
-// (eigen-decomposition of W^T W, near-null directions dropped); one block
+// W orthogonalised twice against [X P] and orthonormalised from W^T W by a
+// pivoted Cholesky factorisation (or SVQB, its eigen-decomposition; near-null
+// directions dropped either way); one block
 // matvec AW; Rayleigh-Ritz on S^T A S in a single work-group (cyclic Jacobi
 // in LDS), which also builds the next search direction P in the small
 // space, orthogonal to the new Ritz vectors; then [X P] <- S [C Z].
@@ -23,6 +24,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 namespace ttf {
@@ -68,6 +70,116 @@ __global__ __launch_bounds__(256) void svqb_kernel(int k, const double* __restri
     double l = D[j][j];
     mask[j] = (l > 1e-14 * lmax && lmax > 0.0) ? 1 : 0;
   }
+}
+
+// The same contract by pivoted Cholesky, in one wave: with the unit-diagonal
+// A = D^-1/2 G D^-1/2 (D = diag G), P^T A P = L L^T with the largest
+// remaining diagonal as pivot, stopped at the first pivot below 1e-14 (rank
+// r); T = D^-1/2 P L^-T on the r kept directions, so that W = Wr T has
+// T^T G T = I, and zero columns (mask 0) after them.  W spans what SVQB's
+// W spans, to rounding, and the Rayleigh-Ritz step depends on the span
+// only; the factorisation takes k short steps where the eigen-decomposition
+// takes several Jacobi sweeps.
+constexpr int kMaxB = 32;   // block columns handled by cholqb_kernel
+__global__ __launch_bounds__(64) void cholqb_kernel(int k, const double* __restrict__ G, double* __restrict__ T,
+                                                    int* __restrict__ mask) {
+  __shared__ double A[kMaxB][kMaxB + 1];   // Schur complements (permuted order)
+  __shared__ double L[kMaxB][kMaxB + 1];   // Cholesky factor (permuted order)
+  __shared__ double sc[kMaxB], lc[kMaxB];
+  __shared__ int perm[kMaxB];
+  __shared__ int piv, rank_;
+  const int lane = threadIdx.x;
+  for (int i = lane; i < k; i += 64) {
+    const double g = G[i + k * i];
+    sc[i] = g > 0.0 ? 1.0 / sqrt(g) : 0.0;
+    perm[i] = i;
+  }
+  if (lane == 0) rank_ = k;
+  __syncthreads();
+  for (int e = lane; e < k * k; e += 64) {
+    const int i = e % k, j = e / k;
+    A[i][j] = 0.5 * (G[i + k * j] + G[j + k * i]) * sc[i] * sc[j];
+    L[i][j] = 0.0;
+  }
+  __syncthreads();
+  for (int j = 0; j < k; ++j) {
+    if (lane == 0) {
+      int p = j;
+      double best = A[j][j];
+      for (int i = j + 1; i < k; ++i)
+        if (A[i][i] > best) { best = A[i][i]; p = i; }
+      if (!(best > 1e-14)) rank_ = j;   // also stops on NaN
+      piv = p;
+    }
+    __syncthreads();
+    if (rank_ <= j) break;   // the same shared value for every lane
+    const int p = piv;
+    if (p != j) {   // swap indices p and j: A's rows and columns, L's rows, perm
+      for (int i = lane; i < k; i += 64) {
+        const double t = A[i][p]; A[i][p] = A[i][j]; A[i][j] = t;
+      }
+      __syncthreads();
+      for (int i = lane; i < k; i += 64) {
+        const double t = A[p][i]; A[p][i] = A[j][i]; A[j][i] = t;
+      }
+      for (int c = lane; c < j; c += 64) {
+        const double t = L[p][c]; L[p][c] = L[j][c]; L[j][c] = t;
+      }
+      if (lane == 0) { const int t = perm[p]; perm[p] = perm[j]; perm[j] = t; }
+      __syncthreads();
+    }
+    const double ljj = sqrt(A[j][j]);
+    for (int i = j + lane; i < k; i += 64) {
+      const double l = i == j ? ljj : A[i][j] / ljj;
+      lc[i] = l;
+      L[i][j] = l;
+    }
+    __syncthreads();
+    const int t = k - j - 1;
+    for (int e = lane; e < t * t; e += 64) {
+      const int i = j + 1 + e % t, c = j + 1 + e / t;
+      A[i][c] -= lc[i] * lc[c];
+    }
+    __syncthreads();
+  }
+  const int r = rank_;
+  // column c of L^-T by back substitution (lane c), scattered to the
+  // original row order with the scaling
+  for (int c = lane; c < k; c += 64) {
+    if (c < r) {
+      A[c][c] = 1.0 / L[c][c];   // A's upper triangle now holds L^-T
+      for (int i = c - 1; i >= 0; --i) {
+        double s = 0.0;
+        for (int l = i + 1; l <= c; ++l) s += L[l][i] * A[l][c];
+        A[i][c] = -s / L[i][i];
+      }
+    }
+    for (int i = 0; i < k; ++i) {
+      const int o = perm[i];
+      T[o + k * c] = (c < r && i <= c) ? sc[o] * A[i][c] : 0.0;
+    }
+    mask[c] = c < r ? 1 : 0;
+  }
+}
+
+// Orthonormalisation of the residual block (TTF_EIG_ORTH: "chol", the
+// default, or "svqb").
+static bool eig_orth_svqb() {
+  static const bool v = [] {
+    const char* e = std::getenv("TTF_EIG_ORTH");
+    return e != nullptr && std::string(e) == "svqb";
+  }();
+  return v;
+}
+
+void orth_block(int k, const double* G, double* T, int* mask, hipStream_t st) {
+  if (eig_orth_svqb()) {
+    hipLaunchKernelGGL(svqb_kernel, dim3(1), dim3(256), 0, st, k, G, T, mask);
+  } else {
+    TTF_REQUIRE(k <= kMaxB, "block_eig: block of %d columns exceeds %d", k, kMaxB);
+    hipLaunchKernelGGL(cholqb_kernel, dim3(1), dim3(64), 0, st, k, G, T, mask);
+  }
+  TTF_HIP(hipGetLastError());
 }
 
 // Rayleigh-Ritz in the small space.  H (m x m) = S^T A S for the basis
@@ -336,7 +448,7 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
     TTF_HIP(hipGetLastError());
     for (int attempt = 0; attempt < 3; ++attempt) {
       tsgram(N, bs, bs, Wr, ld, Wr, ld, Gm, bs, st);
-      hipLaunchKernelGGL(svqb_kernel, dim3(1), dim3(256), 0, st, bs, Gm, Tm, maskW);
+      orth_block(bs, Gm, Tm, maskW, st);
       mul_small(N, bs, bs, Wr, ld, Tm, bs, X, ld, 0.0, st);
       std::vector<int> hm(bs);
       TTF_HIP(hipMemcpyAsync(hm.data(), maskW, sizeof(int) * bs, hipMemcpyDeviceToHost, st));
@@ -412,7 +524,7 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
     }
     // SVQB: W = Wr T, then A W
     tsgram(N, bs, bs, Wr, ld, Wr, ld, Gm, bs, st);
-    hipLaunchKernelGGL(svqb_kernel, dim3(1), dim3(256), 0, st, bs, Gm, Tm, maskW);
+    orth_block(bs, Gm, Tm, maskW, st);
     mul_small(N, bs, bs, Wr, ld, Tm, bs, W, ld, 0.0, st);
     op.apply(bs, W, ld, AW, ld, st);
     matvecs += bs;

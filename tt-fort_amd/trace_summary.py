@@ -21,6 +21,16 @@ n = len(gaps)
 print("gap us percentiles 50/75/90/99:", [round(gaps[int(n * q)] / 1e3, 2) for q in (0.5, 0.75, 0.9, 0.99)])
 print("gap total ms %.2f; gaps > 20us: %d totalling %.2f ms" % (sum(g for g in gaps if g > 0) / 1e6,
       sum(1 for g in gaps if g > 20000), sum(g for g in gaps if g > 20000) / 1e6))
+# idle gaps over 20 us by the pair of kernels around them (a host
+# synchronisation shows as a gap before the next launch)
+pairs = collections.defaultdict(list)
+for i in range(len(rows) - 1):
+    g = rows[i + 1][0] - rows[i][1]
+    if g > 20000:
+        pairs[(rows[i][2][:44], rows[i + 1][2][:44])].append(g)
+print("%-44s -> %-44s %7s %9s" % ("gap after", "before", "count", "total_ms"))
+for k, v in sorted(pairs.items(), key=lambda kv: -sum(kv[1]))[:12]:
+    print("%-44s -> %-44s %7d %9.2f" % (k[0], k[1], len(v), sum(v) / 1e6))
 agg = collections.defaultdict(list)
 for r in rows:
     agg[r[2]].append(r[1] - r[0])
