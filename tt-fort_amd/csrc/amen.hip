@@ -91,8 +91,7 @@ int grid_for(int64_t n) { return (int)std::min<int64_t>(std::max<int64_t>(ceil_d
 double dev_norm(int64_t N, const double* v, double* scratch, hipStream_t st) {
   tsgram((int)N, 1, 1, v, N, v, N, scratch, 1, st);
   double h = 0.0;
-  TTF_HIP(hipMemcpyAsync(&h, scratch, sizeof(double), hipMemcpyDeviceToHost, st));
-  TTF_HIP(hipStreamSynchronize(st));
+  read_back(&h, scratch, sizeof(double), st);
   return std::sqrt(std::max(h, 0.0));
 }
 
@@ -336,8 +335,7 @@ extern "C" int ttf_tt_amen_solve(int d, const int* n, const int* m, const int* r
             flops_total += bfun3_flops(op.s) * nc;
             hipLaunchKernelGGL(resid_norms_kernel, dim3(nc), dim3(256), 0, st, N, candy, N, rhs, scr);
             std::vector<double> hr(nc);
-            TTF_HIP(hipMemcpyAsync(hr.data(), scr, sizeof(double) * nc, hipMemcpyDeviceToHost, st));
-            TTF_HIP(hipStreamSynchronize(st));
+            read_back(hr.data(), scr, sizeof(double) * nc, st);
             for (int t = 0; t < nc; ++t) {
               const double e = std::sqrt(std::max(hr[t], 0.0)) / norm_rhs;
               if (e > res_new * 2.0 && e > eps2) {
@@ -414,8 +412,7 @@ extern "C" int ttf_tt_amen_solve(int d, const int* n, const int* m, const int* r
             flops_total += bfun3_flops(op.s) * nc;
             hipLaunchKernelGGL(resid_norms_kernel, dim3(nc), dim3(256), 0, st, N, candy, N, rhs, scr);
             std::vector<double> hr(nc);
-            TTF_HIP(hipMemcpyAsync(hr.data(), scr, sizeof(double) * nc, hipMemcpyDeviceToHost, st));
-            TTF_HIP(hipStreamSynchronize(st));
+            read_back(hr.data(), scr, sizeof(double) * nc, st);
             for (int t = 0; t < nc; ++t) {
               const double e = std::sqrt(std::max(hr[t], 0.0)) / norm_rhs;
               if (e > res_new * 2.0 && e > eps2) {

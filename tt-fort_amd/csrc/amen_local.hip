@@ -247,8 +247,7 @@ void jac_gen(char type, int rx1, int n, int rx2, int ra1, int ra2, const double*
   hipLaunchKernelGGL(batched_inverse_kernel, dim3((unsigned)nblk), dim3(256), lds, st, bs, jacs, info);
   TTF_HIP(hipGetLastError());
   std::vector<int> hinfo(nblk);
-  TTF_HIP(hipMemcpyAsync(hinfo.data(), info, sizeof(int) * (size_t)nblk, hipMemcpyDeviceToHost, st));
-  TTF_HIP(hipStreamSynchronize(st));
+  read_back(hinfo.data(), info, sizeof(int) * (size_t)nblk, st);
   for (int64_t b = 0; b < nblk; ++b)
     TTF_REQUIRE(hinfo[b] == 0, "djac_gen: block %lld is singular (zero pivot %d)", (long long)b, hinfo[b]);
 }
@@ -362,8 +361,7 @@ GmresStats gmres_solve(const LocalOp& op, const JacPrec& P, const double* rhs, d
       TTF_HIP(hipGetLastError());
     }
     tsgram((int)N, 1, 1, V, N, V, N, hb, 1, st);
-    TTF_HIP(hipMemcpyAsync(hh.data(), hb, sizeof(double), hipMemcpyDeviceToHost, st));
-    TTF_HIP(hipStreamSynchronize(st));
+    read_back(hh.data(), hb, sizeof(double), st);
     const double beta = std::sqrt(std::max(hh[0], 0.0));
     if (it == 0) nrmrhs = beta;
     if (beta == 0.0) { curres = 0.0; break; }
@@ -402,9 +400,7 @@ GmresStats gmres_solve(const LocalOp& op, const JacPrec& P, const double* rhs, d
       if ((j + 1) % kGmresSync != 0 && j + 1 < m) continue;
       const int cnt = j + 1 - j_done;
       hh.resize((size_t)129 * cnt);
-      TTF_HIP(hipMemcpyAsync(hh.data(), Hs + (size_t)129 * j_done, sizeof(double) * 129 * cnt,
-                             hipMemcpyDeviceToHost, st));
-      TTF_HIP(hipStreamSynchronize(st));
+      read_back(hh.data(), Hs + (size_t)129 * j_done, sizeof(double) * 129 * cnt, st);
       for (int jc = j_done; jc <= j; ++jc) {
         const double* c = hh.data() + (size_t)129 * (jc - j_done);
         const int kc = jc + 1;
@@ -475,8 +471,7 @@ GmresStats gmres_flex(const LocalOp& op, const PrecFn& M, const double* rhs, dou
       TTF_HIP(hipGetLastError());
     }
     tsgram((int)N, 1, 1, V, N, V, N, hb, 1, st);
-    TTF_HIP(hipMemcpyAsync(hh.data(), hb, sizeof(double), hipMemcpyDeviceToHost, st));
-    TTF_HIP(hipStreamSynchronize(st));
+    read_back(hh.data(), hb, sizeof(double), st);
     const double beta = std::sqrt(std::max(hh[0], 0.0));
     if (it == 0) nrmrhs = beta;
     if (!(beta > 0.0)) { curres = beta == 0.0 ? 0.0 : curres; break; }
@@ -504,8 +499,7 @@ GmresStats gmres_flex(const LocalOp& op, const PrecFn& M, const double* rhs, dou
         }
         tsgram((int)N, 1, 1, u, N, u, N, hb + 128, 1, st);
       }
-      TTF_HIP(hipMemcpyAsync(hh.data(), hb, sizeof(double) * 129, hipMemcpyDeviceToHost, st));
-      TTF_HIP(hipStreamSynchronize(st));
+      read_back(hh.data(), hb, sizeof(double) * 129, st);
       double* col = &H[(size_t)(m + 1) * j];
       for (int i = 0; i < k; ++i) col[i] = hh[i] + hh[64 + i];
       const double hn = std::sqrt(std::max(hh[128], 0.0));
@@ -581,8 +575,7 @@ int duchol(char trans, int m, int n, const double* dA, int numvec, double* du, d
   std::vector<double> hd(n), lam(numvec + 1, 0.0), hv(64 * 64), hw(64);
   int cnt = 0, real = 0;
   for (cnt = 0; cnt < numvec; ++cnt) {
-    TTF_HIP(hipMemcpyAsync(hd.data(), ddiag, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-    TTF_HIP(hipStreamSynchronize(st));
+    read_back(hd.data(), ddiag, sizeof(double) * n, st);
     int maxi = 0;
     double best = -1.0;
     for (int i = 0; i < n; ++i)
@@ -612,8 +605,7 @@ int duchol(char trans, int m, int n, const double* dA, int numvec, double* du, d
       gemm_launch(s, st);
     }
     double piv = 0.0;
-    TTF_HIP(hipMemcpyAsync(&piv, uc + maxi, sizeof(double), hipMemcpyDeviceToHost, st));
-    TTF_HIP(hipStreamSynchronize(st));
+    read_back(&piv, uc + maxi, sizeof(double), st);
     const double maxa = std::sqrt(std::fabs(piv));
     if (maxa < 1e-300) {   // exact zero remainder (tt_linalg.f90:604-607)
       real = cnt + 1;
@@ -634,8 +626,7 @@ int duchol(char trans, int m, int n, const double* dA, int numvec, double* du, d
     }
     tsgram(n, 1, 1, uc, n, uc, n, small + 128, 1, st);
     std::vector<double> hh(129);
-    TTF_HIP(hipMemcpyAsync(hh.data(), small, sizeof(double) * 129, hipMemcpyDeviceToHost, st));
-    TTF_HIP(hipStreamSynchronize(st));
+    read_back(hh.data(), small, sizeof(double) * 129, st);
     for (int j = 0; j < cnt; ++j) dx[j] = hh[j] + hh[64 + j];
     dx[cnt] = std::sqrt(std::max(hh[128], 0.0));
     if (dx[cnt] > 0.0)

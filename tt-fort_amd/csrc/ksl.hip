@@ -129,8 +129,7 @@ void zgemm_cm(char ta, char tb, int M, int N, int K, zc alpha, const zc* A, int6
 double znorm_dev(int64_t N, const zc* v, zc* scratch, hipStream_t st) {
   zmdot(N, 1, v, N, v, scratch, st);
   zc h;
-  TTF_HIP(hipMemcpyAsync(&h, scratch, sizeof(zc), hipMemcpyDeviceToHost, st));
-  TTF_HIP(hipStreamSynchronize(st));
+  read_back(&h, scratch, sizeof(zc), st);
   return std::sqrt(std::max(h.x, 0.0));
 }
 
@@ -438,8 +437,7 @@ double hager_norm1(int64_t n, const ZOp& op, const ZOp& adj, hipStream_t st) {
   for (int it = 0; it < 5; ++it) {
     op(x, y);
     hipLaunchKernelGGL(zsign_abs_kernel, dim3(blocks), dim3(256), 0, st, n, y, y, part);
-    TTF_HIP(hipMemcpyAsync(hp.data(), part, sizeof(double) * blocks, hipMemcpyDeviceToHost, st));
-    TTF_HIP(hipStreamSynchronize(st));
+    read_back(hp.data(), part, sizeof(double) * blocks, st);
     double s = 0.0;
     for (int b = 0; b < blocks; ++b) s += hp[b];
     if (it > 0 && s <= est) break;
@@ -565,9 +563,7 @@ ExpInfo zgexpv(int64_t n, int m, double t, const zc* v, zc* w, double tol, doubl
       if (j % kArnoldiSync != 0 && j != m) continue;
       const int cnt = j - j_done;
       hh.resize((size_t)129 * cnt);
-      TTF_HIP(hipMemcpyAsync(hh.data(), Hs + (size_t)129 * j_done, sizeof(zc) * 129 * cnt, hipMemcpyDeviceToHost,
-                             st));
-      TTF_HIP(hipStreamSynchronize(st));
+      read_back(hh.data(), Hs + (size_t)129 * j_done, sizeof(zc) * 129 * cnt, st);
       for (int jj = j_done + 1; jj <= j; ++jj) {
         const zc* c = hh.data() + (size_t)129 * (jj - 1 - j_done);
         for (int i = 0; i < jj; ++i)
@@ -601,8 +597,7 @@ ExpInfo zgexpv(int64_t n, int m, double t, const zc* v, zc* w, double tol, doubl
       TTF_HIP(hipMemcpyAsync(Hd, Hc.data(), sizeof(zc) * mx * mx, hipMemcpyHostToDevice, st));
       zexpm_small(mx, sgn * t_step, Hd, Ed, st);
       ++info.nexph;
-      TTF_HIP(hipMemcpyAsync(E.data(), Ed, sizeof(zc) * mx, hipMemcpyDeviceToHost, st));   // first column
-      TTF_HIP(hipStreamSynchronize(st));
+      read_back(E.data(), Ed, sizeof(zc) * mx, st);   // first column
       if (k1 == 0) {
         err_loc = tol;
       } else {
@@ -652,8 +647,7 @@ ExpInfo zexp_mv(int64_t n, int order, double tau, const zc* v, zc* w, double tol
     op(tmp, tmp + 1);
     zc a, v0;
     TTF_HIP(hipMemcpyAsync(&a, tmp + 1, sizeof(zc), hipMemcpyDeviceToHost, st));
-    TTF_HIP(hipMemcpyAsync(&v0, v, sizeof(zc), hipMemcpyDeviceToHost, st));
-    TTF_HIP(hipStreamSynchronize(st));
+    read_back(&v0, v, sizeof(zc), st);
     const double ex = std::exp(a.x * tau);
     const zc e = make_hipDoubleComplex(ex * std::cos(a.y * tau), ex * std::sin(a.y * tau));
     const zc r = make_hipDoubleComplex(v0.x * e.x - v0.y * e.y, v0.x * e.y + v0.y * e.x);

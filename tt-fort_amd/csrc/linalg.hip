@@ -752,8 +752,7 @@ bool cholqr2(int p, int k, double* U, int64_t ldu, hipStream_t st) {
     hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(256), lds, st, k, G, Ri, info);
     TTF_HIP(hipGetLastError());
     int hinfo = 0;
-    TTF_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, st));
-    TTF_HIP(hipStreamSynchronize(st));
+    read_back(&hinfo, info, sizeof(int), st);
     if (hinfo != 0) return false;
     gemm_launch(colmajor<double>('N', 'N', p, k, k, 1.0, U, ldu, Ri, k, 0.0, T, p), st);
     TTF_HIP(hipMemcpy2DAsync(U, sizeof(double) * ldu, T, sizeof(double) * p, sizeof(double) * p, k,
@@ -805,8 +804,7 @@ JacobiResult jacobi_columns(int mw, int nw, double* X, double* V, hipStream_t st
   // column norms bound the absolute noise floor of the cross products
   hipLaunchKernelGGL(colnorm_kernel, dim3(nw), dim3(256), 0, st, mw, nw, X, nrm);
   std::vector<double> hn(nw);
-  TTF_HIP(hipMemcpyAsync(hn.data(), nrm, sizeof(double) * nw, hipMemcpyDeviceToHost, st));
-  TTF_HIP(hipStreamSynchronize(st));
+  read_back(hn.data(), nrm, sizeof(double) * nw, st);
   double fro2 = 0.0;
   for (double v : hn) fro2 += v * v;
   if (!(fro2 > 0.0) || nw < 2) return res;
@@ -855,8 +853,7 @@ JacobiResult jacobi_columns(int mw, int nw, double* X, double* V, hipStream_t st
                          inner, offmax);
       TTF_HIP(hipGetLastError());
     }
-    TTF_HIP(hipMemcpyAsync(&hoff, offmax, sizeof(hoff), hipMemcpyDeviceToHost, st));
-    TTF_HIP(hipStreamSynchronize(st));
+    read_back(&hoff, offmax, sizeof(hoff), st);
     double off;
     std::memcpy(&off, &hoff, sizeof(off));
     if (off <= tol || off <= kQuadStop) break;
@@ -872,8 +869,7 @@ int sorted_norms(int mw, int k, const double* X, double small2, std::vector<doub
   double* nrm = (double*)arena(kArenaSmall, sizeof(double) * (size_t)k);
   hipLaunchKernelGGL(colnorm_kernel, dim3(k), dim3(256), 0, st, mw, k, X, nrm);
   std::vector<double> hn(k);
-  TTF_HIP(hipMemcpyAsync(hn.data(), nrm, sizeof(double) * k, hipMemcpyDeviceToHost, st));
-  TTF_HIP(hipStreamSynchronize(st));
+  read_back(hn.data(), nrm, sizeof(double) * k, st);
   std::vector<int> order(k);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return hn[a] > hn[b]; });
@@ -1117,8 +1113,7 @@ int svd_vectors(int m, int n, const double* A, int64_t lda, bool want_left, doub
     double* VTf = Uf + (size_t)m * q;
     gesvd(m, n, A, lda, Uf, m, S, VTf, q, st, 0);
     hs.assign(q, 0.0);
-    TTF_HIP(hipMemcpyAsync(hs.data(), S, sizeof(double) * q, hipMemcpyDeviceToHost, st));
-    TTF_HIP(hipStreamSynchronize(st));
+    read_back(hs.data(), S, sizeof(double) * q, st);
     if (want_left) {
       TTF_HIP(hipMemcpy2DAsync(O, sizeof(double) * ldo, Uf, sizeof(double) * m, sizeof(double) * m, q,
                                hipMemcpyDeviceToDevice, st));

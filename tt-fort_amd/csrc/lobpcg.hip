@@ -399,8 +399,7 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
     syev_small(N, 1, B, w, Q, 0, st);
     copy_cols(Q, N, Xout, N, N, nev, st);
     std::vector<double> hw(N);
-    TTF_HIP(hipMemcpyAsync(hw.data(), w, sizeof(double) * N, hipMemcpyDeviceToHost, st));
-    TTF_HIP(hipStreamSynchronize(st));
+    read_back(hw.data(), w, sizeof(double) * N, st);
     for (int i = 0; i < nev; ++i) lambda[i] = hw[i];
     stats.dense = 1;
     stats.anorm = std::max(std::fabs(hw[0]), std::fabs(hw[N - 1]));
@@ -451,8 +450,7 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
       orth_block(bs, Gm, Tm, maskW, st);
       mul_small(N, bs, bs, Wr, ld, Tm, bs, X, ld, 0.0, st);
       std::vector<int> hm(bs);
-      TTF_HIP(hipMemcpyAsync(hm.data(), maskW, sizeof(int) * bs, hipMemcpyDeviceToHost, st));
-      TTF_HIP(hipStreamSynchronize(st));
+      read_back(hm.data(), maskW, sizeof(int) * bs, st);
       bool bad = false;
       for (int i = 0; i < bs; ++i) bad = bad || !hm[i];
       if (!bad) break;
@@ -505,8 +503,7 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
     TTF_HIP(hipGetLastError());
     const bool at_cap = matvecs + bs > max_matvecs;
     if (it < 3 || it % kSyncEvery == 0 || at_cap) {
-      TTF_HIP(hipMemcpyAsync(hbuf.data(), theta, sizeof(double) * (2 * kMaxM + 1), hipMemcpyDeviceToHost, st));
-      TTF_HIP(hipStreamSynchronize(st));
+      read_back(hbuf.data(), theta, sizeof(double) * (2 * kMaxM + 1), st);
       anorm = std::max(anorm, hbuf[2 * kMaxM]);   // amax follows nrm in the buffer
       double res2 = 0.0, rmax = 0.0;
       for (int i = 0; i < nev; ++i) {

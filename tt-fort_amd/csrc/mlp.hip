@@ -468,8 +468,7 @@ int ttf_mlp_get_params(float* params) {
   return guarded([&] {
     TTF_REQUIRE(g_model.ready, "ttf_mlp_get_params: no parameters set");
     hipStream_t st = stream();
-    TTF_HIP(hipMemcpyAsync(params, g_model.master, sizeof(float) * (size_t)kMlpParams, hipMemcpyDeviceToHost, st));
-    TTF_HIP(hipStreamSynchronize(st));
+    read_back(params, g_model.master, sizeof(float) * (size_t)kMlpParams, st);
   });
 }
 
@@ -494,8 +493,7 @@ int ttf_mlp_loss_grad(int M, const float* X, const float* T, double* loss, float
     double* out = part + 256;
     batch_loss_grad(M, X, T, w, g, part, out, st);
     TTF_HIP(hipMemcpyAsync(loss, out, sizeof(double), hipMemcpyDeviceToHost, st));
-    if (grad) TTF_HIP(hipMemcpyAsync(grad, g, sizeof(float) * (size_t)kMlpParams, hipMemcpyDeviceToHost, st));
-    TTF_HIP(hipStreamSynchronize(st));
+    if (grad) read_back(grad, g, sizeof(float) * (size_t)kMlpParams, st);
   });
 }
 
@@ -537,8 +535,7 @@ int ttf_mlp_train(long nsamples, const float* X, const float* T, int epochs, int
                            g, g_model.am, g_model.av, lr, beta1, beta2, eps, c1, c2, g_model.wbf, g_model.wr);
         TTF_HIP(hipGetLastError());
       }
-      TTF_HIP(hipMemcpyAsync(hl.data(), losses, sizeof(double) * (size_t)nbat, hipMemcpyDeviceToHost, st));
-      TTF_HIP(hipStreamSynchronize(st));   // idx is rewritten by the next epoch
+      read_back(hl.data(), losses, sizeof(double) * (size_t)nbat, st);   // idx is rewritten by the next epoch
       if (loss_hist && hist_len && *hist_len < hist_cap) {
         double m = 0.0;
         for (double v : hl) m += v;

@@ -38,8 +38,7 @@ void dbfun3(const ttf_fint* rx1, const ttf_fint* m, const ttf_fint* rx2, const t
     TTF_HIP(hipMemcpyAsync(dphi2, phi2, n2 * sizeof(double), hipMemcpyHostToDevice, st));
     TTF_HIP(hipMemcpyAsync(dx, x, nx * sizeof(double), hipMemcpyHostToDevice, st));
     bfun3_chain<double>(s, 1, dphi1, dA, dphi2, dx, (int64_t)nx, dy, (int64_t)ny, st);
-    TTF_HIP(hipMemcpyAsync(y, dy, ny * sizeof(double), hipMemcpyDeviceToHost, st));
-    TTF_HIP(hipStreamSynchronize(st));
+    read_back(y, dy, ny * sizeof(double), st);
   });
 }
 

@@ -18,6 +18,9 @@ hipStream_t g_stream = nullptr;
 hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
 struct ArenaSlot { void* p = nullptr; size_t bytes = 0; };
 ArenaSlot g_arena[kArenaCount];
+// pinned host staging for read_back (allocated on first use)
+constexpr size_t kStageBytes = 256 * 1024;
+void* g_stage = nullptr;
 std::atomic<int> g_local_bits{64};
 
 // A worker's own stream and arenas (ScopedContext); null on threads that use
@@ -25,6 +28,7 @@ std::atomic<int> g_local_bits{64};
 struct Context {
   hipStream_t stream = nullptr;
   ArenaSlot arena[kArenaCount];
+  void* stage = nullptr;
 };
 thread_local Context* t_ctx = nullptr;
 }  // namespace
@@ -89,6 +93,23 @@ void* arena(Arena a, size_t bytes) {
   return s.p;
 }
 
+void read_back(void* host, const void* dev, size_t bytes, hipStream_t st) {
+  if (bytes == 0) {
+    TTF_HIP(hipStreamSynchronize(st));
+    return;
+  }
+  void** stage = t_ctx ? &t_ctx->stage : &g_stage;
+  if (bytes <= kStageBytes) {
+    if (*stage == nullptr) TTF_HIP(hipHostMalloc(stage, kStageBytes, hipHostMallocDefault));
+    TTF_HIP(hipMemcpyAsync(*stage, dev, bytes, hipMemcpyDeviceToHost, st));
+    TTF_HIP(hipStreamSynchronize(st));
+    std::memcpy(host, *stage, bytes);
+  } else {
+    TTF_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, st));
+    TTF_HIP(hipStreamSynchronize(st));
+  }
+}
+
 ScopedContext::ScopedContext() {
   ensure_init();
   TTF_REQUIRE(t_ctx == nullptr, "nested worker contexts");
@@ -108,6 +129,7 @@ ScopedContext::~ScopedContext() {
   (void)hipStreamSynchronize(c->stream);
   for (auto& a : c->arena)
     if (a.p) (void)hipFree(a.p);
+  if (c->stage) (void)hipHostFree(c->stage);
   (void)hipStreamDestroy(c->stream);
   if (t_ctx == c) t_ctx = nullptr;
   delete c;
@@ -136,6 +158,8 @@ int ttf_finalize(void) {
       s.p = nullptr;
       s.bytes = 0;
     }
+    if (g_stage) TTF_HIP(hipHostFree(g_stage));
+    g_stage = nullptr;
     TTF_HIP(hipEventDestroy(g_ev0));
     TTF_HIP(hipEventDestroy(g_ev1));
     TTF_HIP(hipStreamDestroy(g_stream));

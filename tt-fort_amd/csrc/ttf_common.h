@@ -106,6 +106,12 @@ enum Arena {
 };
 void* arena(Arena a, size_t bytes);
 
+// Copies `bytes` of device memory to host memory and waits for the stream,
+// through a pinned staging buffer of the calling thread's context: a copy
+// into pageable memory would be staged and waited for a second time inside
+// the runtime (about 23 us per round trip against 14 us).
+void read_back(void* host, const void* dev, size_t bytes, hipStream_t st);
+
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 }  // namespace ttf
