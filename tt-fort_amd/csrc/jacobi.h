@@ -80,12 +80,13 @@ constexpr double kJacobiSmall = 1e-9;
 // issued before its arithmetic so that one LDS round trip and the two
 // interleaved rotation computations make up the round.  The 2x2 blocks are
 // updated in full (annihilated entries keep their rounding-level values).
-// The loop ends after a sweep with no rotation above kJacobiSmall or after
-// max_sweeps.  `flags` is 3 ints of LDS.  All threads of the block call it;
-// the returned pointer (A or B) holds the diagonalised matrix.
+// The loop ends after a sweep with no rotation above `small` (default
+// kJacobiSmall) or after max_sweeps.  `flags` is 3 ints of LDS.  All threads
+// of the block call it; the returned pointer (A or B) holds the diagonalised
+// matrix.
 template <int LD>
 __device__ double (*jacobi_sym(double (*A)[LD], double (*B)[LD], double (*V)[LD], int n, int max_sweeps,
-                               double tol_abs, int* flags))[LD] {
+                               double tol_abs, int* flags, double small = kJacobiSmall))[LD] {
   const int tid = threadIdx.x, nt = blockDim.x;
   const int nn = n + (n & 1);
   const int half = nn / 2;
@@ -125,7 +126,7 @@ __device__ double (*jacobi_sym(double (*A)[LD], double (*B)[LD], double (*V)[LD]
         double c1, s1, c2, s2;
         jacobi_cs(apq, app, aqq, go1, c1, s1);
         jacobi_cs(auv, auu, avv, go2, c2, s2);
-        if (kq == 0 && fabs(s1) > kJacobiSmall) *moved = 1;
+        if (kq == 0 && fabs(s1) > small) *moved = 1;
         // columns u, v
         const double t0 = c2 * apu - s2 * apv, t1 = s2 * apu + c2 * apv;
         const double t2 = c2 * aqu - s2 * aqv, t3 = s2 * aqu + c2 * aqv;

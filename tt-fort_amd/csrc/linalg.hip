@@ -156,6 +156,7 @@ void syev_small(int n, int count, double* A, double* w, double* Q, int desc, hip
 // as zero and never rotated.
 constexpr int kSvdMaxP = 32;          // columns per pair (2b) at most
 constexpr int kSvdLdsDoubles = 16384; // LDS budget for the pair's columns
+static_assert((kSvdMaxP & (kSvdMaxP - 1)) == 0, "kSvdMaxP must be a power of two");
 
 __device__ inline void circle_pair(int r, int k, int nn, int* p, int* q) {
   int a, b;
@@ -215,13 +216,43 @@ __device__ inline void lds_gram(const double* P, int rows, int ld, int n, double
   const bool va = ci < n, vb = cj < n;
   const double* pa = P + (va ? ci : 0) * ld;
   const double* pb = P + (vb ? cj : 0) * ld;
-  d4_t acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = acc0;
-  for (int k0 = kbeg; k0 < kend; k0 += 8) {
+  d4_t acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = acc0, acc2 = acc0, acc3 = acc0;
+  int k0 = kbeg;
+  // 32 rows a step: all 16 loads in flight before the 8 MFMAs (columns past
+  // n read column 0 and are zeroed by a select, not a branch)
+  for (; k0 + 32 <= kend; k0 += 32) {
+    double a[8], b[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int r = k0 + 4 * u + kk;
+      a[u] = pa[r];
+      b[u] = pb[r];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      a[u] = va ? a[u] : 0.0;
+      b[u] = vb ? b[u] : 0.0;
+    }
+    acc0 = mfma16(a[0], b[0], acc0);
+    acc1 = mfma16(a[1], b[1], acc1);
+    acc2 = mfma16(a[2], b[2], acc2);
+    acc3 = mfma16(a[3], b[3], acc3);
+    acc0 = mfma16(a[4], b[4], acc0);
+    acc1 = mfma16(a[5], b[5], acc1);
+    acc2 = mfma16(a[6], b[6], acc2);
+    acc3 = mfma16(a[7], b[7], acc3);
+  }
+  for (; k0 < kend; k0 += 8) {
     const int r0 = k0 + kk, r1 = k0 + 4 + kk;
     const double a0 = (va && r0 < kend) ? pa[r0] : 0.0, b0 = (vb && r0 < kend) ? pb[r0] : 0.0;
     const double a1 = (va && r1 < kend) ? pa[r1] : 0.0, b1 = (vb && r1 < kend) ? pb[r1] : 0.0;
     acc0 = mfma16(a0, b0, acc0);
     acc1 = mfma16(a1, b1, acc1);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    acc0[r] += acc2[r];
+    acc1[r] += acc3[r];
   }
   if (T == 4) {
 #pragma unroll
@@ -240,30 +271,46 @@ __device__ inline void lds_gram(const double* P, int rows, int ld, int n, double
 }
 
 // Column block copies between global memory (column stride gld, columns
-// cols[0..n)) and LDS (column stride ld); each thread keeps 8 loads in flight.
+// cols[0..n)) and LDS (column stride ld), column by column with each
+// thread's rows: no division in the address arithmetic, and each thread
+// issues the loads of 16 columns before storing any.
 __device__ inline void cols_to_lds(const double* __restrict__ src, int64_t gld, const int* cols, int rows,
                                    int n, double* dst, int ld) {
-  const int total = rows * n, nt = blockDim.x;
-  for (int base = threadIdx.x; base < total; base += 8 * nt) {
-    double v[8];
+  const int nt = blockDim.x;
+  for (int c0 = 0; c0 < n; c0 += 16) {
+    for (int r = threadIdx.x; r < rows; r += nt) {
+      double v[16];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int e = base + u * nt;
-      v[u] = (e < total) ? src[(int64_t)(e % rows) + gld * cols[e / rows]] : 0.0;
-    }
+      for (int u = 0; u < 16; ++u) v[u] = (c0 + u < n) ? src[(int64_t)r + gld * cols[c0 + u]] : 0.0;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int e = base + u * nt;
-      if (e < total) dst[(e / rows) * ld + e % rows] = v[u];
+      for (int u = 0; u < 16; ++u)
+        if (c0 + u < n) dst[(c0 + u) * ld + r] = v[u];
     }
   }
 }
 
 __device__ inline void lds_to_cols(const double* src, int ld, int rows, int n, double* __restrict__ dst,
                                    int64_t gld, const int* cols) {
-  const int total = rows * n;
-  for (int e = threadIdx.x; e < total; e += blockDim.x)
-    dst[(int64_t)(e % rows) + gld * cols[e / rows]] = src[(e / rows) * ld + e % rows];
+  const int nt = blockDim.x;
+  for (int c0 = 0; c0 < n; c0 += 16) {
+    for (int r = threadIdx.x; r < rows; r += nt) {
+      double v[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) v[u] = (c0 + u < n) ? src[(c0 + u) * ld + r] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        if (c0 + u < n) dst[(int64_t)r + gld * cols[c0 + u]] = v[u];
+    }
+  }
+}
+
+// Maximum of a non-negative v over the work-group into *m (an LDS word
+// cleared beforehand, read after a barrier): wave maxima first, so that one
+// lane per wave does the atomic.
+__device__ inline void block_max_into(double v, unsigned long long* m) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  if ((threadIdx.x & 63) == 0) atomicMax(m, (unsigned long long)__double_as_longlong(v));
 }
 
 // A column pair's 2x2 Hestenes step inside the pair block: returns the
@@ -372,11 +419,11 @@ __global__ __launch_bounds__(256) void svd_pair_kernel(int mw, int nw, int bsz, 
   __syncthreads();
   {
     double cm = 0.0;
-    for (int e = tid; e < n * n; e += blockDim.x) {
-      const int i = e % n, j = e / n;
-      if (i < j) cm = fmax(cm, fabs(G[i][j]) * dinv[i] * dinv[j]);
+    for (int e = tid; e < kSvdMaxP * kSvdMaxP; e += blockDim.x) {
+      const int i = e % kSvdMaxP, j = e / kSvdMaxP;   // kSvdMaxP is a power of two
+      if (i < j && j < n) cm = fmax(cm, fabs(G[i][j]) * dinv[i] * dinv[j]);
     }
-    atomicMax(&cmax, (unsigned long long)__double_as_longlong(cm));
+    block_max_into(cm, &cmax);
   }
   __syncthreads();
   if (__longlong_as_double((long long)cmax) <= 0.5 * tol) return;
@@ -421,7 +468,7 @@ __global__ __launch_bounds__(256) void svd_pair_kernel(int mw, int nw, int bsz, 
       __syncthreads();
       if (stop) break;
     }
-    atomicMax(&rmax, (unsigned long long)__double_as_longlong(seen));
+    block_max_into(seen, &rmax);
   }
   __syncthreads();
   if (tid == 0) atomicMax(offmax, rmax);

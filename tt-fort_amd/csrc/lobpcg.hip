@@ -220,7 +220,11 @@ __global__ __launch_bounds__(256) void rr_kernel(int m, int bs, int nx, const do
   f2 = block_sum(f2, red);
   // Y is the second Jacobi buffer; the diagonal is copied out before Y is
   // reused for the sorted vectors
-  double(*D)[kMaxM + 1] = jacobi_sym<kMaxM + 1>(A, Y, V, m, 40, kEps * sqrt(f2), flags);
+  // A sweep whose rotations all have |sin| <= 1e-7 leaves off-diagonal
+  // entries of order 1e-14 ||H||: Ritz values within that of H's
+  // eigenvalues, far inside the solver's tolerance, and the next iteration's
+  // Rayleigh-Ritz starts again from the exact projection.
+  double(*D)[kMaxM + 1] = jacobi_sym<kMaxM + 1>(A, Y, V, m, 40, kEps * sqrt(f2), flags, 1e-7);
   for (int i = tid; i < m; i += blockDim.x) dg[i] = D[i][i];
   __syncthreads();
   // ascending order
