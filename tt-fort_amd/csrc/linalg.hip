@@ -206,7 +206,9 @@ __device__ inline void lds_apply_rotation(double* X, int rows, int ld, int n,
 // rows.  `scratch` holds 1024 doubles.  Ends with a block barrier.
 __device__ inline void lds_gram(const double* P, int rows, int ld, int n, double (*G)[kSvdMaxP + 1],
                                 double* scratch) {
+  // the first four waves compute; the others only take the barriers
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const bool comp = wv < 4;
   const int nt = (n + 15) >> 4, T = nt * nt, parts = 4 / T;
   const int tile = wv % T, part = wv / T;
   const int ti = tile % nt, tj = tile / nt;
@@ -217,7 +219,7 @@ __device__ inline void lds_gram(const double* P, int rows, int ld, int n, double
   const double* pa = P + (va ? ci : 0) * ld;
   const double* pb = P + (vb ? cj : 0) * ld;
   d4_t acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = acc0, acc2 = acc0, acc3 = acc0;
-  int k0 = kbeg;
+  int k0 = comp ? kbeg : kend;
   // 32 rows a step: all 16 loads in flight before the 8 MFMAs (columns past
   // n read column 0 and are zeroed by a select, not a branch)
   for (; k0 + 32 <= kend; k0 += 32) {
@@ -255,14 +257,16 @@ __device__ inline void lds_gram(const double* P, int rows, int ld, int n, double
     acc1[r] += acc3[r];
   }
   if (T == 4) {
+    if (comp)
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
-      G[ti * 16 + mfma_row<double>(lane, r)][tj * 16 + (lane & 15)] = acc0[r] + acc1[r];
+      for (int r = 0; r < 4; ++r)
+        G[ti * 16 + mfma_row<double>(lane, r)][tj * 16 + (lane & 15)] = acc0[r] + acc1[r];
     __syncthreads();
   } else {   // one tile, rows split over the 4 waves: partial tiles summed in order
+    if (comp)
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
-      scratch[part * 256 + mfma_row<double>(lane, r) * 16 + (lane & 15)] = acc0[r] + acc1[r];
+      for (int r = 0; r < 4; ++r)
+        scratch[part * 256 + mfma_row<double>(lane, r) * 16 + (lane & 15)] = acc0[r] + acc1[r];
     __syncthreads();
     for (int e = threadIdx.x; e < 256; e += blockDim.x)
       G[e >> 4][e & 15] = ((scratch[e] + scratch[256 + e]) + scratch[512 + e]) + scratch[768 + e];
@@ -319,15 +323,16 @@ __device__ inline void block_max_into(double v, unsigned long long* m) {
 // segment each) share a pair and reduce the dot products in registers; with
 // NR > 0 each thread keeps its NR elements per column in registers between
 // the dot products and the update (mw <= 16 NR).
-template <int NR>
+template <int NR, int LPP = 16>
 __device__ inline double hestenes_pair(double* ap, double* aq, int mw, int l16, double small2, double tol,
                                        double* c_out, double* s_out) {
+  static_assert(LPP == 16 || LPP == 32, "lanes per pair: 16 or 32");
   double al = 0.0, be = 0.0, ga = 0.0;
   double xr[NR > 0 ? NR : 1], yr[NR > 0 ? NR : 1];
   if constexpr (NR > 0) {
 #pragma unroll
     for (int t = 0; t < NR; ++t) {
-      const int i = l16 + 16 * t;
+      const int i = l16 + LPP * t;
       xr[t] = (i < mw) ? ap[i] : 0.0;
       yr[t] = (i < mw) ? aq[i] : 0.0;
     }
@@ -338,7 +343,7 @@ __device__ inline double hestenes_pair(double* ap, double* aq, int mw, int l16, 
       ga += xr[t] * yr[t];
     }
   } else {
-    for (int i = l16; i < mw; i += 16) {
+    for (int i = l16; i < mw; i += LPP) {
       const double x = ap[i], y = aq[i];
       al += x * x; be += y * y; ga += x * y;
     }
@@ -346,6 +351,11 @@ __device__ inline double hestenes_pair(double* ap, double* aq, int mw, int l16, 
   al = row_sum16(al);
   be = row_sum16(be);
   ga = row_sum16(ga);
+  if constexpr (LPP == 32) {   // the pair's two DPP rows: both get the same total
+    al += __shfl_xor(al, 16);
+    be += __shfl_xor(be, 16);
+    ga += __shfl_xor(ga, 16);
+  }
   *s_out = 0.0;
   if (!(al > small2 && be > small2)) return 0.0;
   // coupling |ga| / sqrt(al be) from the refined reciprocal square root
@@ -361,14 +371,14 @@ __device__ inline double hestenes_pair(double* ap, double* aq, int mw, int l16, 
   if constexpr (NR > 0) {
 #pragma unroll
     for (int t2 = 0; t2 < NR; ++t2) {
-      const int i = l16 + 16 * t2;
+      const int i = l16 + LPP * t2;
       if (i < mw) {
         ap[i] = c * xr[t2] - s * yr[t2];
         aq[i] = s * xr[t2] + c * yr[t2];
       }
     }
   } else {
-    for (int i = l16; i < mw; i += 16) {
+    for (int i = l16; i < mw; i += LPP) {
       const double x = ap[i], y = aq[i];
       ap[i] = c * x - s * y;
       aq[i] = s * x + c * y;
@@ -379,8 +389,8 @@ __device__ inline double hestenes_pair(double* ap, double* aq, int mw, int l16, 
   return rel;
 }
 
-template <int NR>
-__global__ __launch_bounds__(256) void svd_pair_kernel(int mw, int nw, int bsz, int nb, int round,
+template <int NR, int LPP = 16>
+__global__ __launch_bounds__(16 * LPP) void svd_pair_kernel(int mw, int nw, int bsz, int nb, int round,
                                                        double* __restrict__ W, double* __restrict__ V,
                                                        double small2, double tol, int inner_sweeps,
                                                        unsigned long long* offmax) {
@@ -427,7 +437,7 @@ __global__ __launch_bounds__(256) void svd_pair_kernel(int mw, int nw, int bsz, 
   }
   __syncthreads();
   if (__longlong_as_double((long long)cmax) <= 0.5 * tol) return;
-  // ---- 2. Hestenes rotations on the pair's columns; 16 groups of 16
+  // ---- 2. Hestenes rotations on the pair's columns; 16 groups of LPP
   // threads take one column pair each per round of the circle schedule
   for (int e = tid; e < kSvdMaxP * kSvdMaxP; e += blockDim.x) {
     const int j = e / kSvdMaxP, i = e % kSvdMaxP;
@@ -436,7 +446,7 @@ __global__ __launch_bounds__(256) void svd_pair_kernel(int mw, int nw, int bsz, 
   __syncthreads();
   {
     const int nn = n + (n & 1);
-    const int g = tid >> 4, l16 = tid & 15;
+    const int g = tid / LPP, l16 = tid % LPP;
     double seen = 0.0;
     for (int isw = 0; isw < inner_sweeps; ++isw) {
       if (tid == 0) any = 0;
@@ -447,12 +457,11 @@ __global__ __launch_bounds__(256) void svd_pair_kernel(int mw, int nw, int bsz, 
           circle_pair(r, g, nn, &p, &q);
           if (q < n) {
             double c, s;
-            const double rel = hestenes_pair<NR>(lds + p * ldp, lds + q * ldp, mw, l16, small2, tol, &c, &s);
+            const double rel = hestenes_pair<NR, LPP>(lds + p * ldp, lds + q * ldp, mw, l16, small2, tol, &c, &s);
             seen = fmax(seen, rel);
             if (s != 0.0) {
 #pragma unroll
-              for (int h = 0; h < 2; ++h) {
-                const int i = l16 + 16 * h;
+              for (int i = l16; i < kSvdMaxP; i += LPP) {
                 const double u = Qc[p][i], v = Qc[q][i];
                 Qc[p][i] = c * u - s * v;
                 Qc[q][i] = s * u + c * v;
@@ -820,6 +829,18 @@ static int svd_pair_columns() {
   return v;
 }
 
+// Lanes per column pair in the Jacobi SVD (TTF_SVD_LANES: 16 or 32, the
+// default): 32 halves each thread's rows and gives every SIMD two waves, so
+// that one wave's dependent steps overlap the other's (a whole wave per
+// pair, 64, measured slower).
+static int svd_pair_lanes() {
+  static const int v = [] {
+    const char* e = std::getenv("TTF_SVD_LANES");
+    return (e != nullptr && std::atoi(e) == 16) ? 16 : 32;
+  }();
+  return v;
+}
+
 // Hestenes sweeps per pair and round (TTF_SVD_INNER overrides; default 1).
 static int svd_inner_sweeps() {
   static int v = [] {
@@ -876,10 +897,15 @@ JacobiResult jacobi_columns(int mw, int nw, double* X, double* V, hipStream_t st
   const int nr = (int)ceil_div(mw, 16);
   auto kern = nr <= 4 ? svd_pair_kernel<4> : nr <= 8 ? svd_pair_kernel<8>
             : nr <= 16 ? svd_pair_kernel<16> : nr <= 32 ? svd_pair_kernel<32> : svd_pair_kernel<0>;
+  int threads = 256;
+  if (nr <= 16 && svd_pair_lanes() == 32) {
+    kern = nr <= 4 ? svd_pair_kernel<2, 32> : nr <= 8 ? svd_pair_kernel<4, 32> : svd_pair_kernel<8, 32>;
+    threads = 512;
+  }
   static std::atomic<bool> attr_set{false};   // concurrent first calls only repeat the call
   if (!attr_set) {
     for (auto f : {svd_pair_kernel<4>, svd_pair_kernel<8>, svd_pair_kernel<16>, svd_pair_kernel<32>,
-                   svd_pair_kernel<0>})
+                   svd_pair_kernel<0>, svd_pair_kernel<2, 32>, svd_pair_kernel<4, 32>, svd_pair_kernel<8, 32>})
       TTF_HIP(hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)(sizeof(double) * kSvdLdsDoubles)));
     attr_set = true;
@@ -896,7 +922,7 @@ JacobiResult jacobi_columns(int mw, int nw, double* X, double* V, hipStream_t st
   for (res.sweeps = 1; res.sweeps <= 60; ++res.sweeps) {
     TTF_HIP(hipMemsetAsync(offmax, 0, sizeof(unsigned long long), st));
     for (int r = 0; r < nb - 1; ++r) {
-      hipLaunchKernelGGL(kern, dim3(nb / 2), dim3(256), lds, st, mw, nw, bsz, nb, r, X, V, res.small2, tol,
+      hipLaunchKernelGGL(kern, dim3(nb / 2), dim3(threads), lds, st, mw, nw, bsz, nb, r, X, V, res.small2, tol,
                          inner, offmax);
       TTF_HIP(hipGetLastError());
     }
