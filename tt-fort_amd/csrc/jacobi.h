@@ -81,7 +81,8 @@ constexpr double kJacobiSmall = 1e-9;
 // interleaved rotation computations make up the round.  The 2x2 blocks are
 // updated in full (annihilated entries keep their rounding-level values).
 // The loop ends after a sweep with no rotation above `small` (default
-// kJacobiSmall) or after max_sweeps.  `flags` is 3 ints of LDS.  All threads
+// kJacobiSmall) or after max_sweeps; a thread takes at most 4 tasks, so the
+// block needs 4 blockDim.x >= (n / 2 + 1)^2.  `flags` is 3 ints of LDS.  All threads
 // of the block call it; the returned pointer (A or B) holds the diagonalised
 // matrix.
 template <int LD>
@@ -101,18 +102,29 @@ __device__ double (*jacobi_sym(double (*A)[LD], double (*B)[LD], double (*V)[LD]
       V[j][n] = 0.0;
     }
   }
-  const int ntask = half * half;
+  const int ntask = half * half;   // at most 1024 = 4 x 256 tasks
   double(*cur)[LD] = A;
   double(*nxt)[LD] = B;
   if (tid < 3) flags[tid] = 0;
+  // each thread's tasks (pair indices kp, kq), once: the rounds then have no
+  // division on their critical path
+  int kps[4], kqs[4];
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    const int t = tid + w * nt;
+    kps[w] = t % half;
+    kqs[w] = t / half;
+  }
   __syncthreads();
   for (int sweep = 0; sweep < max_sweeps; ++sweep) {
     int* moved = flags + sweep % 3;
     // flags[(sweep+1) % 3] was last read before the previous sweep's barriers
     if (tid == 0) flags[(sweep + 1) % 3] = 0;
     for (int r = 0; r < nn - 1; ++r) {
-      for (int t = tid; t < ntask; t += nt) {
-        const int kp = t % half, kq = t / half;
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        if (tid + w * nt >= ntask) break;
+        const int kp = kps[w], kq = kqs[w];
         int p, q, u, v;
         jacobi_pair(r, kp, nn, p, q);
         jacobi_pair(r, kq, nn, u, v);

@@ -454,7 +454,7 @@ __global__ __launch_bounds__(16 * LPP) void svd_pair_kernel(int mw, int nw, int 
       for (int r = 0; r < nn - 1; ++r) {
         if (g < nn / 2) {
           int p, q;
-          circle_pair(r, g, nn, &p, &q);
+          jacobi_pair(r, g, nn, p, q);   // circle_pair's pairs, without the divisions
           if (q < n) {
             double c, s;
             const double rel = hestenes_pair<NR, LPP>(lds + p * ldp, lds + q * ldp, mw, l16, small2, tol, &c, &s);
