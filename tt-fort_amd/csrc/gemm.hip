@@ -30,11 +30,24 @@ __global__ __launch_bounds__(256) void panel_kernel(GemmOp<T> op) {
   }
   __syncthreads();
   const int64_t total = (int64_t)op.M * op.nbatch;
+  // 32-bit unsigned index arithmetic whenever it fits (a 64-bit division is
+  // a long instruction sequence on the vector ALU)
+  const bool narrow = total <= 0x7fffffff;
   for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
        g += (int64_t)gridDim.x * blockDim.x) {
-    const int i = (int)(g % op.M);
-    const int b = (int)(g / op.M);
-    const int b0 = b % op.nb0, b1 = b / op.nb0;
+    int i, b;
+    if (narrow) {
+      i = (int)((unsigned)g % (unsigned)op.M);
+      b = (int)((unsigned)g / (unsigned)op.M);
+    } else {
+      i = (int)(g % op.M);
+      b = (int)(g / op.M);
+    }
+    int b0 = 0, b1 = b;
+    if (op.nb0 != 1) {
+      b0 = (int)((unsigned)b % (unsigned)op.nb0);
+      b1 = (int)((unsigned)b / (unsigned)op.nb0);
+    }
     const T* a = op.A + b0 * op.sAb0 + b1 * op.sAb1 + i;
     T* c = op.C + b0 * op.sCb0 + b1 * op.sCb1 + i;
     T acc[NN];

@@ -222,11 +222,26 @@ template <typename T>
 __global__ void gemm_splitk_reduce(GemmOp<T> op) {
   int64_t mn = (int64_t)op.M * op.N;
   int64_t total = mn * op.nbatch;
+  // 32-bit unsigned index arithmetic whenever it fits (a 64-bit division is
+  // a long instruction sequence on the vector ALU)
+  const bool narrow = total <= 0x7fffffff;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
-    int b = (int)(t / mn);
-    int64_t e = t % mn;
-    int gi = (int)(e % op.M), gj = (int)(e / op.M);
+    int b, gi, gj;
+    int64_t e;
+    if (narrow) {
+      const unsigned tt = (unsigned)t, m = (unsigned)op.M, umn = (unsigned)mn;
+      b = (int)(tt / umn);
+      const unsigned ee = tt - (unsigned)b * umn;
+      e = ee;
+      gj = (int)(ee / m);
+      gi = (int)(ee - (unsigned)gj * m);
+    } else {
+      b = (int)(t / mn);
+      e = t % mn;
+      gi = (int)(e % op.M);
+      gj = (int)(e / op.M);
+    }
     // load all partial sums first (latency-bound), then add in fixed order
     T part[16];
 #pragma unroll
@@ -235,7 +250,11 @@ __global__ void gemm_splitk_reduce(GemmOp<T> op) {
     T s = 0;
 #pragma unroll
     for (int z = 0; z < 16; ++z) s += part[z];
-    int b0 = b % op.nb0, b1 = b / op.nb0;
+    int b0 = 0, b1 = b;
+    if (op.nb0 != 1) {
+      b0 = (int)((unsigned)b % (unsigned)op.nb0);
+      b1 = (int)((unsigned)b / (unsigned)op.nb0);
+    }
     T* c = op.C + b0 * op.sCb0 + b1 * op.sCb1 + gi * op.sCi + gj * op.sCj;
     T v = op.alpha * s;
     if (op.beta != T(0)) v += op.beta * (*c);

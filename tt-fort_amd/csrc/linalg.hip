@@ -562,10 +562,8 @@ __global__ __launch_bounds__(256) void qr_panel_kernel(int rows, int pc, double*
   }
   __syncthreads();
   // explicit unit-lower V in the panel
-  for (int e = tid; e < pc * rows; e += blockDim.x) {
-    const int i = e % rows, c = e / rows;
-    if (i <= c) P[c * lp + i] = (i == c) ? 1.0 : 0.0;
-  }
+  for (int c = 0; c < pc; ++c)
+    for (int i = tid; i <= c && i < rows; i += blockDim.x) P[c * lp + i] = (i == c) ? 1.0 : 0.0;
   __syncthreads();
   lds_gram(P, rows, lp, pc, S, &T[0][0]);   // S = V^T V (T used as scratch)
   // T: T(j, j) = tau_j, T(0:j, j) = -tau_j T(0:j, 0:j) S(0:j, j)
@@ -580,18 +578,14 @@ __global__ __launch_bounds__(256) void qr_panel_kernel(int rows, int pc, double*
     if (tid == 0) T[j][j] = tau[j];
     __syncthreads();
   }
-  for (int e = tid; e < pc * rows; e += blockDim.x) {
-    const int i = e % rows, c = e / rows;
-    Vb[i + (int64_t)rows * c] = P[c * lp + i];
-  }
+  for (int c = 0; c < pc; ++c)
+    for (int i = tid; i < rows; i += blockDim.x) Vb[i + (int64_t)rows * c] = P[c * lp + i];
   __syncthreads();
   // Y = V T^T: lds_apply_rotation computes X Qp with Qp(k, j) = Qc[j][k] = T[j][k]
   lds_apply_rotation(P, rows, lp, pc, T);
   __syncthreads();
-  for (int e = tid; e < pc * rows; e += blockDim.x) {
-    const int i = e % rows, c = e / rows;
-    Yb[i + (int64_t)rows * c] = P[c * lp + i];
-  }
+  for (int c = 0; c < pc; ++c)
+    for (int i = tid; i < rows; i += blockDim.x) Yb[i + (int64_t)rows * c] = P[c * lp + i];
 }
 
 // X (q x q, ld q) = R^T for the upper triangle R of W (ld ldw).
