@@ -17,7 +17,7 @@
 namespace ttf {
 
 // ------------------------------------------------------------------ tsgram
-constexpr int kTsgRows = 64;
+constexpr int kTsgRows = 64;   // rows per chunk: one per lane of a wave
 
 // G (p x q) = X^T Y for tall X (N x p), Y (N x q) in two launches: every
 // work-group of tsgram_partial sums row chunks of 64 into its partial p x q
@@ -28,40 +28,61 @@ constexpr int kTsgRows = 64;
 __global__ __launch_bounds__(256) void tsgram_partial(int N, int p, int q, const double* __restrict__ X,
                                                       int64_t ldx, const double* __restrict__ Y,
                                                       int64_t ldy, double* __restrict__ part) {
+  // chunk rows staged in LDS (columns past p / q as zeros), partial products
+  // on the matrix cores: the output's 16x16 tiles (at most 16) go to the
+  // four waves in turn, each tile a chain of 64-row x 4-deep MFMAs per chunk
   __shared__ double Xs[kTsgRows][65];
   __shared__ double Ys[kTsgRows][65];
-  const int tid = threadIdx.x;
-  const int pq = p * q;
-  double acc[16];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tp = (p + 15) >> 4, tq = (q + 15) >> 4, ntile = tp * tq;
+  const int fi = lane & 15, fk = lane >> 4;
+  d4_t acc[4];
 #pragma unroll
-  for (int t = 0; t < 16; ++t) acc[t] = 0.0;
+  for (int u = 0; u < 4; ++u) acc[u] = d4_t{0.0, 0.0, 0.0, 0.0};
   const int nchunks = (N + kTsgRows - 1) / kTsgRows;
   for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
     const int r0 = c * kTsgRows;
-    __syncthreads();
-    for (int e = tid; e < kTsgRows * 64; e += 256) {
-      int r = e % kTsgRows, i = e / kTsgRows;
-      bool ok = (r0 + r) < N;
-      if (i < p) Xs[r][i] = ok ? X[(int64_t)(r0 + r) + ldx * i] : 0.0;
-      if (i < q) Ys[r][i] = ok ? Y[(int64_t)(r0 + r) + ldy * i] : 0.0;
+    // thread: row r of the chunk, columns i0, i0 + 4, ...; every load in
+    // flight before the LDS stores (a wave reads 64 consecutive rows)
+    const int r = tid & 63, i0 = tid >> 6;
+    const bool ok = (r0 + r) < N;
+    double xv[16], yv[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int i = i0 + 4 * u;
+      xv[u] = (ok && i < p) ? X[(int64_t)(r0 + r) + ldx * i] : 0.0;
+      yv[u] = (ok && i < q) ? Y[(int64_t)(r0 + r) + ldy * i] : 0.0;
     }
     __syncthreads();
 #pragma unroll
-    for (int t = 0; t < 16; ++t) {
-      int e = tid + 256 * t;
-      if (e < pq) {
-        int i = e % p, j = e / p;
-        double s = 0.0;
-#pragma unroll 8
-        for (int r = 0; r < kTsgRows; ++r) s += Xs[r][i] * Ys[r][j];
-        acc[t] += s;
+    for (int u = 0; u < 16; ++u) {
+      Xs[r][i0 + 4 * u] = xv[u];
+      Ys[r][i0 + 4 * u] = yv[u];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int t = wv + 4 * u;
+      if (t < ntile) {   // uniform over the wave
+        const int ti = t % tp, tj = t / tp;
+#pragma unroll 4
+        for (int k0 = 0; k0 < kTsgRows; k0 += 4)
+          acc[u] = mfma16(Xs[k0 + fk][16 * ti + fi], Ys[k0 + fk][16 * tj + fi], acc[u]);
       }
     }
   }
+  const int pq = p * q;
 #pragma unroll
-  for (int t = 0; t < 16; ++t) {
-    int e = tid + 256 * t;
-    if (e < pq) part[(int64_t)blockIdx.x * pq + e] = acc[t];
+  for (int u = 0; u < 4; ++u) {
+    const int t = wv + 4 * u;
+    if (t < ntile) {
+      const int ti = t % tp, tj = t / tp;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * ti + mfma_row<double>(lane, r), j = 16 * tj + fi;
+        if (i < p && j < q) part[(int64_t)blockIdx.x * pq + i + p * j] = acc[u][r];
+      }
+    }
   }
 }
 
