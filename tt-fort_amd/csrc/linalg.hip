@@ -92,11 +92,17 @@ __global__ __launch_bounds__(256) void tsgram_reduce(int p, int q, int nparts, c
                                                      double* __restrict__ G, int ldg) {
   const int pq = p * q, tid = threadIdx.x;
   const int e = blockIdx.x * (256 / kTsgSub) + tid / kTsgSub, k = tid % kTsgSub;
-  double s = 0.0;
-  if (e < pq) {
-#pragma unroll 4
-    for (int w = k; w < nparts; w += kTsgSub) s += part[(int64_t)w * pq + e];
+  // each lane's partials (nparts <= 256: at most 16) all loaded before the
+  // sum, in a fixed order
+  double v[256 / kTsgSub];
+#pragma unroll
+  for (int u = 0; u < 256 / kTsgSub; ++u) {
+    const int w = k + kTsgSub * u;
+    v[u] = (e < pq && w < nparts) ? part[(int64_t)w * pq + e] : 0.0;
   }
+  double s = 0.0;
+#pragma unroll
+  for (int u = 0; u < 256 / kTsgSub; ++u) s += v[u];
   s = row_sum16(s);
   if (k == 0 && e < pq) G[(e % p) + (int64_t)ldg * (e / p)] = s;
 }
@@ -105,7 +111,7 @@ void tsgram(int N, int p, int q, const double* X, int64_t ldx, const double* Y, 
             double* G, int ldg, hipStream_t st) {
   TTF_REQUIRE(p >= 1 && q >= 1 && p <= 64 && q <= 64, "tsgram: p=%d q=%d out of range", p, q);
   const int nchunks = (int)ceil_div(std::max(N, 1), kTsgRows);
-  const int nparts = std::min(nchunks, 256);
+  const int nparts = std::min(nchunks, 256);   // tsgram_reduce takes at most 256 partials
   double* part = (double*)arena(kArenaGram, sizeof(double) * (size_t)nparts * p * q);
   hipLaunchKernelGGL(tsgram_partial, dim3(nparts), dim3(256), 0, st, N, p, q, X, ldx, Y, ldy, part);
   TTF_HIP(hipGetLastError());

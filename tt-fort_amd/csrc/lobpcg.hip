@@ -311,23 +311,45 @@ __global__ __launch_bounds__(256) void residual_kernel(int N, int bs, const doub
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int per = (N + gridDim.x - 1) / gridDim.x;
   const int r0 = blockIdx.x * per, r1 = min(N, r0 + per);
-  for (int c = 0; c < bs; ++c) {
-    const double th = theta[c];
-    const double floor_c = delta * (1.0 + fabs(th));
-    double s = 0.0;
+  // eight columns at a time, each row's loads of all eight issued together
+  for (int c0 = 0; c0 < bs; c0 += 8) {
+    double s[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s[u] = 0.0;
     for (int i = r0 + tid; i < r1; i += 256) {
-      const double w = AX[i + ld * c] - th * X[i + ld * c];
-      if (D) {
-        double den = D[i] - th;
-        if (fabs(den) < floor_c) den = den >= 0.0 ? floor_c : -floor_c;
-        W[i + ld * c] = w / den;
-      } else {
-        W[i + ld * c] = w;
+      double ax[8], xx[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int c = c0 + u;
+        ax[u] = c < bs ? AX[i + ld * c] : 0.0;
+        xx[u] = c < bs ? X[i + ld * c] : 0.0;
       }
-      s += w * w;
+      const double di = D ? D[i] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int c = c0 + u;
+        if (c < bs) {
+          const double th = theta[c];
+          const double w = ax[u] - th * xx[u];
+          if (D) {
+            const double floor_c = delta * (1.0 + fabs(th));
+            double den = di - th;
+            if (fabs(den) < floor_c) den = den >= 0.0 ? floor_c : -floor_c;
+            W[i + ld * c] = w / den;
+          } else {
+            W[i + ld * c] = w;
+          }
+          s[u] += w * w;
+        }
+      }
     }
-    s = wave_sum64(s);
-    if (lane == 0) red[wv][c] = s;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (c0 + u < bs) {   // uniform
+        const double t = wave_sum64(s[u]);
+        if (lane == 0) red[wv][c0 + u] = t;
+      }
+    }
   }
   __syncthreads();
   if (tid < bs) part[(int64_t)blockIdx.x * bs + tid] = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
@@ -340,9 +362,16 @@ __global__ __launch_bounds__(256) void residual_norms(int bs, int nparts, const 
                                                       double* __restrict__ nrm) {
   const int t = blockIdx.x * 256 + threadIdx.x;
   const int c = t / 16, k = t % 16;
+  // nparts <= 256: at most 16 partials per lane, all loaded before the sum
+  double v[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int w = k + 16 * u;
+    v[u] = (c < bs && w < nparts) ? part[(int64_t)w * bs + c] : 0.0;
+  }
   double s = 0.0;
-  if (c < bs)
-    for (int w = k; w < nparts; w += 16) s += part[(int64_t)w * bs + c];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) s += v[u];
   s = row_sum16(s);
   if (k == 0 && c < bs) nrm[c] = sqrt(s);
 }
