@@ -38,19 +38,23 @@ __device__ inline void jacobi_pair(int r, int k, int nn, int& p, int& q) {
 // c^2 + s^2 = 1 to rounding (the rotation stays orthogonal).  Branch free,
 // so the rotations of two pairs interleave.
 __device__ __forceinline__ void jacobi_cs(double apq, double app, double aqq, bool go, double& c, double& s) {
-  const double d = aqq - app;
-  const double q = go ? d * d + 4.0 * apq * apq : 1.0;
+  // `go` only selects the inputs: (d, a) = (1, 0) gives q = 1, den = 2,
+  // t = 0, hence c = 1, s = 0, with no branch for the compiler to make
+  // around the arithmetic (which would serialise a thread's two rotations)
+  const double a = go ? apq : 0.0;
+  const double d = go ? aqq - app : 1.0;
+  const double q = d * d + 4.0 * a * a;
   double rq = __builtin_amdgcn_rsq(q);
   rq = rq * (1.5 - 0.5 * q * rq * rq);
-  const double den = go ? fabs(d) + q * rq : 1.0;
+  const double den = fabs(d) + q * rq;
   double ri = __builtin_amdgcn_rcp(den);
   ri = ri * (2.0 - den * ri);
-  const double t = go ? (d >= 0.0 ? 2.0 : -2.0) * apq * ri : 0.0;
+  const double t = (d >= 0.0 ? 2.0 : -2.0) * a * ri;
   const double x = 1.0 + t * t;
   double cc = __builtin_amdgcn_rsq(x);
   cc = cc * (1.5 - 0.5 * x * cc * cc);
   cc = cc * (1.5 - 0.5 * x * cc * cc);
-  c = go ? cc : 1.0;
+  c = cc;
   s = t * cc;
 }
 
