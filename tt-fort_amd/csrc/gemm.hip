@@ -83,8 +83,11 @@ static bool panel_eligible(const GemmOp<T>& op) {
 
 // ---------------------------------------------------------------- autotuner
 struct Cfg {
-  int bm, bk, ks;   // square tiles bm x bm, K chunk bk, split-K factor ks
+  int bm, bk, ks;   // tiles bm x bm (or bm x bn), K chunk bk, split-K factor ks
+  int bn = 0;       // 0: square tiles; 80: 64 x 80 tiles, four waves stacked in rows
 };
+
+static int cfg_bn(const Cfg& c) { return c.bn > 0 ? c.bn : c.bm; }
 
 struct Key {
   int type, M, N, K, nbatch, flags;
@@ -110,7 +113,8 @@ bool g_table_loaded = false;
 }  // namespace
 
 // Persistent tuning table: when TTF_TUNE_FILE names a file, its entries
-// ("type M N K nbatch flags bm bk ks" per line) are loaded on first use and
+// ("type M N K nbatch flags bm bk ks bn" per line; a line without bn means
+// square tiles) are loaded on first use and
 // every newly tuned shape is appended, so that separate processes (a timed
 // run and its counter-collection reruns, or successive jobs) launch the same
 // configuration for the same product.  Called with g_tune_mu held.
@@ -126,12 +130,16 @@ static void load_tune_table() {
   if (!path) return;
   FILE* fp = std::fopen(path, "r");
   if (!fp) return;
-  Key k{};
-  Cfg c{};
-  while (std::fscanf(fp, "%d %d %d %d %d %d %d %d %d", &k.type, &k.M, &k.N, &k.K, &k.nbatch, &k.flags,
-                     &c.bm, &c.bk, &c.ks) == 9) {
-    const bool valid = (c.bm == 32 || c.bm == 64) && (c.bk == 16 || c.bk == 32 || c.bk == 64) &&
-                       c.ks >= 1 && c.ks <= 16;
+  char line[256];
+  while (std::fgets(line, sizeof(line), fp)) {
+    Key k{};
+    Cfg c{};
+    const int got = std::sscanf(line, "%d %d %d %d %d %d %d %d %d %d", &k.type, &k.M, &k.N, &k.K, &k.nbatch,
+                                &k.flags, &c.bm, &c.bk, &c.ks, &c.bn);
+    if (got == 9) c.bn = 0;
+    const bool valid = (got == 9 || got == 10) && (c.bm == 32 || c.bm == 64) &&
+                       (c.bk == 16 || c.bk == 32 || c.bk == 64) && c.ks >= 1 && c.ks <= 16 &&
+                       (c.bn == 0 || (c.bn == 80 && c.bm == 64 && c.bk <= 32));
     if (valid) g_tuned[k] = c;
   }
   std::fclose(fp);
@@ -142,8 +150,8 @@ static void record_tuned(const Key& k, const Cfg& c) {
   if (!path) return;
   FILE* fp = std::fopen(path, "a");
   if (!fp) return;
-  std::fprintf(fp, "%d %d %d %d %d %d %d %d %d\n", k.type, k.M, k.N, k.K, k.nbatch, k.flags, c.bm, c.bk,
-               c.ks);
+  std::fprintf(fp, "%d %d %d %d %d %d %d %d %d %d\n", k.type, k.M, k.N, k.K, k.nbatch, k.flags, c.bm, c.bk,
+               c.ks, c.bn);
   std::fclose(fp);
 }
 
@@ -170,7 +178,11 @@ static Cfg heuristic(int M, int N, int K, int nbatch) {
 
 template <typename T, bool CT>
 static void launch_tiles(const GemmOp<T>& op, const Cfg& c, dim3 grid, hipStream_t st) {
-  if (c.bm == 64 && c.bk == 16)
+  if (cfg_bn(c) == 80 && c.bk == 16)
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, 64, 80, 16, 4, CT, 4>), grid, dim3(256), 0, st, op);
+  else if (cfg_bn(c) == 80)
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, 64, 80, 32, 4, CT, 4>), grid, dim3(256), 0, st, op);
+  else if (c.bm == 64 && c.bk == 16)
     hipLaunchKernelGGL((gemm_mfma_kernel<T, 64, 64, 16, 4, CT>), grid, dim3(256), 0, st, op);
   else if (c.bm == 64 && c.bk == 32)
     hipLaunchKernelGGL((gemm_mfma_kernel<T, 64, 64, 32, 4, CT>), grid, dim3(256), 0, st, op);
@@ -187,7 +199,7 @@ static void launch_tiles(const GemmOp<T>& op, const Cfg& c, dim3 grid, hipStream
 template <typename T>
 static void launch_cfg(GemmOp<T> op, const Cfg& c, hipStream_t st) {
   op.ksplit = c.ks;
-  int64_t tiles = ceil_div(op.M, c.bm) * ceil_div(op.N, c.bm);
+  int64_t tiles = ceil_div(op.M, c.bm) * ceil_div(op.N, cfg_bn(c));
   dim3 grid((unsigned)tiles, (unsigned)op.nbatch, (unsigned)op.ksplit);
   // column-contiguous outputs (and the column-major split-K slices) take the
   // swapped-fragment variant, whose lanes store consecutive rows
@@ -216,13 +228,22 @@ static T* splitk_work(const GemmOp<T>& op, int ks) {
 template <typename T>
 static Cfg tune(const GemmOp<T>& op0, hipStream_t st) {
   std::vector<Cfg> cands;
-  for (int bm : {64, 32})
-    for (int bk : {16, 32, 64})
-      for (int ks : {1, 2, 4, 8}) {
-        if (ks > 1 && op0.K / ks < 64) continue;
-        int64_t tiles = ceil_div(op0.M, bm) * ceil_div(op0.N, bm) * (int64_t)op0.nbatch;
-        if (ks > 1 && tiles * ks > 4096) continue;
-        cands.push_back({bm, bk, ks});
+  // square tiles, and 64 x 80 tiles (four waves of 16 x 80) when N takes
+  // them evenly: N = 640 (a bond of 128 x MPO rank 5) splits into 8 such
+  // columns, so that a batch of 8 fills the 256 CUs once
+  for (int bn : {0, 80})
+    for (int bm : {64, 32})
+      for (int bk : {16, 32, 64}) {
+        if (bn == 80 && (bm != 64 || bk > 32 || op0.N % 80 != 0)) continue;
+        for (int ks : {1, 2, 4, 8}) {
+          if (ks > 1 && op0.K / ks < 64) continue;
+          const int cbn = bn > 0 ? bn : bm;
+          int64_t tiles = ceil_div(op0.M, bm) * ceil_div(op0.N, cbn) * (int64_t)op0.nbatch;
+          if (ks > 1 && tiles * ks > 4096) continue;
+          Cfg c{bm, bk, ks};
+          c.bn = bn;
+          cands.push_back(c);
+        }
       }
   hipEvent_t e0, e1;
   TTF_HIP(hipEventCreate(&e0));

@@ -83,11 +83,12 @@ __device__ __forceinline__ int xcd_remap(int wg, int total) {
 // in swapped order, B^T A^T, so that each accumulator holds a tile of C^T:
 // lane l then owns row l%16 of the 16x16 block and consecutive lanes store
 // consecutive rows (contiguous in memory) instead of consecutive columns.
-template <typename T, int BM, int BN, int BK, int NW = 4, bool CT = false>
+template <typename T, int BM, int BN, int BK, int NW = 4, bool CT = false, int WROWS = 0>
 __global__ __launch_bounds__(64 * NW) void gemm_mfma_kernel(GemmOp<T> op) {
   constexpr int NT = 64 * NW;
   constexpr int PAD = 2;
-  constexpr int WR = (NW == 4) ? 2 : 1;   // wave rows
+  // wave rows: 2 x 2 waves by default, WROWS x (NW / WROWS) when given
+  constexpr int WR = WROWS > 0 ? WROWS : ((NW == 4) ? 2 : 1);
   constexpr int WC = NW / WR;             // wave columns
   constexpr int WM = BM / WR, WN = BN / WC;
   constexpr int FM = WM / 16, FN = WN / 16;
