@@ -466,10 +466,12 @@ __global__ __launch_bounds__(16 * LPP) void svd_pair_kernel(int mw, int nw, int 
   if (__longlong_as_double((long long)cmax) <= 0.5 * tol) return;
   // ---- 2. Hestenes rotations on the pair's columns; 16 groups of LPP
   // threads take one column pair each per round of the circle schedule
-  for (int e = tid; e < kSvdMaxP * kSvdMaxP; e += blockDim.x) {
-    const int j = e / kSvdMaxP, i = e % kSvdMaxP;
-    Qc[j][i] = (i == j) ? 1.0 : 0.0;
-  }
+  const bool accum = (V != nullptr);
+  if (accum)
+    for (int e = tid; e < kSvdMaxP * kSvdMaxP; e += blockDim.x) {
+      const int j = e / kSvdMaxP, i = e % kSvdMaxP;
+      Qc[j][i] = (i == j) ? 1.0 : 0.0;
+    }
   __syncthreads();
   {
     const int nn = n + (n & 1);
@@ -487,11 +489,13 @@ __global__ __launch_bounds__(16 * LPP) void svd_pair_kernel(int mw, int nw, int 
             const double rel = hestenes_pair<NR, LPP>(lds + p * ldp, lds + q * ldp, mw, l16, small2, tol, &c, &s);
             seen = fmax(seen, rel);
             if (s != 0.0) {
+              if (accum) {   // the rotations are only needed to update V
 #pragma unroll
-              for (int i = l16; i < kSvdMaxP; i += LPP) {
-                const double u = Qc[p][i], v = Qc[q][i];
-                Qc[p][i] = c * u - s * v;
-                Qc[q][i] = s * u + c * v;
+                for (int i = l16; i < kSvdMaxP; i += LPP) {
+                  const double u = Qc[p][i], v = Qc[q][i];
+                  Qc[p][i] = c * u - s * v;
+                  Qc[q][i] = s * u + c * v;
+                }
               }
               if (l16 == 0) any = 1;
             }
