@@ -328,7 +328,19 @@ void zgemm_launch(ZGemmOp op, hipStream_t st) {
   TTF_REQUIRE(op.nbatch <= 65535, "zgemm: batch count %d exceeds the grid limit", op.nbatch);
   Cfg c = heuristic(op.M, op.N, op.K, op.nbatch);
   int64_t tiles = ceil_div(op.M, c.bm) * ceil_div(op.N, c.bm);
-  dim3 grid((unsigned)tiles, (unsigned)op.nbatch, 1);
+  // few work-groups with a long K: slices of K in parallel (at least 48 deep
+  // each), summed by a second launch
+  op.ksplit = 1;
+  op.work = nullptr;
+  if (c.bm == 32 && tiles * op.nbatch < 32 && op.K >= 128) {
+    const int ks = (int)std::min<int64_t>({8, op.K / 48, ceil_div(64, tiles * op.nbatch)});
+    if (ks > 1) {
+      op.ksplit = ks;
+      op.work = (hipDoubleComplex*)arena(kArenaSplitK, sizeof(hipDoubleComplex) * (size_t)ks * op.nbatch *
+                                                           (size_t)op.M * op.N);
+    }
+  }
+  dim3 grid((unsigned)tiles, (unsigned)op.nbatch, (unsigned)op.ksplit);
   // 32x32 tiles come with few work-groups, each one's K loop latency-bound
   // on its global loads: a 32-deep K chunk halves the trips
   if (c.bm == 64)
@@ -338,6 +350,12 @@ void zgemm_launch(ZGemmOp op, hipStream_t st) {
   else
     hipLaunchKernelGGL((zgemm_mfma_kernel<32, 32>), grid, dim3(256), 0, st, op);
   TTF_HIP(hipGetLastError());
+  if (op.ksplit > 1) {
+    const int64_t total = (int64_t)op.M * op.N * op.nbatch;
+    hipLaunchKernelGGL(zgemm_splitk_reduce, dim3((unsigned)std::min<int64_t>(ceil_div(total, 256), 4096)), dim3(256),
+                       0, st, op);
+    TTF_HIP(hipGetLastError());
+  }
 }
 
 long gemm_tuned_shapes() {

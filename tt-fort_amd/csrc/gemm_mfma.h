@@ -275,6 +275,10 @@ struct ZGemmOp {
   int nb0, nbatch;
   int conjA, conjB;
   hipDoubleComplex alpha, beta;
+  // split-K (ksplit > 1): slice z of K goes to work + (z nbatch + b) M N,
+  // column-major and without alpha / beta; zgemm_splitk_reduce finishes C
+  int ksplit;
+  hipDoubleComplex* work;
 };
 
 template <int BM, int BN, int BK = 16>
@@ -298,6 +302,9 @@ __global__ __launch_bounds__(256) void zgemm_mfma_kernel(ZGemmOp op) {
   const hipDoubleComplex* __restrict__ A = op.A + b0 * op.sAb0 + b1 * op.sAb1;
   const hipDoubleComplex* __restrict__ B = op.B + b0 * op.sBb0 + b1 * op.sBb1;
   const bool a_ifast = (op.sAi == 1), b_jfast = (op.sBj == 1);
+  const int ks = op.ksplit > 1 ? op.ksplit : 1;
+  const int kchunk = ((op.K + ks - 1) / ks + BK - 1) / BK * BK;
+  const int kbeg = blockIdx.z * kchunk, kend = min(op.K, kbeg + kchunk);
   const double sa = op.conjA ? -1.0 : 1.0, sb = op.conjB ? -1.0 : 1.0;
 
   double2 ra[LA], rb[LB];
@@ -316,7 +323,7 @@ __global__ __launch_bounds__(256) void zgemm_mfma_kernel(ZGemmOp op) {
 #pragma unroll
     for (int q = 0; q < LA; ++q) {
       int gi = tm * BM + ai[q], gk = k0 + ak[q];
-      if (gi < op.M && gk < op.K) {
+      if (gi < op.M && gk < kend) {
         hipDoubleComplex v = A[gi * op.sAi + gk * op.sAk];
         ra[q] = make_double2(v.x, sa * v.y);
       } else ra[q] = make_double2(0.0, 0.0);
@@ -324,7 +331,7 @@ __global__ __launch_bounds__(256) void zgemm_mfma_kernel(ZGemmOp op) {
 #pragma unroll
     for (int q = 0; q < LB; ++q) {
       int gj = tn * BN + bj[q], gk = k0 + bk[q];
-      if (gj < op.N && gk < op.K) {
+      if (gj < op.N && gk < kend) {
         hipDoubleComplex v = B[gk * op.sBk + gj * op.sBj];
         rb[q] = make_double2(v.x, sb * v.y);
       } else rb[q] = make_double2(0.0, 0.0);
@@ -337,15 +344,15 @@ __global__ __launch_bounds__(256) void zgemm_mfma_kernel(ZGemmOp op) {
 #pragma unroll
     for (int j = 0; j < FN; ++j) { cr[i][j] = d4_t{0, 0, 0, 0}; ci[i][j] = d4_t{0, 0, 0, 0}; }
 
-  if (op.K > 0) load(0);
-  for (int k0 = 0; k0 < op.K; k0 += BK) {
+  if (kbeg < kend) load(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < LA; ++q) { Ar[ak[q]][ai[q]] = ra[q].x; Ai[ak[q]][ai[q]] = ra[q].y; }
 #pragma unroll
     for (int q = 0; q < LB; ++q) { Br[bk[q]][bj[q]] = rb[q].x; Bi[bk[q]][bj[q]] = rb[q].y; }
     __syncthreads();
-    if (k0 + BK < op.K) load(k0 + BK);
+    if (k0 + BK < kend) load(k0 + BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       double ar[FM], ai_[FM], nai[FM], br[FN], bi[FN];
@@ -370,6 +377,20 @@ __global__ __launch_bounds__(256) void zgemm_mfma_kernel(ZGemmOp op) {
         }
     }
   }
+  if (ks > 1) {   // raw partial sums of this K slice
+    hipDoubleComplex* __restrict__ Wz = op.work + ((int64_t)blockIdx.z * op.nbatch + b) * op.M * op.N;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gi = tm * BM + wm * WM + i * 16 + mfma_row<double>(lane, r);
+          const int gj = tn * BN + wn * WN + j * 16 + (lane & 15);
+          if (gi < op.M && gj < op.N) Wz[gi + (int64_t)gj * op.M] = make_hipDoubleComplex(cr[i][j][r], ci[i][j][r]);
+        }
+    return;
+  }
   hipDoubleComplex* __restrict__ C = op.C + b0 * op.sCb0 + b1 * op.sCb1;
   const bool beta0 = (op.beta.x == 0.0 && op.beta.y == 0.0);
 #pragma unroll
@@ -393,6 +414,33 @@ __global__ __launch_bounds__(256) void zgemm_mfma_kernel(ZGemmOp op) {
           *c = make_hipDoubleComplex(vr, vi);
         }
       }
+}
+
+// C = alpha (sum of the ksplit partial slices, in slice order) + beta C.
+static __global__ void zgemm_splitk_reduce(ZGemmOp op) {
+  const int64_t mn = (int64_t)op.M * op.N, total = mn * op.nbatch;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(t / mn);
+    const int64_t e = t - (int64_t)b * mn;
+    const int gj = (int)(e / op.M), gi = (int)(e - (int64_t)gj * op.M);
+    double xr = 0.0, xi = 0.0;
+    for (int z = 0; z < op.ksplit; ++z) {
+      const hipDoubleComplex v = op.work[((int64_t)z * op.nbatch + b) * mn + e];
+      xr += v.x;
+      xi += v.y;
+    }
+    const int b0 = b % op.nb0, b1 = b / op.nb0;
+    hipDoubleComplex* c = op.C + b0 * op.sCb0 + b1 * op.sCb1 + gi * op.sCi + gj * op.sCj;
+    double vr = op.alpha.x * xr - op.alpha.y * xi;
+    double vi = op.alpha.x * xi + op.alpha.y * xr;
+    if (op.beta.x != 0.0 || op.beta.y != 0.0) {
+      const hipDoubleComplex o = *c;
+      vr += op.beta.x * o.x - op.beta.y * o.y;
+      vi += op.beta.x * o.y + op.beta.y * o.x;
+    }
+    *c = make_hipDoubleComplex(vr, vi);
+  }
 }
 
 // ---------------------------------------------------------------- launchers
