@@ -168,3 +168,57 @@ def test_invalid_sizes_report_error(ttlib):
     assert st != 0
     assert b"invalid sizes" in lib.ttf_last_error()
     lib.ttf_clear_error()
+
+
+def test_dbfun3_rect_tiles_pinned(ttlib, tmp_path):
+    """The 64 x 80 GEMM tiles (four waves in rows), which the autotuner may
+    pick for stage 1 when N is a multiple of 80, pinned through a tuning
+    table in a child process (the table is read at the first GEMM of a
+    process) at the (128, 2, 128, R = 5) shape, against the oracle; both
+    K chunks (32 and 16) are covered."""
+    import json
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    tune = tmp_path / "tune.txt"
+    # type M N K nbatch flags bm bk ks bn: stage 1 is M=256 N=640 K=128 with
+    # A and C row-fast (flags 5), one batch per vector
+    pinned = ["0 256 640 128 8 5 64 32 1 80", "0 256 640 128 4 5 64 16 1 80"]
+    tune.write_text("\n".join(pinned) + "\n")
+    code = r'''
+import json, os, sys
+root = sys.argv[1]
+sys.path[:0] = [root, os.path.join(root, "tt-fort_amd")]
+import numpy as np
+import ttfort
+from oracle import kernels as K
+lib = ttfort.load()
+ttfort.check(lib.ttf_init(0), lib)
+shape = (128, 2, 128, 128, 2, 128, 5, 5)
+rx1, m, rx2, ry1, n, ry2, ra1, ra2 = shape
+rng = np.random.default_rng(21)
+worst = 0.0
+for nvec in (8, 4):
+    phi1 = rng.standard_normal(ry1 * rx1 * ra1)
+    A = rng.standard_normal(ra1 * n * m * ra2)
+    phi2 = rng.standard_normal(rx2 * ra2 * ry2)
+    x = rng.standard_normal(rx1 * m * rx2 * nvec)
+    y = ttfort.bfun3(shape, phi1, A, phi2, x, nvec=nvec)
+    nx, ny = rx1 * m * rx2, ry1 * n * ry2
+    scale = (np.abs(phi1).max() * np.abs(A).max() * np.abs(phi2).max() * np.abs(x).max()
+             * rx1 * m * rx2 * ra1 * ra2)
+    for v in range(nvec):
+        ref = K.dbfun3(*shape, phi1, A, phi2, x[v * nx:(v + 1) * nx])
+        worst = max(worst, float(np.abs(y[v * ny:(v + 1) * ny] - ref).max() / scale))
+print(json.dumps({"rel": worst}))
+'''
+    env = dict(os.environ, TTF_TUNE_FILE=str(tune))
+    r = subprocess.run([sys.executable, "-c", code, root], env=env, capture_output=True, text=True,
+                       timeout=600)
+    assert r.returncode == 0, r.stderr[-3000:]
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    assert res["rel"] <= 1e-13, res
+    # the pinned stage-1 entries were used: no new stage-1 line was tuned and appended
+    lines = [ln.strip() for ln in tune.read_text().splitlines() if ln.strip()]
+    assert [ln for ln in lines if ln.startswith("0 256 640 128 ")] == pinned, lines
