@@ -68,6 +68,38 @@ def test_gmres(ttlib, ptype):
     assert np.linalg.norm(sol - ref) <= 1e-7 * np.linalg.norm(ref)
 
 
+def test_gmres_long_vectors(ttlib):
+    """GMRES on 102 400-entry Krylov vectors: the fused Gram-Schmidt then
+    works on row chunks of 512 (200 work-groups) rather than one row per
+    thread; same checks as test_gmres."""
+    import ttfort
+    rng = np.random.default_rng(55)
+    rx1, n, rx2, ra1, ra2 = 40, 64, 40, 2, 2
+    # _local_problem's interface noise grows like sqrt(rank) and would swamp
+    # the identity part at rank 40: scaled down so that the system is well
+    # conditioned and GMRES converges
+    P1 = rng.standard_normal((rx1, rx1, ra1)) / np.sqrt(rx1)
+    P2 = rng.standard_normal((rx2, ra2, rx2)) / np.sqrt(rx2)
+    P1 = 0.5 * (P1 + np.transpose(P1, (1, 0, 2)))
+    P2 = 0.5 * (P2 + np.transpose(P2, (2, 1, 0)))
+    P1[:, :, 0] = np.eye(rx1) * 2 + 0.1 * P1[:, :, 0]
+    P2[:, 0, :] = np.eye(rx2) * 2 + 0.1 * P2[:, 0, :]
+    Am = 0.2 * rng.standard_normal((ra1, n, n, ra2))
+    Am[0, :, :, 0] += 4 * np.eye(n) - np.eye(n, k=1) - np.eye(n, k=-1)
+    phi1, A, phi2 = K._flat(P1), K._flat(Am), K._flat(P2)
+    N = rx1 * n * rx2
+    rhs = rng.standard_normal(N)
+    jacs = ttfort.djac_gen("c", rx1, n, rx2, ra1, ra2, phi1, A, phi2)
+    tol = 1e-10
+    sol, it, relres = ttfort.gmres(phi1, A, phi2, rhs, rx1, n, rx2, ra1, ra2, 40, tol, 4, "c", jacs)
+    B = lambda v: K.dbfun3(rx1, n, rx2, rx1, n, rx2, ra1, ra2, phi1, A, phi2, v)
+    P = lambda v: K.djac_apply("c", rx1, n, rx2, jacs, v)
+    true_res = np.linalg.norm(B(P(sol)) - rhs) / np.linalg.norm(rhs)
+    assert relres < tol and true_res < 10 * tol, (relres, true_res)
+    ref, info = K.dgmresr_hh(B, rhs, 40, tol, 4, prec=P)
+    assert np.linalg.norm(sol - ref) <= 1e-7 * np.linalg.norm(ref)
+
+
 @pytest.mark.parametrize("trans,m,n,numvec", [("N", 30, 50, 4), ("T", 40, 64, 4), ("N", 7, 5, 3)])
 def test_duchol(ttlib, trans, m, n, numvec):
     import ttfort
