@@ -83,76 +83,81 @@ __global__ __launch_bounds__(256) void svqb_kernel(int k, const double* __restri
 constexpr int kMaxB = 32;   // block columns handled by cholqb_kernel
 __global__ __launch_bounds__(64) void cholqb_kernel(int k, const double* __restrict__ G, double* __restrict__ T,
                                                     int* __restrict__ mask) {
-  // lane i owns row i of A; no rows or columns move: the pivots are kept in
-  // perm, and column j of the factor, L[j][i], is indexed by the original row
-  __shared__ double A[kMaxB][kMaxB + 1];   // Schur complements, then L^-T in pivot order
-  __shared__ double L[kMaxB][kMaxB + 1];
-  __shared__ double sc[kMaxB], lc[kMaxB], dinv[kMaxB];
+  __shared__ double A[kMaxB][kMaxB + 1];   // Schur complements (permuted order)
+  __shared__ double L[kMaxB][kMaxB + 1];   // Cholesky factor (permuted order)
+  __shared__ double sc[kMaxB], lc[kMaxB];
   __shared__ int perm[kMaxB];
+  __shared__ int piv, rank_;
   const int lane = threadIdx.x;
-  const bool row = lane < k;
-  const double g = row ? G[lane + k * lane] : 0.0;
-  const double sci = (row && g > 0.0) ? 1.0 / sqrt(g) : 0.0;
-  if (row) sc[lane] = sci;
+  for (int i = lane; i < k; i += 64) {
+    const double g = G[i + k * i];
+    sc[i] = g > 0.0 ? 1.0 / sqrt(g) : 0.0;
+    perm[i] = i;
+  }
+  if (lane == 0) rank_ = k;
   __syncthreads();
-  if (row)
-    for (int c = 0; c < k; ++c) A[lane][c] = 0.5 * (G[lane + k * c] + G[c + k * lane]) * sci * sc[c];
+  for (int e = lane; e < k * k; e += 64) {
+    const int i = e % k, j = e / k;
+    A[i][j] = 0.5 * (G[i + k * j] + G[j + k * i]) * sc[i] * sc[j];
+    L[i][j] = 0.0;
+  }
   __syncthreads();
-  bool done = !row;
-  int r = k;
   for (int j = 0; j < k; ++j) {
-    // pivot: the largest remaining diagonal, lowest index on ties; every
-    // lane ends with the same (v, p)
-    double v = done ? -1.0 : A[lane][lane];
-    int p = lane;
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-      const double ov = __shfl_xor(v, o);
-      const int op = __shfl_xor(p, o);
-      if (ov > v || (ov == v && op < p)) { v = ov; p = op; }
-    }
-    // lane 0's result for every lane, so that the exit below is uniform even
-    // when a NaN keeps the lanes from agreeing
-    v = __shfl(v, 0);
-    p = __shfl(p, 0);
-    if (!(v > 1e-14)) { r = j; break; }   // also stops on NaN
-    const double lpp = sqrt(v);
-    const double li = done ? 0.0 : (lane == p ? lpp : A[lane][p] / lpp);
-    if (row) {
-      lc[lane] = li;
-      L[j][lane] = li;
-    }
-    if (lane == p) {
-      perm[j] = p;
-      dinv[j] = 1.0 / lpp;
-      done = true;
+    if (lane == 0) {
+      int p = j;
+      double best = A[j][j];
+      for (int i = j + 1; i < k; ++i)
+        if (A[i][i] > best) { best = A[i][i]; p = i; }
+      if (!(best > 1e-14)) rank_ = j;   // also stops on NaN
+      piv = p;
     }
     __syncthreads();
-    if (!done)
-      for (int c = 0; c < k; ++c) A[lane][c] -= li * lc[c];
-    __syncthreads();
-  }
-  // column c of L^-T (pivot order: Lp(a, b) = L[b][perm[a]]) by back
-  // substitution in lane c, into A's upper triangle (A is dead)
-  if (lane < r) {
-    const int c = lane;
-    A[c][c] = dinv[c];
-    for (int i = c - 1; i >= 0; --i) {
-      double s = 0.0;
-      for (int l = i + 1; l <= c; ++l) s += L[i][perm[l]] * A[l][c];
-      A[i][c] = -s * dinv[i];
-    }
-  }
-  __syncthreads();
-  // T = D^-1/2 P L^-T: row perm[i] of column c is sc[perm[i]] (L^-T)(i, c)
-  if (row) {
-    const int c = lane;
-    for (int i = 0; i < k; ++i) T[i + k * c] = 0.0;
-    if (c < r)
-      for (int i = 0; i <= c; ++i) {
-        const int o = perm[i];
-        T[o + k * c] = sc[o] * A[i][c];
+    if (rank_ <= j) break;   // the same shared value for every lane
+    const int p = piv;
+    if (p != j) {   // swap indices p and j: A's rows and columns, L's rows, perm
+      for (int i = lane; i < k; i += 64) {
+        const double t = A[i][p]; A[i][p] = A[i][j]; A[i][j] = t;
       }
+      __syncthreads();
+      for (int i = lane; i < k; i += 64) {
+        const double t = A[p][i]; A[p][i] = A[j][i]; A[j][i] = t;
+      }
+      for (int c = lane; c < j; c += 64) {
+        const double t = L[p][c]; L[p][c] = L[j][c]; L[j][c] = t;
+      }
+      if (lane == 0) { const int t = perm[p]; perm[p] = perm[j]; perm[j] = t; }
+      __syncthreads();
+    }
+    const double ljj = sqrt(A[j][j]);
+    for (int i = j + lane; i < k; i += 64) {
+      const double l = i == j ? ljj : A[i][j] / ljj;
+      lc[i] = l;
+      L[i][j] = l;
+    }
+    __syncthreads();
+    const int t = k - j - 1;
+    for (int e = lane; e < t * t; e += 64) {
+      const int i = j + 1 + e % t, c = j + 1 + e / t;
+      A[i][c] -= lc[i] * lc[c];
+    }
+    __syncthreads();
+  }
+  const int r = rank_;
+  // column c of L^-T by back substitution (lane c), scattered to the
+  // original row order with the scaling
+  for (int c = lane; c < k; c += 64) {
+    if (c < r) {
+      A[c][c] = 1.0 / L[c][c];   // A's upper triangle now holds L^-T
+      for (int i = c - 1; i >= 0; --i) {
+        double s = 0.0;
+        for (int l = i + 1; l <= c; ++l) s += L[l][i] * A[l][c];
+        A[i][c] = -s / L[i][i];
+      }
+    }
+    for (int i = 0; i < k; ++i) {
+      const int o = perm[i];
+      T[o + k * c] = (c < r && i <= c) ? sc[o] * A[i][c] : 0.0;
+    }
     mask[c] = c < r ? 1 : 0;
   }
 }
