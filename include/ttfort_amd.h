@@ -399,6 +399,36 @@ int ttf_ztt_ksl(int d, const int* n, const int* m, const int* ra, const ttf_zdou
                 int typ, int order, int usenrm, ttf_zdouble* result, int64_t result_cap,
                 int64_t* result_size, ttf_ksl_stats* stats);
 
+/* ---------------------------------------------------------------- SDV files */
+/* Tensor trains in the reference's SDV stream format (ttio.f90:10-17
+ * header; 29-192 dtt_write / ztt_write; 196-399 dtt_read / ztt_read): a
+ * 128-byte header, then l, m, n(l:m), r(l-1:m) as int32, then the cores
+ * (r(k-1), n(k), r(k)) column-major and back to back, float64 or complex128.
+ * Host memory and host code only: these calls never touch the device and
+ * work without ttf_init.  The reference's readers allocate the cores; here
+ * the caller asks ttf_sdv_dims for the sizes first.
+ *
+ * Status: 0, or the reference readers' info codes -1 (no such file), -2
+ * (not a TT file), -3 (version other than 1), positive errno values for
+ * I/O failures; and, without a counterpart there, -4 (the file ends before
+ * its sizes say), -5 (invalid sizes, or a caller buffer too small), -6 (a
+ * real file given to ttf_ztt_read or a complex one to ttf_dtt_read; the
+ * reference reads it without a check). */
+/* Sizes of the tensor train in `path`: l, m, the d = m - l + 1 mode sizes
+ * n and the d + 1 ranks r (n and r may both be NULL; otherwise dcap >= d),
+ * the element count sum r(k-1) n(k) r(k) of the cores and whether they are
+ * complex (inf(2) = 1 in the header). */
+int ttf_sdv_dims(const char* path, int dcap, int* l, int* m, int* n, int* r, int64_t* count, int* is_complex);
+/* ttio.f90:196-297, dtt_read: the cores into `cores` (cap elements). */
+int ttf_dtt_read(const char* path, int64_t cap, double* cores);
+/* ttio.f90:298-399, ztt_read. */
+int ttf_ztt_read(const char* path, int64_t cap, ttf_zdouble* cores);
+/* ttio.f90:29-109, dtt_write: cores l..m with mode sizes n (m - l + 1
+ * entries) and ranks r (m - l + 2 entries).  Replaces an existing file. */
+int ttf_dtt_write(const char* path, int l, int m, const int* n, const int* r, const double* cores);
+/* ttio.f90:110-192, ztt_write. */
+int ttf_ztt_write(const char* path, int l, int m, const int* n, const int* r, const ttf_zdouble* cores);
+
 /* ------------------------------------------ reference-compatible symbols */
 /* tt_linalg.f90:127-159, dbfun3 bind(c).  Host pointers; res1/res2 ignored. */
 void dbfun3(const ttf_fint* rx1, const ttf_fint* m, const ttf_fint* rx2,

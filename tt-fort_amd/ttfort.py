@@ -719,3 +719,77 @@ def tt_eigb_batch(problems, B, nswp=20, nworkers=8):
         out.append((lam[c * B:(c + 1) * B].copy(), res[res_off[c]:res_off[c] + sizes[c]].copy(), ry,
                     stats[c].as_dict() if hasattr(stats[c], "as_dict") else None))
     return out
+
+
+# ------------------------------------------------------------------ SDV files
+_sig("ttf_sdv_dims", _c_int, C.c_char_p, _c_int, _IP, _IP, _IP, _IP, _I64P, _IP)
+_sig("ttf_dtt_read", _c_int, C.c_char_p, _c_i64, _vp)
+_sig("ttf_ztt_read", _c_int, C.c_char_p, _c_i64, _vp)
+_sig("ttf_dtt_write", _c_int, C.c_char_p, _c_int, _c_int, _IP, _IP, _vp)
+_sig("ttf_ztt_write", _c_int, C.c_char_p, _c_int, _c_int, _IP, _IP, _vp)
+
+
+class SDVError(TTFError):
+    """A failed SDV read or write; `status` is the library's code (the
+    reference readers' info: -1 no such file, -2 not a TT file, -3 other
+    version, > 0 errno; -4 truncated, -5 invalid sizes or buffer, -6 real /
+    complex mismatch)."""
+
+    def __init__(self, msg, status):
+        super().__init__(msg)
+        self.status = status
+
+
+def _sdv_check(status, lib):
+    if status != 0:
+        msg = lib.ttf_last_error().decode(errors="replace")
+        lib.ttf_clear_error()
+        raise SDVError(msg or "SDV call failed with status %d" % status, status)
+
+
+def sdv_read(path):
+    """Read a tensor train from an SDV file (ttio.f90:196-399, dtt_read and
+    ztt_read, chosen by the file's header).  Returns dict(l, m, n, r, cores):
+    n has m - l + 1 entries, r has m - l + 2, core k is a column-major array
+    of shape (r[k], n[k], r[k + 1]), float64 or complex128.  Host code only:
+    no device is needed."""
+    lib = load()
+    fn = os.fsencode(path)
+    l, m, cplx = C.c_int(), C.c_int(), C.c_int()
+    count = _c_i64()
+    _sdv_check(lib.ttf_sdv_dims(fn, 0, C.byref(l), C.byref(m), None, None, C.byref(count), C.byref(cplx)), lib)
+    d = m.value - l.value + 1
+    n = np.zeros(d, dtype=np.int32)
+    r = np.zeros(d + 1, dtype=np.int32)
+    _sdv_check(lib.ttf_sdv_dims(fn, d, None, None, _ip(n), _ip(r), C.byref(count), None), lib)
+    dt = np.complex128 if cplx.value else np.float64
+    x = np.zeros(max(count.value, 1), dtype=dt)
+    rd = lib.ttf_ztt_read if cplx.value else lib.ttf_dtt_read
+    _sdv_check(rd(fn, count.value, x.ctypes.data), lib)
+    cores, p = [], 0
+    for k in range(d):
+        sz = int(r[k]) * int(n[k]) * int(r[k + 1])
+        cores.append(x[p:p + sz].reshape((int(r[k]), int(n[k]), int(r[k + 1])), order="F"))
+        p += sz
+    return dict(l=l.value, m=m.value, n=n, r=r, cores=cores)
+
+
+def sdv_write(path, cores, l=1):
+    """Write cores (arrays of shape (r[k], n[k], r[k + 1]), consecutive
+    ranks matching) to an SDV file as ttio.f90:29-192 (dtt_write, or
+    ztt_write when any core is complex) does.  Replaces an existing file."""
+    lib = load()
+    if not cores:
+        raise SDVError("sdv_write: no cores", -5)
+    cplx = any(np.iscomplexobj(c) for c in cores)
+    dt = np.complex128 if cplx else np.float64
+    shapes = [np.shape(c) for c in cores]
+    for k, s in enumerate(shapes):
+        if len(s) != 3 or (k > 0 and s[0] != shapes[k - 1][2]):
+            raise SDVError("sdv_write: core %d has shape %r; cores must be (r[k], n[k], r[k+1]) with "
+                           "matching ranks" % (k, s), -5)
+    n = np.array([s[1] for s in shapes], dtype=np.int32)
+    r = np.array([shapes[0][0]] + [s[2] for s in shapes], dtype=np.int32)
+    x = np.concatenate([np.asarray(c, dtype=dt).reshape(-1, order="F") for c in cores])
+    wr = lib.ttf_ztt_write if cplx else lib.ttf_dtt_write
+    _sdv_check(wr(os.fsencode(path), l, l + len(cores) - 1, _ip(n), _ip(r), x.ctypes.data), lib)
