@@ -146,6 +146,18 @@ def test_status_codes(tmp_path, golden_dir):
     assert _status(lambda: ttfort.sdv_read(_write_bytes(str(tmp_path / "x3"), bytes(neg)))) == -5
     empty = real[:128] + struct.pack("<2i", 5, 4) + real[136:]
     assert _status(lambda: ttfort.sdv_read(_write_bytes(str(tmp_path / "x4"), empty))) == -5
+    # a corrupt rank whose cores cannot be in the file is reported by the
+    # size query itself, before a caller allocates for it
+    huge = bytearray(real)
+    huge[off_r:off_r + 4] = struct.pack("<i", 0x7fffffff)
+    p = _write_bytes(str(tmp_path / "x6"), bytes(huge))
+    lib = ttfort.load()
+    cnt = C.c_int64(-7)
+    assert lib.ttf_sdv_dims(os.fsencode(p), 0, None, None, None, None, C.byref(cnt), None) == -4
+    lib.ttf_clear_error()
+    assert _status(lambda: ttfort.sdv_read(p)) == -4
+    # a directory as the file to read: errno (EISDIR)
+    assert _status(lambda: ttfort.sdv_read(str(tmp_path))) > 0
     # a directory as the file to write: errno (EISDIR)
     t = ttfort.sdv_read(os.path.join(golden_dir, "Heisen_20.sdv"))
     assert _status(lambda: ttfort.sdv_write(str(tmp_path), t["cores"])) > 0

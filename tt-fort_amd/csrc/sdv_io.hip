@@ -17,6 +17,8 @@
 // i(3:8) are left unset there; here they are zeros.
 #include "ttf_common.h"
 
+#include <sys/stat.h>
+
 #include <cerrno>
 #include <cstring>
 #include <vector>
@@ -77,6 +79,12 @@ int open_dims(const char* path, File& f, Dims& dm) {
     set_error("sdv: cannot open %s: %s", path, std::strerror(e));
     return e == ENOENT ? kMissing : (e > 0 ? e : kMissing);
   }
+  struct stat sb;
+  if (fstat(fileno(f.fp), &sb) != 0 || S_ISDIR(sb.st_mode)) {
+    const int e = S_ISDIR(sb.st_mode) ? EISDIR : errno;
+    set_error("sdv: cannot read %s: %s", path, std::strerror(e));
+    return e > 0 ? e : kShort;
+  }
   unsigned char head[kHeaderBytes];
   if (!read_exact(f.fp, head, sizeof(head))) {
     set_error("sdv: %s is shorter than the 128-byte header", path);
@@ -117,6 +125,13 @@ int open_dims(const char* path, File& f, Dims& dm) {
   if (dm.count < 0) {
     set_error("sdv: %s holds a negative or oversized mode size or rank", path);
     return kBadArg;
+  }
+  // the cores must be in the file: a corrupt size is reported here, before
+  // a caller allocates for it
+  const int64_t have = (int64_t)sb.st_size - (kHeaderBytes + 8 + 4 * (2 * d + 1));
+  if (have / (dm.complex_ ? 16 : 8) < dm.count) {
+    set_error("sdv: %s ends inside its cores (%lld elements expected)", path, (long long)dm.count);
+    return kShort;
   }
   return 0;
 }
