@@ -193,3 +193,41 @@ def test_kind_and_capacity_checks(tmp_path, golden_dir):
     assert lib.ttf_dtt_write(p, 1, 0, ttfort._ip(n), ttfort._ip(r), x.ctypes.data) == -5
     lib.ttf_clear_error()
     assert lib.ttf_dtt_write(p, 1, 2, ttfort._ip(n), ttfort._ip(r), x.ctypes.data) == 0
+
+
+@pytest.mark.parametrize("name", ["Heisen_20.sdv", "cme20_A.sdv"])
+def test_flat_arrays_round_trip(golden_dir, tmp_path, name):
+    """sdv_to_arrays / arrays_to_sdv (python_conv.f90:12-97): 1-based core
+    offsets, the cores back to back, and a file rewritten from the arrays
+    identical to the reference's after the header."""
+    src = os.path.join(golden_dir, name)
+    n, r, d, ps, cr = ttfort.sdv_to_arrays(src)
+    ref = O.read_sdv(src)
+    assert d == len(ref["cores"]) and ps[0] == 1
+    assert all(ps[k + 1] - ps[k] == r[k] * n[k] * r[k + 1] for k in range(d))
+    assert ps[d] - 1 == cr.size
+    assert np.array_equal(cr, np.concatenate(ref["cores"]))
+    out = str(tmp_path / name)
+    ttfort.arrays_to_sdv(n, r, d, ps, cr, out)
+    assert _payload(out) == _payload(src)
+
+
+def test_flat_arrays_with_gaps_and_bad_offsets(tmp_path):
+    rng = np.random.default_rng(4)
+    n, r = [2, 3, 2], [1, 2, 2, 1]
+    cores = _random_tt(rng, n, r, False)
+    sizes = [c.size for c in cores]
+    # cores placed with gaps between them in a larger array
+    ps = [1, 1 + sizes[0] + 5, 1 + sizes[0] + 5 + sizes[1] + 3, 0]
+    ps[3] = ps[2] + sizes[2]
+    cr = np.full(ps[3] - 1 + 4, np.nan)
+    for k in range(3):
+        cr[ps[k] - 1:ps[k] - 1 + sizes[k]] = cores[k].reshape(-1, order="F")
+    path = str(tmp_path / "g.sdv")
+    ttfort.arrays_to_sdv(n, r, 3, ps, cr, path)
+    got = ttfort.sdv_read(path)
+    for a, b in zip(got["cores"], cores):
+        assert np.array_equal(a, b)
+    bad = list(ps)
+    bad[2] = cr.size   # the last core would run past the end of cr
+    assert _status(lambda: ttfort.arrays_to_sdv(n, r, 3, bad, cr, str(tmp_path / "b.sdv"))) == -5

@@ -747,12 +747,9 @@ def _sdv_check(status, lib):
         raise SDVError(msg or "SDV call failed with status %d" % status, status)
 
 
-def sdv_read(path):
-    """Read a tensor train from an SDV file (ttio.f90:196-399, dtt_read and
-    ztt_read, chosen by the file's header).  Returns dict(l, m, n, r, cores):
-    n has m - l + 1 entries, r has m - l + 2, core k is a column-major array
-    of shape (r[k], n[k], r[k + 1]), float64 or complex128.  Host code only:
-    no device is needed."""
+def _sdv_flat(path):
+    """(l, m, n, r, x): sizes and the cores back to back, as the file holds
+    them."""
     lib = load()
     fn = os.fsencode(path)
     l, m, cplx = C.c_int(), C.c_int(), C.c_int()
@@ -762,16 +759,55 @@ def sdv_read(path):
     n = np.zeros(d, dtype=np.int32)
     r = np.zeros(d + 1, dtype=np.int32)
     _sdv_check(lib.ttf_sdv_dims(fn, d, None, None, _ip(n), _ip(r), C.byref(count), None), lib)
-    dt = np.complex128 if cplx.value else np.float64
-    x = np.zeros(max(count.value, 1), dtype=dt)
+    x = np.zeros(max(count.value, 1), dtype=np.complex128 if cplx.value else np.float64)
     rd = lib.ttf_ztt_read if cplx.value else lib.ttf_dtt_read
     _sdv_check(rd(fn, count.value, x.ctypes.data), lib)
+    return l.value, m.value, n, r, x[:count.value]
+
+
+def sdv_read(path):
+    """Read a tensor train from an SDV file (ttio.f90:196-399, dtt_read and
+    ztt_read, chosen by the file's header).  Returns dict(l, m, n, r, cores):
+    n has m - l + 1 entries, r has m - l + 2, core k is a column-major array
+    of shape (r[k], n[k], r[k + 1]), float64 or complex128.  Host code only:
+    no device is needed."""
+    l, m, n, r, x = _sdv_flat(path)
     cores, p = [], 0
-    for k in range(d):
+    for k in range(len(n)):
         sz = int(r[k]) * int(n[k]) * int(r[k + 1])
         cores.append(x[p:p + sz].reshape((int(r[k]), int(n[k]), int(r[k + 1])), order="F"))
         p += sz
-    return dict(l=l.value, m=m.value, n=n, r=r, cores=cores)
+    return dict(l=l, m=m, n=n, r=r, cores=cores)
+
+
+def sdv_to_arrays(path):
+    """The tensor train of an SDV file as the flat arrays of the ttpy
+    converters (dtt_read, then python_conv.f90:12-56, dsdv_to_arrays /
+    zsdv_to_arrays): returns (n, r, d, ps, cr) with ps the 1-based core
+    offsets, ps[0] = 1 and ps[k + 1] = ps[k] + r[k] n[k] r[k + 1], and cr
+    the cores back to back.  n is the file's n(l:m); the reference copies
+    n(1:d) instead, which differs when l != 1."""
+    _, _, n, r, x = _sdv_flat(path)
+    d = len(n)
+    ps = np.ones(d + 1, dtype=np.int32)
+    ps[1:] = 1 + np.cumsum(r[:-1].astype(np.int64) * n * r[1:])
+    return n, r, d, ps, x
+
+
+def arrays_to_sdv(n, r, d, ps, cr, path):
+    """Write flat arrays (n, r, d, ps, cr) as python_conv.f90:59-97
+    (darrays_to_sdv / zarrays_to_sdv, cores l = 1 .. d) followed by
+    dtt_write / ztt_write do; cr complex selects the complex format."""
+    n = np.asarray(n, dtype=np.int32)[:d]
+    r = np.asarray(r, dtype=np.int32)[:d + 1]
+    ps = np.asarray(ps, dtype=np.int64)[:d + 1]
+    cr = np.asarray(cr)
+    sizes = r[:-1].astype(np.int64) * n * r[1:]
+    if np.any(ps[:-1] < 1) or np.any(ps[:-1] - 1 + sizes > cr.size):
+        raise SDVError("arrays_to_sdv: core offsets ps run outside cr", -5)
+    cores = [cr[ps[k] - 1:ps[k] - 1 + sizes[k]].reshape((int(r[k]), int(n[k]), int(r[k + 1])), order="F")
+             for k in range(d)]
+    sdv_write(path, cores, l=1)
 
 
 def sdv_write(path, cores, l=1):
