@@ -9,12 +9,15 @@ to time the reference algorithm on host cores (the ``cpu_baseline`` leg of
 Modules
 -------
 kernels    -- local operator kernels: bfun3 family, phi updates, full local
-              matrix, chop rules, QR, block-Jacobi, Householder GMRES.
-expokit    -- Krylov exp(tA)v (ZGEXPV) with the (6,6) Pade on the Hessenberg
-              (ZGPADM) and the thin zexp_mv layer.
+              matrix, chop rules, QR, block-Jacobi, duchol, Householder
+              GMRES, and the Krylov exp(tA)v (ZGEXPV) with the (6,6) Pade
+              on the Hessenberg (ZGPADM) and the thin zexp_mv layer.
 operators  -- SDV file reader, MPO builders for the benchmark Hamiltonians,
-              small-TT helpers (full vectors, random cores).
-drivers    -- the sweep drivers tt_eigb, tt_amen_solve and ztt_ksl.
+              small-TT helpers (full vectors, random cores), and the exact
+              tensor-DST solution of the Dirichlet Laplacian that pins the
+              AMEn solves.
+drivers    -- the sweep drivers tt_eigb and ztt_ksl, and tt_mvk4 (the
+              approximate TT-matrix by TT-vector product).
 
 Each function cites the reference file:line it follows.  Functions whose
 results no golden vector, closed form or invariant pins are marked

@@ -15,6 +15,10 @@ tt_eigb   -- tt_eigb.f90:110-436, block eigensolver sweep.  The local
              reproducible even between two runs of the reference (PRIMME's
              DYNAMIC switching is timing dependent), so parity is judged on
              eigenvalues and residuals.
+ztt_ksl   -- tt_ksl.f90:431-668, the projector-splitting time step.
+tt_mvk4   -- tt_adapt_als.f90:67-500, the approximate TT-matrix by TT-vector
+             product by alternating projections with enrichment; pinned
+             against exact products on full vectors.
 """
 from __future__ import annotations
 
@@ -233,3 +237,109 @@ def ztt_ksl(d, n, ra, crA, crY0, ry, tau, order=8, typ=2, tol=1e-8):
             continue
         i += dir_
     return cr, ry
+
+
+def tt_mvk4(d, n, m, rx, ra, crA, crX, crY0, ry, eps, kickrank=5, nswp=20):
+    """tt_adapt_als.f90:67-500 (tt_mvk4): y ~ A x for a TT-matrix A (cores
+    (ra_k, n_k, m_k, ra_k+1)) and a TT vector x (cores (rx_k, m_k, rx_k+1)),
+    by alternating projections started from y0 (cores (ry_k, n_k, ry_k+1)).
+
+    As the reference executes it:
+      * y0 is left-orthogonalised by QR (:178-186), building the left
+        interfaces with dphi_left;
+      * each step sets the core to its projection dbfun3(phi_k, A_k,
+        phi_k+1, x_k) and records its relative change (:197-201);
+      * left-to-right steps truncate by SVD with my_chop3 at eps/sqrt(d)
+        (QR when kickrank = 0) and move the factor right (:213-245);
+      * right-to-left steps truncate likewise and enrich the kept basis
+        with min(kickrank, n ry) directions from duchol of the partial
+        product dbfun3_right(A_k, phi_k+1, x_k), then re-orthogonalise
+        (:246-291);
+      * a sweep ends at the last core, and the run stops when the largest
+        change in the sweep is below eps (:297-309).
+    The reference's rmax sizes its work arrays only; no rank is capped
+    (ranks beyond rmax overrun those arrays there), and none is here.
+
+    Returns (cores of y as a flat array, ry, sweeps).
+    """
+    n = [int(v) for v in n[:d]]
+    m = [int(v) for v in m[:d]]
+    rx = [int(v) for v in rx[:d + 1]]
+    ra = [int(v) for v in ra[:d + 1]]
+    ry = [int(v) for v in ry[:d + 1]]
+    eps2 = eps / np.sqrt(d)
+    pa = np.concatenate([[0], np.cumsum([ra[k] * n[k] * m[k] * ra[k + 1] for k in range(d)])])
+    px = np.concatenate([[0], np.cumsum([rx[k] * m[k] * rx[k + 1] for k in range(d)])])
+    A = [np.asarray(crA[pa[k]:pa[k + 1]], dtype=float) for k in range(d)]
+    X = [np.asarray(crX[px[k]:px[k + 1]], dtype=float) for k in range(d)]
+    cr, p = [], 0
+    for k in range(d):
+        sz = ry[k] * n[k] * ry[k + 1]
+        cr.append(np.array(crY0[p:p + sz], dtype=float))
+        p += sz
+    phi = [None] * (d + 1)
+    phi[0] = np.ones(1)
+    phi[d] = np.ones(1)
+
+    def shape(k):
+        return (rx[k], m[k], rx[k + 1], ry[k], n[k], ry[k + 1], ra[k], ra[k + 1])
+
+    # left-orthogonalisation of y0 (:178-186)
+    for k in range(d - 1):
+        rnew = min(ry[k] * n[k], ry[k + 1])
+        Q, R = np.linalg.qr(F(cr[k], ry[k] * n[k], ry[k + 1]), mode="reduced")
+        cr[k + 1] = _flat(R @ F(cr[k + 1], ry[k + 1], n[k + 1] * ry[k + 2]))
+        cr[k] = _flat(Q)
+        ry[k + 1] = rnew
+        phi[k + 1] = K.dphi_left(*shape(k), phi[k], A[k], X[k], cr[k])
+
+    i, direction, swp, err_max = d - 1, -1, 1, 0.0
+    while swp <= nswp:
+        cur = K.dbfun3(*shape(i), phi[i], A[i], phi[i + 1], X[i])        # :197
+        err = np.linalg.norm(cr[i] - cur) / np.linalg.norm(cur)         # :199-201
+        err_max = max(err_max, err)
+        if direction > 0 and i < d - 1:                                 # :213-245
+            M = F(cur, ry[i] * n[i], ry[i + 1])
+            if kickrank > 0:
+                U, s, VT = _svd(M)
+                nn = K.my_chop3(s, eps2)
+                Q, R = U[:, :nn], s[:nn, None] * VT[:nn, :]
+            else:
+                Q, R = np.linalg.qr(M, mode="reduced")
+            rnew = Q.shape[1]
+            cr[i] = _flat(Q)
+            cr[i + 1] = _flat(R @ F(cr[i + 1], ry[i + 1], n[i + 1] * ry[i + 2]))
+            ry[i + 1] = rnew
+            phi[i + 1] = K.dphi_left(*shape(i), phi[i], A[i], X[i], cr[i])
+        if direction < 0 and i > 0:                                     # :246-291
+            M = F(cur, ry[i], n[i] * ry[i + 1])
+            if kickrank > 0:
+                U, s, VT = _svd(M)
+                nn = K.my_chop3(s, eps2)
+                V = VT[:nn, :].T                                        # (n ry2, nn)
+                R = s[:nn, None] * U[:, :nn].T                          # (nn, ry1)
+                z = K.dbfun3_right(*shape(i), A[i], phi[i + 1], X[i])   # (rx1 ra1, n ry2)
+                du, _, mm = K.duchol("N", rx[i] * ra[i], n[i] * ry[i + 1], z,
+                                     min(kickrank, n[i] * ry[i + 1]))
+                W = np.hstack([V, du[:, :mm]])
+                R = np.vstack([R, np.zeros((mm, ry[i]))])
+                Q, R2 = np.linalg.qr(W, mode="reduced")
+                R = R2 @ R
+            else:
+                Q, R = np.linalg.qr(M.T, mode="reduced")
+            rnew = Q.shape[1]
+            cr[i] = _flat(Q.T)                                          # (rnew, n ry2)
+            cr[i - 1] = _flat(F(cr[i - 1], ry[i - 1] * n[i - 1], ry[i]) @ R.T)
+            ry[i] = rnew
+            phi[i] = K.dphi_right(*shape(i), phi[i + 1], A[i], X[i], cr[i])
+        if direction > 0 and i == d - 1:                                # :297-309
+            cr[i] = cur
+            direction, i, swp = -1, i + 1, swp + 1
+            if err_max < eps:
+                break
+            err_max = 0.0
+        if direction < 0 and i == 0:                                    # :310-316
+            cr[i] = cur
+            direction, i = 1, i - 1
+        i += direction
+    return np.concatenate(cr), np.array(ry), swp - 1
