@@ -245,18 +245,18 @@ def tt_mvk4(d, n, m, rx, ra, crA, crX, crY0, ry, eps, kickrank=5, nswp=20):
     by alternating projections started from y0 (cores (ry_k, n_k, ry_k+1)).
 
     As the reference executes it:
-      * y0 is left-orthogonalised by QR (:178-186), building the left
+      * y0 is left-orthogonalised by QR (:197-222), building the left
         interfaces with dphi_left;
       * each step sets the core to its projection dbfun3(phi_k, A_k,
-        phi_k+1, x_k) and records its relative change (:197-201);
+        phi_k+1, x_k) and records its relative change (:243-255);
       * left-to-right steps truncate by SVD with my_chop3 at eps/sqrt(d)
-        (QR when kickrank = 0) and move the factor right (:213-245);
+        (QR when kickrank = 0) and move the factor right (:269-341);
       * right-to-left steps truncate likewise and enrich the kept basis
         with min(kickrank, n ry) directions from duchol of the partial
         product dbfun3_right(A_k, phi_k+1, x_k), then re-orthogonalise
-        (:246-291);
+        (:349-426);
       * a sweep ends at the last core, and the run stops when the largest
-        change in the sweep is below eps (:297-309).
+        change in the sweep is below eps (:428-447).
     The reference's rmax sizes its work arrays only; no rank is capped
     (ranks beyond rmax overrun those arrays there), and none is here.
 
@@ -284,7 +284,7 @@ def tt_mvk4(d, n, m, rx, ra, crA, crX, crY0, ry, eps, kickrank=5, nswp=20):
     def shape(k):
         return (rx[k], m[k], rx[k + 1], ry[k], n[k], ry[k + 1], ra[k], ra[k + 1])
 
-    # left-orthogonalisation of y0 (:178-186)
+    # left-orthogonalisation of y0 (:197-222)
     for k in range(d - 1):
         rnew = min(ry[k] * n[k], ry[k + 1])
         Q, R = np.linalg.qr(F(cr[k], ry[k] * n[k], ry[k + 1]), mode="reduced")
@@ -295,10 +295,10 @@ def tt_mvk4(d, n, m, rx, ra, crA, crX, crY0, ry, eps, kickrank=5, nswp=20):
 
     i, direction, swp, err_max = d - 1, -1, 1, 0.0
     while swp <= nswp:
-        cur = K.dbfun3(*shape(i), phi[i], A[i], phi[i + 1], X[i])        # :197
-        err = np.linalg.norm(cr[i] - cur) / np.linalg.norm(cur)         # :199-201
+        cur = K.dbfun3(*shape(i), phi[i], A[i], phi[i + 1], X[i])        # :243
+        err = np.linalg.norm(cr[i] - cur) / np.linalg.norm(cur)         # :253-255
         err_max = max(err_max, err)
-        if direction > 0 and i < d - 1:                                 # :213-245
+        if direction > 0 and i < d - 1:                                 # :269-341
             M = F(cur, ry[i] * n[i], ry[i + 1])
             if kickrank > 0:
                 U, s, VT = _svd(M)
@@ -311,7 +311,7 @@ def tt_mvk4(d, n, m, rx, ra, crA, crX, crY0, ry, eps, kickrank=5, nswp=20):
             cr[i + 1] = _flat(R @ F(cr[i + 1], ry[i + 1], n[i + 1] * ry[i + 2]))
             ry[i + 1] = rnew
             phi[i + 1] = K.dphi_left(*shape(i), phi[i], A[i], X[i], cr[i])
-        if direction < 0 and i > 0:                                     # :246-291
+        if direction < 0 and i > 0:                                     # :349-426
             M = F(cur, ry[i], n[i] * ry[i + 1])
             if kickrank > 0:
                 U, s, VT = _svd(M)
@@ -332,13 +332,13 @@ def tt_mvk4(d, n, m, rx, ra, crA, crX, crY0, ry, eps, kickrank=5, nswp=20):
             cr[i - 1] = _flat(F(cr[i - 1], ry[i - 1] * n[i - 1], ry[i]) @ R.T)
             ry[i] = rnew
             phi[i] = K.dphi_right(*shape(i), phi[i + 1], A[i], X[i], cr[i])
-        if direction > 0 and i == d - 1:                                # :297-309
+        if direction > 0 and i == d - 1:                                # :428-447
             cr[i] = cur
             direction, i, swp = -1, i + 1, swp + 1
             if err_max < eps:
                 break
             err_max = 0.0
-        if direction < 0 and i == 0:                                    # :310-316
+        if direction < 0 and i == 0:                                    # :448-461
             cr[i] = cur
             direction, i = 1, i - 1
         i += direction
