@@ -8,7 +8,7 @@ C-ABI.  Pins:
     (complex, from test_ksl_cme.f90) read to the same bits as the oracle's
     independent reader, and rewrite to the same bytes after the header;
   * the header the writers produce, built field by field from ttio.f90:10-17
-    and the writers' assignments (ttio.f90:72-74, 155-158);
+    and the writers' assignments (ttio.f90:73-74, 154-156);
   * round trips over ragged ranks, a single core, l != 1 and complex cores;
   * every status code, including the reference readers' -1 / -2 / -3.
 """

@@ -27,7 +27,7 @@ namespace ttf {
 namespace {
 
 constexpr int kHeaderBytes = 128;
-constexpr int32_t kVersion = 1;    // ver(1) the readers accept (ttio.f90:5)
+constexpr int32_t kVersion = 1;    // ver(1) the readers accept (ttio.f90:8)
 constexpr int32_t kTtSize = 1023;  // inf(1) written by the reference
 
 // Status values, after the info codes of dtt_read (ttio.f90:196-297)
