@@ -190,7 +190,9 @@ int ttf_dgram(int N, int p, int q, const double* X, int64_t ldx, const double* Y
 typedef struct {
   int sweeps;            /* sweeps performed */
   long total_matvecs;    /* local matvecs (vectors) over the run */
-  double last_ermax;     /* max relative change of sum(lambda) in the last sweep */
+  double last_ermax;     /* max relative decrease of sum(lambda) over the last
+                            completed sweep (tt_eigb.f90:294-295), the quantity
+                            the stopping rule compares with eps */
   int max_rank;          /* largest TT rank at exit */
   double seconds;        /* wall time of the call */
   double solve_seconds;  /* local eigensolves */
@@ -200,6 +202,11 @@ typedef struct {
   double sweep_seconds[64];   /* wall time of each sweep */
   double sweep_flops[64];     /* algorithmic local-matvec flops of each sweep */
   long sweep_matvecs[64];     /* local matvecs (vectors) of each sweep */
+  double sweep_ermax[64];     /* ermax of each sweep */
+  double sweep_lambda_sum[64];   /* sum of the B eigenvalues at the end of each sweep */
+  int converged;              /* 1 if a sweep ended with ermax < eps (the
+                                 reference's exit, tt_eigb.f90:401-403), 0 if
+                                 the run stopped at nswp sweeps */
 } ttf_eigb_stats;
 
 /* Block eigensolver in TT format: the B lowest eigenpairs of the symmetric
@@ -234,6 +241,34 @@ int ttf_tt_eigb_batch(int nchains, const int* d, const int* n_all, const int* ra
                       const double* eps, const int* rmax, int B, int nswp, int nworkers, double* lambda_all,
                       double* result_all, const int64_t* result_off, int64_t* result_size, ttf_eigb_stats* stats,
                       int* status);
+
+/* ------------------------------------------- integer decisions (test hooks)
+ * The rank decisions of the truncations, callable on their own so that they
+ * can be compared bit for bit with a restatement on identical inputs. */
+/* Rank of the relative chop rule my_chop3(n, s, eps) (tt_adapt_als.f90:40-61
+ * == matrix_util.f90:254-275) as ttf_tt_eigb applies it to the singular
+ * values of a block core: the reference's loop, including its "+1" and the
+ * strict comparison.  Host arrays; needs no GPU.  -1 on bad arguments. */
+int ttf_my_chop3(int n, const double* s, double eps);
+/* Residual-based truncation rank of tt_amen_solve (tt_adapt_als.f90:819-833,
+ * :937-950) on a given sequence of candidate residuals: errs[c] is the
+ * relative local residual of the core truncated to rank c, c = 1..k-1
+ * (errs[0] unused).  Scans c = k-1 down to 1 and returns one more than the
+ * first c with errs[c] > 2 res_new and errs[c] > eps2, or 1.  The same scan
+ * ttf_tt_amen_solve runs (in batches of 8 candidates per block matvec).
+ * Host arrays; needs no GPU.  -1 on bad arguments. */
+int ttf_residual_rank_rule(int k, const double* errs, double res_new, double eps2);
+/* The whole residual-based rank search of one local step on the device, as
+ * ttf_tt_amen_solve runs it: SVD of the local core x (rx1, n, rx2) unfolded
+ * as (rx1 n) x rx2 for dir = +1 or rx1 x (n rx2) for dir = -1, candidates
+ * formed from the singular factors, multiplied by the local operator
+ * (phi1, A, phi2 as ttf_dbfun3), residuals against rhs, and the scan above.
+ * *rank receives the rank before any rank cap; errs (host, optional,
+ * min(rows, cols) entries) the residual of each candidate evaluated and -1
+ * for the others.  Device pointers except errs; synchronous. */
+int ttf_amen_truncation_rank(int dir, int rx1, int n, int rx2, int ra1, int ra2, const double* phi1,
+                             const double* A, const double* phi2, const double* x, const double* rhs,
+                             double res_new, double eps2, int* rank, double* errs);
 
 /* ------------------------------------------------- AMEn building blocks */
 /* Block-Jacobi preconditioner of one local step (tt_linalg.f90:363-495,

@@ -13,7 +13,12 @@ Pinning (see tests/test_oracle_kernels.py):
     the local matrix, against each other (phi_left of x with y == y' B x), and
     against the exact Heisenberg spectrum from the reference's own MPO
     fixture Heisen_20.sdv.
-  * chop rules are checked against hand-worked singular-value cases.
+  * the bfun3 halves (dbfun3_right, dbfun3_left_intended) by contracting
+    their open interface with phi1 / phi2 back into the Kronecker-pinned
+    bfun3, with enough rows that the contraction is injective.
+  * chop rules are checked against hand-worked singular-value cases; the
+    residual-rank scan against its closed form (one more than the largest
+    candidate above the bound) and against the residuals themselves.
   * QR/SVD against orthogonality and reconstruction invariants.
   * GMRES against dense solves; block-Jacobi against block inverses.
 """
@@ -23,10 +28,10 @@ import numpy as np
 
 __all__ = [
     "F", "dtransp", "dperm1324",
-    "dbfun3", "zbfun3", "zbfun3_transp", "dbfun3_right", "dbfun3_left",
+    "dbfun3", "zbfun3", "zbfun3_transp", "dbfun3_right", "dbfun3_left_intended",
     "dphi_left", "dphi_right", "zphi_left", "zphi_right",
     "dphi2_left", "dphi2_right", "d2d_fullmat", "local_matrix_kron",
-    "my_chop3", "chop", "dqr", "djac_gen", "djac_apply", "dgmresr_hh",
+    "my_chop3", "chop", "residual_rank", "amen_truncation_rank", "dqr", "djac_gen", "djac_apply", "dgmresr_hh",
     "bfun3_flops",
 ]
 
@@ -102,23 +107,6 @@ def dbfun3_right(rx1, m, rx2, ry1, n, ry2, ra1, ra2, A, phi2, x):
     res1 = dtransp(rx1, m * ra2 * ry2, _flat(res1))                    # :228
     res2 = F(A, ra1 * n, m * ra2) @ F(res1, m * ra2, ry2 * rx1)        # :232
     return dtransp(ra1 * n * ry2, rx1, _flat(res2))                    # :237
-
-
-def dbfun3_left(rx1, m, rx2, ry1, n, ry2, ra1, ra2, phi1, A, x):
-    """tt_linalg.f90:242-256, restated as the reference executes it.
-
-    Output y of size (ry1, n, rx2, ra2).  NOTE: the reference computes the
-    transposed copy res1 = phi1^T (:249) but then passes the untransposed
-    `phi1` to the GEMM at :250 with leading dimension ra1*ry1.  For ra1 > 1
-    this reads phi1 in a different index order than the comment intends.
-    The restatement keeps the reference's behaviour; `dbfun3_left_intended`
-    gives the contraction the comments describe.
-    """
-    res2 = F(phi1, ra1 * ry1, rx1) @ F(x, rx1, m * rx2)                # :250
-    res1 = dperm1324(ra1, ry1, m, rx2, _flat(res2))                    # :251
-    res2 = dperm1324(ra1, n, m, ra2, A)                                # :252
-    y = F(res1, ra1 * m, ry1 * rx2).T @ F(res2, ra1 * m, n * ra2)      # :253
-    return dperm1324(ry1, rx2, n, ra2, _flat(y))                       # :254
 
 
 def dbfun3_left_intended(rx1, m, rx2, ry1, n, ry2, ra1, ra2, phi1, A, x):
@@ -241,6 +229,40 @@ def my_chop3(s, eps):
             break
         i -= 1
     return min(i + 1, n)
+
+
+def residual_rank(err_of, k, res_new, eps2):
+    """tt_adapt_als.f90:819-833 (forward) and :937-950 (backward): the
+    residual-based truncation rank.  err_of(c) is the relative local residual
+    of the core truncated to rank c.  The DO loop runs nn = k-1 down to 1
+    and exits at the first nn with err > 2 res_new and err > eps2; after a
+    loop that runs out nn is 0; the rank is nn + 1."""
+    nn = k - 1
+    while nn >= 1:
+        err = err_of(nn)
+        if err > res_new * 2.0 and err > eps2:
+            break
+        nn -= 1
+    return max(nn, 0) + 1
+
+
+def amen_truncation_rank(direction, rx1, n, rx2, ra1, ra2, phi1, A, phi2, x, rhs, res_new, eps2):
+    """One local step's residual-based rank search (tt_adapt_als.f90:809-833
+    for direction +1, :928-950 for -1): SVD of the local core unfolded as
+    (rx1 n) x rx2 (forward) or rx1 x (n rx2) (backward), candidates
+    U(:, :c) diag(s[:c]) V(:, :c)^T, their local residuals through dbfun3,
+    and the scan of `residual_rank`.  Returns (rank, errs) with errs[c] the
+    residual of candidate c for every c = 1..k-1 (errs[0] = -1)."""
+    rows, cols = (rx1 * n, rx2) if direction > 0 else (rx1, n * rx2)
+    U, s, VT = np.linalg.svd(F(x, rows, cols), full_matrices=False)
+    k = s.size
+    shape = (rx1, n, rx2, rx1, n, rx2, ra1, ra2)
+    nrhs = np.linalg.norm(rhs)
+    errs = np.full(k, -1.0)
+    for c in range(1, k):
+        cand = (U[:, :c] * s[:c]) @ VT[:c, :]
+        errs[c] = np.linalg.norm(dbfun3(*shape, phi1, A, phi2, _flat(cand)) - rhs) / nrhs
+    return residual_rank(lambda c: errs[c], k, res_new, eps2), errs
 
 
 def chop(s, tol=None, rmax=None):
@@ -623,14 +645,6 @@ def zsfun(rx1, ry1, ra, rx2, ry2, phi1, phi2, S):
     res1 = F(S, rx1, rx2) @ F(phi2, rx2, ra * ry2)                       # (rx1, ra*ry2)
     res1 = np.reshape(res1, (rx1 * ra, ry2), order="F")
     return _flat(F(phi1, ry1, rx1 * ra) @ res1)
-
-
-def zsfun_transp(rx1, ry1, ra, rx2, ry2, phi1, phi2, Sy):
-    """The adjoint of zsfun (what normest's transposed product must apply)."""
-    P1 = F(phi1, ry1, rx1, ra)
-    P2 = F(phi2, rx2, ra, ry2)
-    Y = F(Sy, ry1, ry2)
-    return _flat(np.einsum("cba,dae,ce->bd", np.conj(P1), np.conj(P2), Y))
 
 
 def onenorm_dense(matvec, n, dtype=np.complex128):

@@ -203,3 +203,56 @@ def test_amen_mixed_precision(ttlib, prec):
     assert st["last_res_max"] < eps
     assert np.linalg.norm(x - x_ref) <= 1e-6 * np.linalg.norm(x_ref)
     assert ttops.residual(crA, ra, nn, res, rx_out, crY, ry) <= 1e-6
+
+
+@pytest.mark.parametrize("direction,rx1,n,rx2", [(1, 6, 4, 20), (-1, 20, 4, 6)])
+def test_amen_truncation_rank_matches_oracle(ttlib, direction, rx1, n, rx2):
+    """The device residual-rank search of one local step (SVD, candidate
+    cores, block matvecs, scan) picks the oracle's rank on the same inputs,
+    and the candidate residuals it evaluated match the oracle's.  The bound
+    2 res_new is put halfway (geometrically) between two candidate residuals
+    that lie in the second batch of the device scan, so the decision is
+    clear of rounding; 19 candidates span three batches of 8."""
+    import ttfort
+    rng = np.random.default_rng(31 + direction)
+    ra1, ra2 = 3, 2
+    phi1 = rng.standard_normal(rx1 * rx1 * ra1)
+    A = rng.standard_normal(ra1 * n * n * ra2)
+    phi2 = rng.standard_normal(rx2 * ra2 * rx2)
+    rows, cols = (rx1 * n, rx2) if direction > 0 else (rx1, n * rx2)
+    k = min(rows, cols)
+    Uq, _ = np.linalg.qr(rng.standard_normal((rows, k)))
+    Vq, _ = np.linalg.qr(rng.standard_normal((cols, k)))
+    s = 10.0 ** (-0.5 * np.arange(k))
+    x = ((Uq * s) @ Vq.T).reshape(-1, order="F")
+    shape = (rx1, n, rx2, rx1, n, rx2, ra1, ra2)
+    rhs = K.dbfun3(*shape, phi1, A, phi2, x)
+    _, errs_o = K.amen_truncation_rank(direction, rx1, n, rx2, ra1, ra2, phi1, A, phi2, x, rhs, 1.0, 0.0)
+    for j in (12, 9, 3):
+        bound = np.sqrt(errs_o[j] * errs_o[j + 1])
+        res_new, eps2 = bound / 2, 1e-14
+        r_o, _ = K.amen_truncation_rank(direction, rx1, n, rx2, ra1, ra2, phi1, A, phi2, x, rhs, res_new, eps2)
+        r_d, errs_d = ttfort.amen_truncation_rank(direction, rx1, n, rx2, ra1, ra2, phi1, A, phi2, x, rhs,
+                                                  res_new, eps2)
+        assert r_d == r_o == j + 1, (r_d, r_o, j)
+        seen = errs_d >= 0
+        assert seen[j] and seen[k - 1] and not seen[0]
+        assert np.allclose(errs_d[seen], errs_o[seen], rtol=1e-6, atol=1e-13), (errs_d[seen], errs_o[seen])
+
+
+def test_amen_rank_cap_without_rmax(ttlib):
+    """rmax <= 0 caps the ranks at the library's capacity (256 here): with a
+    right-hand side of rank 256 on the middle bond, whose solution needs
+    more, the truncation keeps 256 columns of a 576 x 256 unfolding and the
+    kick must add none beyond the cap."""
+    import ttfort
+    d, n = 4, 24
+    crA, ra, nn = mpo.laplace_dd(d, n)
+    ry = [1, n, 256, n, 1]
+    rng = np.random.default_rng(41)
+    crY = np.concatenate([rng.standard_normal(ry[k] * n * ry[k + 1]) for k in range(d)])
+    rx = list(ry)
+    crX = np.concatenate([rng.standard_normal(rx[k] * n * rx[k + 1]) for k in range(d)])
+    res, rx_out, st = ttfort.tt_amen_solve(nn, ra, crA, ry, crY, rx, crX, 1e-12, kickrank=4, prec="n", nswp=2,
+                                           rmax=0)
+    assert max(rx_out) == 256 and st["max_rank"] == 256, (list(rx_out), st["max_rank"])

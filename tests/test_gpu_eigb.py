@@ -4,10 +4,10 @@ Pins: the lowest eigenvalues of the open Heisenberg XXX chain from exact
 diagonalisation (tests/golden/heisenberg_ed.json, made by
 make_heisenberg_ed.py), the reference's own MPO fixture Heisen_20.sdv as the
 operator, and the FP64 residual ||H x_b - lambda_b x_b|| of the returned
-block TT evaluated on full vectors.  Tolerances: with full TT ranks the
-eigenvalues match to 1e-9 relative; with the truncation of the
-configuration (rmax 40, tol 1e-6, d = 20) to 1e-6 relative, the accuracy
-the truncation tolerance allows.
+block TT, on full vectors and in TT format.  Tolerances: with full TT ranks
+and tol 1e-12 the eigenvalues match to 1e-10 relative; with the truncation
+of the configuration (rmax 40, tol 1e-6, d = 20) to 5e-6 relative, the
+accuracy the truncation tolerance allows.
 """
 import json
 import os
@@ -39,23 +39,50 @@ def _initial(rng, d, n, r0, B):
     return np.concatenate(cores), ry
 
 
-def test_eigb_heisenberg_d10_full_rank(ttlib, golden_dir):
-    """rmax 160 = 2^5 * B admits the exact block eigenvectors at d = 10."""
+@pytest.mark.parametrize("d,rmax", [(10, 160), (12, 320)])
+def test_eigb_heisenberg_full_rank_1e10(ttlib, golden_dir, d, rmax):
+    """rmax = 2^(d/2) * B admits the exact block eigenvectors, so with
+    tol 1e-12 the eigenvalues must meet the 1e-10 relative bar against exact
+    diagonalisation (tests/golden/heisenberg_ed.json, Lanczos to 1e-14)."""
     import ttfort
+    import ttops
     ed = json.load(open(os.path.join(golden_dir, "heisenberg_ed.json")))
-    d, B = 10, 5
+    B = 5
     cores, ra, n = O.heisenberg_mpo(d)
     crA = O.flatten_cores(cores)
-    rng = np.random.default_rng(1)
+    rng = np.random.default_rng(d)
     crY, ry = _initial(rng, d, n, 4, B)
-    lam, res, ry_out, st = ttfort.tt_eigb(n, ra, crA, crY, ry, 1e-10, 160, B, nswp=10)
-    assert np.allclose(lam, ed["10"][:B], rtol=1e-9, atol=1e-9), (lam, ed["10"][:B])
+    lam, res, ry_out, st = ttfort.tt_eigb(n, ra, crA, crY, ry, 1e-12, rmax, B, nswp=10)
+    ref = np.array(ed[str(d)][:B])
+    assert np.all(np.abs(lam - ref) <= 1e-10 * np.abs(ref)), (lam - ref) / np.abs(ref)
+    assert st["converged"] == 1 and st["last_ermax"] < 1e-12, st["sweep_ermax"]
     X = _block_full(_split(res, n, ry_out), n, ry_out)
     H = O.mpo_to_dense(cores, ra, n, n)
     for b in range(B):
         r = np.linalg.norm(H @ X[:, b] - lam[b] * X[:, b]) / np.linalg.norm(X[:, b])
-        assert r < 1e-7, (b, r)
-    assert np.allclose(X.T @ X, np.eye(B), atol=1e-9)
+        assert r < 1e-8, (b, r)
+    assert np.allclose(X.T @ X, np.eye(B), atol=1e-10)
+    # the TT-format residuals used at sizes where full vectors are out of reach
+    rt, G = ttops.block_eig_residuals(crA, ra, n, res, ry_out, lam)
+    for b in range(B):
+        r = np.linalg.norm(H @ X[:, b] - lam[b] * X[:, b]) / np.linalg.norm(X[:, b])
+        assert abs(rt[b] - r) <= 1e-12 + 1e-6 * r
+    assert np.allclose(G, X.T @ X, atol=1e-12)
+
+
+def test_eigb_stats_report_unconverged_runs(ttlib):
+    """A run stopped by nswp reports the ermax of its last sweep and
+    converged = 0 (not the reset value 0.0)."""
+    import ttfort
+    d, B = 10, 3
+    cores, ra, n = O.heisenberg_mpo(d)
+    rng = np.random.default_rng(3)
+    crY, ry = _initial(rng, d, n, 4, B)
+    lam, res, ry_out, st = ttfort.tt_eigb(n, ra, O.flatten_cores(cores), crY, ry, 1e-14, 16, B, nswp=2)
+    assert st["sweeps"] == 2 and st["nrec"] == 2
+    assert st["converged"] == 0
+    assert st["last_ermax"] == st["sweep_ermax"][1]
+    assert st["last_ermax"] != 0.0
 
 
 def test_eigb_reference_fixture_d20(ttlib, golden_dir):

@@ -209,3 +209,105 @@ def test_heisenberg_ed_golden_consistent(golden_dir):
     w = np.linalg.eigvalsh(0.5 * (H + H.T))[:8]
     assert np.allclose(w, ed["8"], atol=1e-10)
     assert np.isclose(ed["20"][0], -8.682473334399, atol=1e-11)
+
+
+# shapes with ra > 1 and enough rows on the contracted side (ry1 >= rx1 ra1,
+# ry2 >= rx2 ra2) that contracting with a random interface loses nothing
+HALF_SHAPES = [
+    # rx1, m, rx2, ry1, n, ry2, ra1, ra2
+    (3, 2, 4, 7, 2, 9, 2, 2),
+    (2, 3, 2, 9, 3, 7, 4, 3),
+    (1, 2, 3, 3, 2, 3, 3, 1),
+]
+
+
+@pytest.mark.parametrize("shape", HALF_SHAPES)
+def test_bfun3_right_half_contracts_to_bfun3(shape):
+    """sum_{b1,a1} phi1(c1,b1,a1) right(b1,a1,i,c2) == (B x)(c1,i,c2), with B
+    the Kronecker-built local matrix; phi1 of full column rank makes the
+    check pin every entry of the half product."""
+    rx1, m, rx2, ry1, n, ry2, ra1, ra2 = shape
+    rng = np.random.default_rng(11)
+    t = _rand(rng, shape)
+    half = K.dbfun3_right(*shape, t["A"], t["phi2"], t["x"])
+    assert half.size == rx1 * ra1 * n * ry2
+    P1 = K.F(t["phi1"], ry1, rx1 * ra1)
+    assert np.linalg.matrix_rank(P1) == rx1 * ra1
+    y = (P1 @ K.F(half, rx1 * ra1, n * ry2)).reshape(-1, order="F")
+    B = K.local_matrix_kron(ry1, n, ry2, rx1, m, rx2, ra1, ra2, t["phi1"], t["A"], t["phi2"])
+    assert np.allclose(y, B @ t["x"], rtol=1e-13, atol=1e-13 * np.abs(B).sum())
+    # a transposed A (i <-> j) in the half product would fail the same check
+    At = np.transpose(K.F(t["A"], ra1, n, m, ra2), (0, 2, 1, 3)).reshape(-1, order="F")
+    if n == m:
+        bad = K.dbfun3_right(*shape, At, t["phi2"], t["x"])
+        ybad = (P1 @ K.F(bad, rx1 * ra1, n * ry2)).reshape(-1, order="F")
+        assert not np.allclose(ybad, B @ t["x"], rtol=1e-6)
+
+
+@pytest.mark.parametrize("shape", HALF_SHAPES)
+def test_bfun3_left_half_contracts_to_bfun3(shape):
+    """sum_{b2,a2} left(c1,i,b2,a2) phi2(b2,a2,c2) == (B x)(c1,i,c2)."""
+    rx1, m, rx2, ry1, n, ry2, ra1, ra2 = shape
+    rng = np.random.default_rng(12)
+    t = _rand(rng, shape)
+    half = K.dbfun3_left_intended(*shape, t["phi1"], t["A"], t["x"])
+    assert half.size == ry1 * n * rx2 * ra2
+    P2 = K.F(t["phi2"], rx2 * ra2, ry2)
+    y = (K.F(half, ry1 * n, rx2 * ra2) @ P2).reshape(-1, order="F")
+    B = K.local_matrix_kron(ry1, n, ry2, rx1, m, rx2, ra1, ra2, t["phi1"], t["A"], t["phi2"])
+    assert np.allclose(y, B @ t["x"], rtol=1e-13, atol=1e-13 * np.abs(B).sum())
+
+
+def _rank_closed_form(errs, res_new, eps2):
+    """One more than the largest candidate c in 1..k-1 whose residual exceeds
+    both bounds; 1 if there is none."""
+    above = [c for c in range(1, len(errs)) if errs[c] > 2 * res_new and errs[c] > eps2]
+    return (max(above) if above else 0) + 1
+
+
+def test_residual_rank_rule_closed_form():
+    rng = np.random.default_rng(13)
+    for _ in range(2000):
+        k = int(rng.integers(1, 40))
+        errs = np.concatenate([[-1.0], np.sort(10.0 ** rng.uniform(-12, 0, k - 1))[::-1]])
+        if rng.random() < 0.3:
+            rng.shuffle(errs[1:])   # non-monotone residual sequences too
+        res_new = 10.0 ** rng.uniform(-12, -1)
+        eps2 = 10.0 ** rng.uniform(-12, -1)
+        got = K.residual_rank(lambda c: errs[c], k, res_new, eps2)
+        assert got == _rank_closed_form(errs, res_new, eps2)
+    # exact ties: a residual equal to 2 res_new (or to eps2) does not stop
+    errs = np.array([-1.0, 4.0, 2.0, 1.0, 0.5])
+    assert K.residual_rank(lambda c: errs[c], 5, 1.0, 0.1) == 2      # 2.0 == 2*1.0 does not stop
+    assert K.residual_rank(lambda c: errs[c], 5, 0.5, 2.0) == 2      # 2.0 == eps2 does not stop
+    assert K.residual_rank(lambda c: errs[c], 5, 0.49, 0.1) == 4      # 1.0 > 0.98 stops at 3
+    assert K.residual_rank(lambda c: errs[c], 1, 0.0, 0.0) == 1      # nothing to scan
+
+
+def test_amen_truncation_rank_properties():
+    """The chosen rank's truncation meets the bound, one rank less does not,
+    and the full-rank candidate reproduces the core's own residual."""
+    rng = np.random.default_rng(14)
+    for direction in (1, -1):
+        rx1, n, rx2, ra1, ra2 = 5, 3, 6, 2, 3
+        phi1 = rng.standard_normal(rx1 * rx1 * ra1)
+        A = rng.standard_normal(ra1 * n * n * ra2)
+        phi2 = rng.standard_normal(rx2 * ra2 * rx2)
+        rows, cols = (rx1 * n, rx2) if direction > 0 else (rx1, n * rx2)
+        k = min(rows, cols)
+        Uq, _ = np.linalg.qr(rng.standard_normal((rows, k)))
+        Vq, _ = np.linalg.qr(rng.standard_normal((cols, k)))
+        s = 10.0 ** -np.arange(k)
+        x = ((Uq * s) @ Vq.T).reshape(-1, order="F")
+        shape = (rx1, n, rx2, rx1, n, rx2, ra1, ra2)
+        rhs = K.dbfun3(*shape, phi1, A, phi2, x)
+        for res_new in (1e-9, 1e-5, 1e-2):
+            r, errs = K.amen_truncation_rank(direction, rx1, n, rx2, ra1, ra2, phi1, A, phi2, x, rhs, res_new, 1e-12)
+            bound = max(2 * res_new, 1e-12)
+            assert 1 <= r <= k
+            if r < k:
+                assert errs[r] <= bound
+            if r > 1:
+                assert errs[r - 1] > bound
+            # graded singular values: residuals fall with the rank
+            assert np.all(np.diff(errs[1:]) < 0)

@@ -104,9 +104,117 @@ void gemm_nn(int M, int N, int K, double alpha, const double* A, int64_t lda, co
 
 }  // namespace
 
+// Stopping test of the residual-based truncation (tt_adapt_als.f90:829-831,
+// :945-947): a candidate rank ends the downward scan when its relative local
+// residual exceeds both twice the solve's residual and eps2.
+static inline bool residual_rank_stop(double e, double res_new, double eps2) {
+  return e > res_new * 2.0 && e > eps2;
+}
+
+// The scan itself (tt_adapt_als.f90:819-833 forward, :937-950 backward):
+// candidates k-1, k-2, ..., 1 from the top; the rank is one more than the
+// first candidate that stops the scan, and 1 when none does (the DO loop
+// ends with nn = 0).  eval(c0, nc, e) fills e[t] with the relative residual
+// of candidate c0 - t for t < nc; candidates are asked for in batches of up
+// to `batch`, in scan order, so the decision is the serial loop's.
+template <class Eval>
+static int residual_rank_scan(int k, int batch, double res_new, double eps2, Eval&& eval) {
+  std::vector<double> e(std::max(batch, 1));
+  for (int c0 = k - 1; c0 >= 1; c0 -= batch) {
+    const int nc = std::min(batch, c0);
+    eval(c0, nc, e.data());
+    for (int t = 0; t < nc; ++t)
+      if (residual_rank_stop(e[t], res_new, eps2)) return c0 - t + 1;
+  }
+  return 1;
+}
+
+constexpr int kCand = 8;   // candidate ranks per block matvec
+
+// Residual-based truncation rank of the local solution on the device.
+// dir > 0: candidate c is U(:, 0:c) Rm(0:c, :) with U (mrow x k) the left
+// singular vectors and Rm = S V^T (k x ncol, leading dimension k).
+// dir < 0: candidate c is Rm(:, 0:c) U(:, 0:c)^T with Rm = U S (mrow x k)
+// and U the right singular vectors (ncol x k).
+// cand and candy hold kCand local vectors each, scr at least kCand doubles.
+// errs (host, k entries, optional) receives the residual of every candidate
+// evaluated and -1 for the others.
+static int residual_rank_search(const LocalOp& op, int dir, int mrow, int ncol, int k, const double* U,
+                                const double* Rm, const double* rhs, double norm_rhs, double res_new, double eps2,
+                                double* cand, double* candy, double* scr, long* matvecs, double* flops,
+                                double* errs, hipStream_t st) {
+  const int64_t N = (int64_t)mrow * ncol;
+  if (errs)
+    for (int c = 0; c < k; ++c) errs[c] = -1.0;
+  return residual_rank_scan(k, kCand, res_new, eps2, [&](int c0, int nc, double* e) {
+    for (int t = 0; t < nc; ++t) {
+      if (dir > 0)
+        gemm_nn(mrow, ncol, c0 - t, 1.0, U, mrow, Rm, k, 0.0, cand + (size_t)N * t, mrow, st);
+      else
+        gemm_nn(mrow, ncol, c0 - t, 1.0, Rm, mrow, U, ncol, 0.0, cand + (size_t)N * t, mrow, st, 'N', 'T');
+    }
+    op.apply(nc, cand, N, candy, N, st);
+    if (matvecs) *matvecs += nc;
+    if (flops) *flops += bfun3_flops(op.s) * nc;
+    hipLaunchKernelGGL(resid_norms_kernel, dim3(nc), dim3(256), 0, st, N, candy, N, rhs, scr);
+    std::vector<double> hr(nc);
+    read_back(hr.data(), scr, sizeof(double) * nc, st);
+    for (int t = 0; t < nc; ++t) {
+      e[t] = std::sqrt(std::max(hr[t], 0.0)) / norm_rhs;
+      if (errs) errs[c0 - t] = e[t];
+    }
+  });
+}
+
 }  // namespace ttf
 
 using namespace ttf;
+
+// Test hooks: the integer decisions of the truncations, callable alone.
+extern "C" int ttf_residual_rank_rule(int k, const double* errs, double res_new, double eps2) {
+  if (k < 1 || !errs) return -1;
+  return residual_rank_scan(k, kCand, res_new, eps2, [&](int c0, int nc, double* e) {
+    for (int t = 0; t < nc; ++t) e[t] = errs[c0 - t];
+  });
+}
+
+extern "C" int ttf_amen_truncation_rank(int dir, int rx1, int n, int rx2, int ra1, int ra2, const double* phi1,
+                                        const double* A, const double* phi2, const double* x, const double* rhs,
+                                        double res_new, double eps2, int* rank, double* errs) {
+  return guarded([&] {
+    TTF_REQUIRE(dir == 1 || dir == -1, "ttf_amen_truncation_rank: dir=%d", dir);
+    TTF_REQUIRE(rx1 >= 1 && n >= 1 && rx2 >= 1 && ra1 >= 1 && ra2 >= 1 && rank,
+                "ttf_amen_truncation_rank: bad arguments");
+    hipStream_t st = stream();
+    const int mrow = dir > 0 ? rx1 * n : rx1, ncol = dir > 0 ? rx2 : n * rx2;
+    const int k = std::min(mrow, ncol);
+    const int64_t N = (int64_t)rx1 * n * rx2;
+    DevBuf bU, bR, bS, bc, bcy, bsc, bx;
+    double* U = bU.get((size_t)std::max(mrow, ncol) * k);
+    double* Rm = bR.get((size_t)std::max(mrow, ncol) * k);
+    double* Sv = bS.get((size_t)k + 64);
+    double* cand = bc.get((size_t)N * kCand);
+    double* candy = bcy.get((size_t)N * kCand);
+    double* scr = bsc.get(256);
+    double* xc = bx.get((size_t)N);
+    TTF_HIP(hipMemcpyAsync(xc, x, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+    std::vector<double> hs;
+    int kpos = 0;
+    if (dir > 0) {
+      svd_vectors(mrow, ncol, xc, mrow, true, Sv, U, mrow, hs, &kpos, st);
+      gemm_nn(k, ncol, mrow, 1.0, U, mrow, xc, mrow, 0.0, Rm, k, st, 'T', 'N');
+    } else {
+      svd_vectors(mrow, ncol, xc, mrow, false, Sv, U, ncol, hs, &kpos, st);
+      gemm_nn(mrow, k, ncol, 1.0, xc, mrow, U, ncol, 0.0, Rm, mrow, st);
+    }
+    const double norm_rhs = dev_norm(N, rhs, scr, st);
+    TTF_REQUIRE(norm_rhs > 0.0, "ttf_amen_truncation_rank: zero right-hand side");
+    LocalOp op{Shape{rx1, n, rx2, rx1, n, rx2, ra1, ra2}, phi1, A, phi2};
+    *rank = residual_rank_search(op, dir, mrow, ncol, k, U, Rm, rhs, norm_rhs, res_new, eps2, cand, candy, scr,
+                                 nullptr, nullptr, errs, st);
+    TTF_HIP(hipStreamSynchronize(st));
+  });
+}
 
 extern "C" int ttf_tt_amen_solve(int d, const int* n, const int* m, const int* ry, const int* ra, const double* crA,
                                  const double* crY, const double* crX0, int* rx, double eps, int kickrank, int nswp,
@@ -133,6 +241,9 @@ extern "C" int ttf_tt_amen_solve(int d, const int* n, const int* m, const int* r
     for (int k = 0; k <= d; ++k) { ramax = std::max(ramax, ra[k]); rymax = std::max(rymax, ry[k]); }
     int R = rmax > 0 ? rmax : 256;
     for (int k = 0; k <= d; ++k) R = std::max(R, rx[k]);
+    // every buffer below is sized for ranks up to R: truncation and kick
+    // keep each new rank within rcap (rmax when given, else the capacity)
+    const int rcap = rmax > 0 ? std::min(rmax, R) : R;
     const int kmax = std::max(kickrank, 0);
     const size_t Ncap = (size_t)R * nmax * R;
     std::vector<int64_t> pa(d + 1, 0), py(d + 1, 0);
@@ -179,7 +290,6 @@ extern "C" int ttf_tt_amen_solve(int d, const int* n, const int* m, const int* r
     double* R2 = bR2.get((size_t)(R + kmax) * R * nmax);
     double* Q = bQ.get((size_t)R * nmax * (R + kmax));
     double* Sv = bS.get((size_t)R * nmax + 64);
-    constexpr int kCand = 8;
     double* cand = bcand.get(Ncap * kCand);
     double* candy = bcy.get(Ncap * kCand);
     double* tmp = btmp.get(Ncap + (size_t)R * nmax * (R + kmax));
@@ -324,29 +434,9 @@ extern "C" int ttf_tt_amen_solve(int d, const int* n, const int* m, const int* r
           const int k = rnew;
           gemm_nn(k, ncol, mrow, 1.0, U, mrow, x, mrow, 0.0, Rm, k, st, 'T', 'N');   // S V^T
           // residual-based rank (:819-833): scan candidates from the top
-          int nn = 0;
-          bool stop = false;
-          for (int c0 = k - 1; c0 >= 1 && !stop; c0 -= kCand) {
-            const int nc = std::min(kCand, c0);   // candidates c0, c0-1, ..., c0-nc+1
-            for (int t = 0; t < nc; ++t)
-              gemm_nn(mrow, ncol, c0 - t, 1.0, U, mrow, Rm, k, 0.0, cand + (size_t)N * t, mrow, st);
-            op.apply(nc, cand, N, candy, N, st);
-            mv_total += nc;
-            flops_total += bfun3_flops(op.s) * nc;
-            hipLaunchKernelGGL(resid_norms_kernel, dim3(nc), dim3(256), 0, st, N, candy, N, rhs, scr);
-            std::vector<double> hr(nc);
-            read_back(hr.data(), scr, sizeof(double) * nc, st);
-            for (int t = 0; t < nc; ++t) {
-              const double e = std::sqrt(std::max(hr[t], 0.0)) / norm_rhs;
-              if (e > res_new * 2.0 && e > eps2) {
-                nn = c0 - t;
-                stop = true;
-                break;
-              }
-            }
-          }
-          nn = nn + 1;
-          if (rmax > 0) nn = std::min(nn, rmax);
+          int nn = residual_rank_search(op, 1, mrow, ncol, k, U, Rm, rhs, norm_rhs, res_new, eps2, cand, candy,
+                                        scr, &mv_total, &flops_total, nullptr, st);
+          nn = std::min(nn, rcap);
           nn = std::min(nn, k);
           complete_columns(mrow, kpos, nn, U, mrow, st);
           if (kick > 0) {
@@ -355,7 +445,7 @@ extern "C" int ttf_tt_amen_solve(int d, const int* n, const int* m, const int* r
             bfun3_left_half(s, phiA[i].p, dA + pa[i], x, kblk + (size_t)mrow * ry[i + 1], st);
             const int kcols = r2 * ra[i + 1] + ry[i + 1];
             int mm = duchol('T', kcols, mrow, kblk, std::min(kick, mrow), du, nullptr, st);
-            if (rmax > 0) mm = std::min(mm, std::max(rmax - nn, 0));
+            mm = std::min(mm, std::max(rcap - nn, 0));
             // [U(:, 0:nn) du] -> Q R2, R = R2(:, 0:nn) (S V^T)(0:nn, :)
             TTF_HIP(hipMemcpyAsync(tmp, U, sizeof(double) * (size_t)mrow * nn, hipMemcpyDeviceToDevice, st));
             TTF_HIP(hipMemcpyAsync(tmp + (size_t)mrow * nn, du, sizeof(double) * (size_t)mrow * mm,
@@ -377,6 +467,7 @@ extern "C" int ttf_tt_amen_solve(int d, const int* n, const int* m, const int* r
           TTF_HIP(hipMemcpyAsync(x, Q, sizeof(double) * (size_t)mrow * rnew, hipMemcpyDeviceToDevice, st));
           R_out = Rm;
         }
+        TTF_REQUIRE(rnew <= R, "ttf_tt_amen_solve: rank %d exceeds the capacity %d", rnew, R);
         // next core <- R * core(i+1)
         const int ncol2 = n[i + 1] * rx[i + 2];
         gemm_nn(rnew, ncol2, ncol, 1.0, R_out, rnew, core[i + 1].p, ncol, 0.0, tmp, rnew, st);
@@ -401,29 +492,10 @@ extern "C" int ttf_tt_amen_solve(int d, const int* n, const int* m, const int* r
           svd_vectors(mrow, ncol, x, mrow, false, Sv, U, ncol, hs, &kpos, st);   // V (ncol x k)
           const int k = rnew;
           gemm_nn(mrow, k, ncol, 1.0, x, mrow, U, ncol, 0.0, Rm, mrow, st);      // U S (mrow x k)
-          int nn = 0;
-          bool stop = false;
-          for (int c0 = k - 1; c0 >= 1 && !stop; c0 -= kCand) {
-            const int nc = std::min(kCand, c0);
-            for (int t = 0; t < nc; ++t)
-              gemm_nn(mrow, ncol, c0 - t, 1.0, Rm, mrow, U, ncol, 0.0, cand + (size_t)N * t, mrow, st, 'N', 'T');
-            op.apply(nc, cand, N, candy, N, st);
-            mv_total += nc;
-            flops_total += bfun3_flops(op.s) * nc;
-            hipLaunchKernelGGL(resid_norms_kernel, dim3(nc), dim3(256), 0, st, N, candy, N, rhs, scr);
-            std::vector<double> hr(nc);
-            read_back(hr.data(), scr, sizeof(double) * nc, st);
-            for (int t = 0; t < nc; ++t) {
-              const double e = std::sqrt(std::max(hr[t], 0.0)) / norm_rhs;
-              if (e > res_new * 2.0 && e > eps2) {
-                nn = c0 - t;
-                stop = true;
-                break;
-              }
-            }
-          }
-          nn = nn + 1;
-          if (rmax > 0) nn = std::min(nn, rmax);
+          // residual-based rank (:937-950)
+          int nn = residual_rank_search(op, -1, mrow, ncol, k, U, Rm, rhs, norm_rhs, res_new, eps2, cand, candy,
+                                        scr, &mv_total, &flops_total, nullptr, st);
+          nn = std::min(nn, rcap);
           nn = std::min(nn, k);
           complete_columns(ncol, kpos, nn, U, ncol, st);
           // (U S)(:, 0:nn)^T: nn x mrow
@@ -440,7 +512,7 @@ extern "C" int ttf_tt_amen_solve(int d, const int* n, const int* m, const int* r
                                      sizeof(double) * r1 * ra[i], sizeof(double) * r1 * ra[i], ncol,
                                      hipMemcpyDeviceToDevice, st));
             int mm = duchol('N', krows, ncol, kblk2 + (size_t)krows * ncol, std::min(kick, ncol), du, nullptr, st);
-            if (rmax > 0) mm = std::min(mm, std::max(rmax - nn, 0));
+            mm = std::min(mm, std::max(rcap - nn, 0));
             TTF_HIP(hipMemcpyAsync(tmp, U, sizeof(double) * (size_t)ncol * nn, hipMemcpyDeviceToDevice, st));
             TTF_HIP(hipMemcpyAsync(tmp + (size_t)ncol * nn, du, sizeof(double) * (size_t)ncol * mm,
                                    hipMemcpyDeviceToDevice, st));
@@ -458,6 +530,7 @@ extern "C" int ttf_tt_amen_solve(int d, const int* n, const int* m, const int* r
           orth_factor(ncol, mrow, tmp, Q, Rm, Sv, st);
           Lt = Rm;
         }
+        TTF_REQUIRE(rnew <= R, "ttf_tt_amen_solve: rank %d exceeds the capacity %d", rnew, R);
         // core i = Q^T (rnew x ncol); core(i-1) <- core(i-1) Lt^T
         transpose(ncol, rnew, Q, ncol, x, rnew, st);
         const int mp = rx[i - 1] * n[i - 1];

@@ -55,6 +55,14 @@ struct Timer {
 
 using namespace ttf;
 
+// Test hook: the chop rule exactly as the sweep applies it to the singular
+// values it reads back.
+extern "C" int ttf_my_chop3(int n, const double* s, double eps) {
+  if (n < 0 || (n > 0 && !s)) return -1;
+  std::vector<double> v(s, s + n);
+  return my_chop3(v, n, eps);
+}
+
 // Guard vectors of the local block eigensolver (block size B + guard):
 // TTF_EIG_GUARD overrides the default 3 (0..16).
 static int eig_guard_vectors() {
@@ -152,8 +160,11 @@ extern "C" int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, con
     long total_mv = 0;
     std::vector<double> lam(B, 0.0), hs;
     int last_swp = 0;
-    std::vector<double> rec_sec, rec_flops;
+    std::vector<double> rec_sec, rec_flops, rec_er, rec_fv;
     std::vector<long> rec_mv;
+    // ermax of the last completed sweep and whether it met eps
+    double last_er = 0.0;
+    bool converged = false;
     Timer tsweep;
     double sweep_flops = 0.0;
     long sweep_mv = 0;
@@ -259,10 +270,16 @@ extern "C" int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, con
         rec_sec.push_back(tsweep.ms() * 1e-3);
         rec_flops.push_back(sweep_flops);
         rec_mv.push_back(sweep_mv);
+        rec_er.push_back(ermax);
+        rec_fv.push_back(fvold);
         tsweep = Timer();
         sweep_flops = 0.0;
         sweep_mv = 0;
-        if (ermax < eps) break;
+        last_er = ermax;
+        if (ermax < eps) {
+          converged = true;
+          break;
+        }
         ermax = 0.0;
       } else if (dir < 0 && k == 1) {
         dir = 1;
@@ -299,7 +316,8 @@ extern "C" int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, con
     if (stats) {
       stats->sweeps = last_swp > 0 ? last_swp : nswp0;
       stats->total_matvecs = total_mv;
-      stats->last_ermax = ermax;
+      stats->last_ermax = last_er;
+      stats->converged = converged ? 1 : 0;
       int mx = 0;
       for (int q = 0; q <= d; ++q) mx = std::max(mx, ry[q]);
       stats->max_rank = mx;
@@ -312,6 +330,8 @@ extern "C" int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, con
         stats->sweep_seconds[q] = rec_sec[q];
         stats->sweep_flops[q] = rec_flops[q];
         stats->sweep_matvecs[q] = rec_mv[q];
+        stats->sweep_ermax[q] = rec_er[q];
+        stats->sweep_lambda_sum[q] = rec_fv[q];
       }
     }
   });

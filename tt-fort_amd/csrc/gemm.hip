@@ -123,6 +123,8 @@ static const char* tune_file() {
   return (f && f[0]) ? f : nullptr;
 }
 
+static int splitk_rule(int M, int N, int K, int nbatch);
+
 static void load_tune_table() {
   if (g_table_loaded) return;
   g_table_loaded = true;
@@ -140,7 +142,10 @@ static void load_tune_table() {
     const bool valid = (got == 9 || got == 10) && (c.bm == 32 || c.bm == 64) &&
                        (c.bk == 16 || c.bk == 32 || c.bk == 64) && c.ks >= 1 && c.ks <= 16 &&
                        (c.bn == 0 || (c.bn == 80 && c.bm == 64 && c.bk <= 32));
-    if (valid) g_tuned[k] = c;
+    if (valid) {
+      c.ks = splitk_rule(k.M, k.N, k.K, k.nbatch);   // the split is never tuned
+      g_tuned[k] = c;
+    }
   }
   std::fclose(fp);
 }
@@ -163,17 +168,26 @@ static bool autotune_enabled() {
   return g_autotune.load() == 1;
 }
 
+// Split-K factor of a product, a function of its shape alone: the tuner
+// varies tiles and K chunks, which leave every sum's order unchanged, but
+// never the split, so one shape gives the same bits in every process and
+// on every stream whatever the tuning found.  Products with fewer than 128
+// 64 x 64 tiles split K into power-of-two ranges of at least 64 until about
+// 512 work-groups are in flight.
+static int splitk_rule(int M, int N, int K, int nbatch) {
+  const int64_t t64 = ceil_div(M, 64) * ceil_div(N, 64) * (int64_t)nbatch;
+  if (t64 >= 128 || K < 128) return 1;
+  int ks = 1;
+  while (ks < 8 && K / (2 * ks) >= 64 && t64 * ks < 512) ks *= 2;
+  return ks;
+}
+
 static Cfg heuristic(int M, int N, int K, int nbatch) {
   const int64_t cus = 256;
+  const int ks = splitk_rule(M, N, K, nbatch);
   int64_t t64 = ceil_div(M, 64) * ceil_div(N, 64) * (int64_t)nbatch;
-  if (t64 >= cus / 2) return {64, 16, 1};
-  int64_t t32 = ceil_div(M, 32) * ceil_div(N, 32) * (int64_t)nbatch;
-  Cfg c{32, 16, 1};
-  if (t32 < cus && K >= 256) {
-    int ks = (int)std::min<int64_t>(ceil_div(2 * cus, t32), K / 128);
-    c.ks = std::max(1, std::min(ks, 16));
-  }
-  return c;
+  if (t64 >= cus / 2) return {64, 16, ks};
+  return {32, 16, ks};
 }
 
 template <typename T, bool CT>
@@ -222,8 +236,8 @@ static T* splitk_work(const GemmOp<T>& op, int ks) {
   return (T*)arena(kArenaSplitK, need);
 }
 
-// Times each candidate configuration twice with events and keeps the
-// fastest.  Only products that overwrite C (beta == 0) are tuned, since the
+// Times each candidate tile shape and K chunk twice with events and keeps
+// the fastest; the split-K factor is fixed by splitk_rule.  Only products that overwrite C (beta == 0) are tuned, since the
 // candidates run the product several times.
 template <typename T>
 static Cfg tune(const GemmOp<T>& op0, hipStream_t st) {
@@ -235,15 +249,9 @@ static Cfg tune(const GemmOp<T>& op0, hipStream_t st) {
     for (int bm : {64, 32})
       for (int bk : {16, 32, 64}) {
         if (bn == 80 && (bm != 64 || bk > 32 || op0.N % 80 != 0)) continue;
-        for (int ks : {1, 2, 4, 8}) {
-          if (ks > 1 && op0.K / ks < 64) continue;
-          const int cbn = bn > 0 ? bn : bm;
-          int64_t tiles = ceil_div(op0.M, bm) * ceil_div(op0.N, cbn) * (int64_t)op0.nbatch;
-          if (ks > 1 && tiles * ks > 4096) continue;
-          Cfg c{bm, bk, ks};
-          c.bn = bn;
-          cands.push_back(c);
-        }
+        Cfg c{bm, bk, splitk_rule(op0.M, op0.N, op0.K, op0.nbatch)};
+        c.bn = bn;
+        cands.push_back(c);
       }
   hipEvent_t e0, e1;
   TTF_HIP(hipEventCreate(&e0));
@@ -265,7 +273,10 @@ static Cfg tune(const GemmOp<T>& op0, hipStream_t st) {
   }
   TTF_HIP(hipEventDestroy(e0));
   TTF_HIP(hipEventDestroy(e1));
-  ++g_tune_count;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    ++g_tune_count;
+  }
   return best;
 }
 

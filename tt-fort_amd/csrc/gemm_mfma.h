@@ -19,7 +19,9 @@
 //
 // Split-K: with ksplit > 1 each work-group reduces a K range into a
 // workspace slice; gemm_splitk_reduce then sums the slices in a fixed order
-// (deterministic) and applies alpha/beta.
+// (deterministic) and applies alpha/beta.  Every output element is summed
+// over K in increasing order, four products per MFMA, whatever the tile
+// shape and K chunk, so only ksplit changes the rounding of a product.
 #pragma once
 
 #include "ttf_common.h"
@@ -115,8 +117,12 @@ __global__ __launch_bounds__(64 * NW) void gemm_mfma_kernel(GemmOp<T> op) {
   const T* __restrict__ A = op.A + b0 * op.sAb0 + b1 * op.sAb1;
   const T* __restrict__ B = op.B + b0 * op.sBb0 + b1 * op.sBb1;
 
-  // K range of this work-group
-  const int kchunk = ((op.K + op.ksplit - 1) / op.ksplit + BK - 1) / BK * BK;
+  // K range of this work-group.  Split-K ranges are multiples of 64 for every
+  // BK, so the slices (and the order of the sums) depend on ksplit alone.
+  constexpr int KG = 64;
+  static_assert(KG % BK == 0, "split-K granularity must be a multiple of BK");
+  const int kchunk = op.ksplit > 1 ? ((op.K + op.ksplit - 1) / op.ksplit + KG - 1) / KG * KG
+                                   : ((op.K + BK - 1) / BK) * BK;
   const int kbeg = blockIdx.z * kchunk;
   const int kend = min(op.K, kbeg + kchunk);
 

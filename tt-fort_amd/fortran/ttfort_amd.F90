@@ -34,6 +34,8 @@ module ttfort_amd
     integer(c_int) :: nrec
     real(c_double) :: sweep_seconds(64), sweep_flops(64)
     integer(c_long) :: sweep_matvecs(64)
+    real(c_double) :: sweep_ermax(64), sweep_lambda_sum(64)
+    integer(c_int) :: converged
   end type ttf_eigb_stats
 
   type, bind(c) :: ttf_amen_stats

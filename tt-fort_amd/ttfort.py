@@ -96,7 +96,8 @@ class EigbStats(C.Structure):
                 ("max_rank", C.c_int), ("seconds", C.c_double), ("solve_seconds", C.c_double),
                 ("trunc_seconds", C.c_double), ("phi_seconds", C.c_double), ("nrec", C.c_int),
                 ("sweep_seconds", C.c_double * 64), ("sweep_flops", C.c_double * 64),
-                ("sweep_matvecs", C.c_long * 64)]
+                ("sweep_matvecs", C.c_long * 64), ("sweep_ermax", C.c_double * 64),
+                ("sweep_lambda_sum", C.c_double * 64), ("converged", C.c_int)]
 
     def as_dict(self):
         out = {}
@@ -122,6 +123,10 @@ class AmenStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+_sig("ttf_my_chop3", _c_int, _c_int, _DP, C.c_double)
+_sig("ttf_residual_rank_rule", _c_int, _c_int, _DP, C.c_double, C.c_double)
+_sig("ttf_amen_truncation_rank", _c_int, _c_int, *([_c_int] * 5), _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_double,
+     _IP, _DP)
 _sig("ttf_djac_gen", _c_int, C.c_char, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp)
 _sig("ttf_djac_size", _c_i64, C.c_char, _c_int, _c_int, _c_int)
 _sig("ttf_djac_apply", _c_int, C.c_char, _c_int, _c_int, _c_int, _vp, _vp, _vp)
@@ -429,6 +434,34 @@ def tt_eigb(n, ra, crA, crY0, ry, eps, rmax, B, nswp=20, max_full_size=2000, ver
                           _dp(lam), B, kickrank, nswp, max_full_size, verb, _dp(res), cap,
                           C.byref(size), C.byref(st)), lib)
     return lam, res[:size.value].copy(), ry_, st.as_dict()
+
+
+def my_chop3(s, eps):
+    """Rank of the relative chop rule as ttf_tt_eigb applies it (host code)."""
+    lib = load()
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    return int(lib.ttf_my_chop3(s.size, _dp(s), eps))
+
+
+def residual_rank_rule(errs, res_new, eps2):
+    """The AMEn residual-rank scan on given candidate residuals errs[c],
+    c = 1..k-1 (errs[0] unused; k = len(errs))."""
+    lib = load()
+    e = np.ascontiguousarray(errs, dtype=np.float64)
+    return int(lib.ttf_residual_rank_rule(e.size, _dp(e), res_new, eps2))
+
+
+def amen_truncation_rank(direction, rx1, n, rx2, ra1, ra2, phi1, A, phi2, x, rhs, res_new, eps2):
+    """The device residual-rank search of one AMEn local step.  Returns
+    (rank, errs) with errs[c] the candidate residuals evaluated (-1 if not)."""
+    lib = load()
+    arrs = [DeviceArray.from_numpy(np.ascontiguousarray(v, dtype=np.float64)) for v in (phi1, A, phi2, x, rhs)]
+    k = min(rx1 * n, rx2) if direction > 0 else min(rx1, n * rx2)
+    errs = np.zeros(k)
+    rank = C.c_int()
+    check(lib.ttf_amen_truncation_rank(direction, rx1, n, rx2, ra1, ra2, *[a.ptr for a in arrs], res_new, eps2,
+                                       C.byref(rank), _dp(errs)), lib)
+    return rank.value, errs
 
 
 def djac_gen(ptype, rx1, n, rx2, ra1, ra2, phi1, A, phi2):

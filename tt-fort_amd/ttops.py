@@ -101,3 +101,51 @@ def round_tt(cores, eps, rmax=None):
         cores[k] = U[:, :keep].reshape(r0, nn, keep)
         cores[k + 1] = np.einsum("ab,bic->aic", s[:keep, None] * Vt[:keep], cores[k + 1])
     return cores
+
+
+def block_eig_residuals(crA, ra, n, cr, ry, lam):
+    """FP64 eigen-residuals ||A x_b - lambda_b x_b|| / ||x_b|| of a block TT
+    whose last core carries the block index (ranks ry, ry[-1] = B), as
+    tt_eigb returns it.
+
+    The difference A x_b - lambda_b x_b is formed exactly in TT format
+    (cores [A x | x] block-diagonal, the last one [A x_b ; -lambda_b x_b]) and
+    its norm taken by left-orthogonalisation, so residuals far below |lambda|
+    are not lost to cancellation.  The left sweep is shared by all B
+    eigenpairs; only the last core differs.  Also returns the Gram matrix
+    X^T X of the block (from the last core, exact when the other cores are
+    left-orthogonal, which the norms do not assume)."""
+    d = len(n)
+    A = split(crA, ra, n, n)
+    X = split(cr, ry, n)
+    B = X[-1].shape[2]
+    R = np.ones((1, 1))
+    Rx = np.ones((1, 1))   # left-orthogonalisation of x alone, for the Gram matrix and norms
+    for k in range(d - 1):
+        a, x = A[k], X[k]
+        ax = np.einsum("aijb,cjd->acibd", a, x, optimize=True)
+        ax = ax.reshape(a.shape[0] * x.shape[0], a.shape[1], a.shape[3] * x.shape[2], order="F")
+        r0a, nn, r1a = ax.shape
+        r0x, _, r1x = x.shape
+        if k == 0:
+            core = np.concatenate([ax, x], axis=2)
+        else:
+            core = np.zeros((r0a + r0x, nn, r1a + r1x))
+            core[:r0a, :, :r1a] = ax
+            core[r0a:, :, r1a:] = x
+        c = np.einsum("ab,bic->aic", R, core, optimize=True)
+        _, R = np.linalg.qr(c.reshape(c.shape[0] * nn, c.shape[2], order="F"))
+        cx = np.einsum("ab,bic->aic", Rx, x, optimize=True)
+        _, Rx = np.linalg.qr(cx.reshape(cx.shape[0] * nn, cx.shape[2], order="F"))
+    a, x = A[-1], X[-1]
+    res = np.zeros(B)
+    gram_core = np.einsum("ab,bic->aic", Rx, x, optimize=True).reshape(-1, B, order="F")
+    G = gram_core.T @ gram_core
+    for b in range(B):
+        xb = x[:, :, b:b + 1]
+        ax = np.einsum("aijb,cjd->acibd", a, xb, optimize=True)
+        ax = ax.reshape(a.shape[0] * xb.shape[0], a.shape[1], 1, order="F")
+        last = np.concatenate([ax, -lam[b] * xb], axis=0)
+        v = np.einsum("ab,bic->aic", R, last, optimize=True)
+        res[b] = np.linalg.norm(v) / np.sqrt(G[b, b])
+    return res, G
