@@ -98,6 +98,21 @@ long ttf_autotune_count(void);
 int ttf_dbfun3(int rx1, int m, int rx2, int ry1, int n, int ry2, int ra1, int ra2,
                int nvec, const double* phi1, const double* A, const double* phi2,
                const double* x, int64_t ldx, double* y, int64_t ldy);
+/* ttf_dbfun3 for callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops, harnesses that forbid allocation).
+ * No counterpart in the reference.  Same arguments, kernels and results as
+ * ttf_dbfun3, but the stage intermediates and split-K slices live in one
+ * fixed device array of 64 MiB (8 M doubles) per GPU, held until the process
+ * ends, and no allocation or tuning run happens in the call.  Products
+ * needing more scratch than that are refused with an error before any
+ * launch (a block of 8 vectors at (r, n, r, R) = (128, 2, 128, 5) takes
+ * 4.7 M doubles).
+ * All calls on one GPU share its array: calls that overlap on one GPU (from
+ * several host threads or concurrent streams) race.  Device pointers;
+ * asynchronous on the library stream. */
+int ttf_dbfun3_noalloc(int rx1, int m, int rx2, int ry1, int n, int ry2, int ra1, int ra2,
+                       int nvec, const double* phi1, const double* A, const double* phi2,
+                       const double* x, int64_t ldx, double* y, int64_t ldy);
 /* Times the three dbfun3 GEMM stages: each stage is launched `iters` times
  * back to back between two events on the library stream (after one launch
  * that settles its tuning) and stage_ms[k] receives the mean milliseconds

@@ -21,6 +21,15 @@ inline double bfun3_flops(const Shape& s) {
 template <typename T>
 void bfun3_chain(const Shape& s, int nvec, const T* phi1, const T* A, const T* phi2, const T* x,
                  int64_t ldx, T* y, int64_t ldy, hipStream_t st);
+// Elements of workspace bfun3_chain_ws needs for nvec vectors: the two
+// stage intermediates and the largest stage's split-K slices.
+size_t bfun3_workspace(const Shape& s, int nvec);
+// bfun3_chain with every intermediate in ws (ws_cap elements, an error if
+// too small): no allocation and no tuning runs; the same bits as
+// bfun3_chain.
+template <typename T>
+void bfun3_chain_ws(const Shape& s, int nvec, const T* phi1, const T* A, const T* phi2, const T* x,
+                    int64_t ldx, T* y, int64_t ldy, T* ws, size_t ws_cap, hipStream_t st);
 template <typename T>
 void bfun3_stage12(const Shape& s, int nvec, const T* A, const T* phi2, const T* x, int64_t ldx,
                    T* T1, T* T2, hipStream_t st);

@@ -73,6 +73,38 @@ void bfun3_chain(const Shape& s, int nvec, const T* phi1, const T* A, const T* p
   gemm_launch(bfun3_op3(s, nvec, phi1, (const T*)T2, y, ldy), st);
 }
 
+size_t bfun3_workspace(const Shape& s, int nvec) {
+  if (nvec <= 0) return 0;
+  const int64_t sz1 = (int64_t)s.rx1 * s.m * s.ra2 * s.ry2;
+  const int64_t sz2 = (int64_t)s.rx1 * s.ra1 * s.n * s.ry2;
+  const size_t inter = (size_t)(sz1 + sz2) * nvec;
+  size_t split = 0;
+  split = std::max(split, gemm_workspace(bfun3_op1<double>(s, nvec, nullptr, nullptr, 1, nullptr)));
+  split = std::max(split, gemm_workspace(bfun3_op2<double>(s, nvec, nullptr, nullptr, nullptr)));
+  split = std::max(split, gemm_workspace(bfun3_op3<double>(s, nvec, nullptr, nullptr, nullptr, 1)));
+  return inter + split;
+}
+
+template <typename T>
+void bfun3_chain_ws(const Shape& s, int nvec, const T* phi1, const T* A, const T* phi2, const T* x,
+                    int64_t ldx, T* y, int64_t ldy, T* ws, size_t ws_cap, hipStream_t st) {
+  if (nvec <= 0) return;
+  const size_t need = bfun3_workspace(s, nvec);
+  TTF_REQUIRE(ws != nullptr && need <= ws_cap, "bfun3: workspace of %zu elements exceeds the %zu given", need,
+              ws_cap);
+  const int64_t sz1 = (int64_t)s.rx1 * s.m * s.ra2 * s.ry2;
+  const int64_t sz2 = (int64_t)s.rx1 * s.ra1 * s.n * s.ry2;
+  T* T1 = ws;
+  T* T2 = T1 + sz1 * nvec;
+  T* split = T2 + sz2 * nvec;
+  const size_t split_cap = ws_cap - (size_t)(sz1 + sz2) * nvec;
+  gemm_launch_ws(bfun3_op1(s, nvec, phi2, x, ldx, T1), split, split_cap, st);
+  gemm_launch_ws(bfun3_op2(s, nvec, A, (const T*)T1, T2), split, split_cap, st);
+  gemm_launch_ws(bfun3_op3(s, nvec, phi1, (const T*)T2, y, ldy), split, split_cap, st);
+}
+
+template void bfun3_chain_ws<double>(const Shape&, int, const double*, const double*, const double*,
+                                     const double*, int64_t, double*, int64_t, double*, size_t, hipStream_t);
 template void bfun3_stage12<double>(const Shape&, int, const double*, const double*, const double*,
                                     int64_t, double*, double*, hipStream_t);
 template void bfun3_chain<double>(const Shape&, int, const double*, const double*, const double*,

@@ -330,8 +330,46 @@ void gemm_launch(GemmOp<T> op, hipStream_t st) {
   launch_cfg(op, c, st);
 }
 
+template <typename T>
+size_t gemm_workspace(const GemmOp<T>& op) {
+  if (op.M <= 0 || op.N <= 0 || op.nbatch <= 0 || panel_eligible(op)) return 0;
+  const int ks = splitk_rule(op.M, op.N, op.K, op.nbatch);
+  return ks > 1 ? (size_t)ks * op.nbatch * (size_t)op.M * op.N : 0;
+}
+
+template <typename T>
+void gemm_launch_ws(GemmOp<T> op, T* ws, size_t ws_cap, hipStream_t st) {
+  if (op.M <= 0 || op.N <= 0 || op.nbatch <= 0) return;
+  if (op.nb0 <= 0) op.nb0 = 1;
+  TTF_REQUIRE(op.nbatch <= 65535, "gemm: batch count %d exceeds the grid limit", op.nbatch);
+  if (op.K <= 0) op.K = 0;
+  if (panel_eligible(op)) {
+    gemm_launch(op, st);   // the panel kernel needs no workspace and is never tuned
+    return;
+  }
+  Key key{type_id<T>(), op.M, op.N, op.K, op.nbatch,
+          (op.sAi == 1 ? 1 : 0) | (op.sBj == 1 ? 2 : 0) | (op.sCi == 1 ? 4 : 0)};
+  Cfg c = heuristic(op.M, op.N, op.K, op.nbatch);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end()) c = it->second;
+  }
+  if (c.ks > 1) {
+    const size_t need = (size_t)c.ks * op.nbatch * (size_t)op.M * op.N;
+    TTF_REQUIRE(ws != nullptr && need <= ws_cap, "gemm: split-K workspace of %zu elements exceeds the %zu given",
+                need, ws_cap);
+    op.work = ws;
+  }
+  launch_cfg(op, c, st);
+}
+
 template void gemm_launch<double>(GemmOp<double>, hipStream_t);
 template void gemm_launch<float>(GemmOp<float>, hipStream_t);
+template size_t gemm_workspace<double>(const GemmOp<double>&);
+template size_t gemm_workspace<float>(const GemmOp<float>&);
+template void gemm_launch_ws<double>(GemmOp<double>, double*, size_t, hipStream_t);
+template void gemm_launch_ws<float>(GemmOp<float>, float*, size_t, hipStream_t);
 
 void zgemm_launch(ZGemmOp op, hipStream_t st) {
   if (op.M <= 0 || op.N <= 0 || op.nbatch <= 0) return;

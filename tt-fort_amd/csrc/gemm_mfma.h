@@ -458,6 +458,16 @@ static __global__ void zgemm_splitk_reduce(ZGemmOp op) {
 // go to a dedicated vector kernel instead of the matrix cores.
 template <typename T>
 void gemm_launch(GemmOp<T> op, hipStream_t st);
+// Elements of workspace gemm_launch needs for op's split-K slices (0 when
+// the product runs without them); a function of the shape alone.
+template <typename T>
+size_t gemm_workspace(const GemmOp<T>& op);
+// gemm_launch without any allocation and without tuning runs: split-K
+// slices go to ws (ws_cap elements; an error if too small), the tiles are
+// the ones tuned earlier for this shape if any, else the fixed heuristic.
+// The split-K factor, and so every bit of C, is the same as gemm_launch's.
+template <typename T>
+void gemm_launch_ws(GemmOp<T> op, T* ws, size_t ws_cap, hipStream_t st);
 void zgemm_launch(ZGemmOp op, hipStream_t st);
 
 // Convenience constructor for a column-major GEMM with optional transposes:

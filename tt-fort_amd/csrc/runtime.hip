@@ -56,6 +56,7 @@ static void init_locked(int device) {
   TTF_REQUIRE(std::strncmp(prop.gcnArchName, "gfx950", 6) == 0,
               "this library is built for gfx950 (MI355X); device %d is %s", device,
               prop.gcnArchName);
+  noalloc_setup(device);
   TTF_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
   TTF_HIP(hipEventCreate(&g_ev0));
   TTF_HIP(hipEventCreate(&g_ev1));

@@ -80,6 +80,10 @@ class ScopedContext {
 // or 32 (fp32 matrix cores, FP64 refinement); set by ttf_set_local_precision.
 int local_matvec_bits();
 void ensure_init();
+// Resolves this device's copy of the fixed scratch array of the
+// no-allocation entry points (noalloc.hip); called by the library's init
+// with `device` current.
+void noalloc_setup(int device);
 
 enum Arena {
   kArenaBfun = 0,    // contraction intermediates (res1 / res2)

@@ -73,7 +73,7 @@ _sig("ttf_timer_start", _c_int)
 _sig("ttf_timer_stop", _c_int, C.POINTER(C.c_float))
 _sig("ttf_set_autotune", _c_int, _c_int)
 _sig("ttf_autotune_count", C.c_long)
-for _nm in ("ttf_dbfun3", "ttf_sbfun3", "ttf_zbfun3", "ttf_zbfun3_transp"):
+for _nm in ("ttf_dbfun3", "ttf_dbfun3_noalloc", "ttf_sbfun3", "ttf_zbfun3", "ttf_zbfun3_transp"):
     _sig(_nm, _c_int, *(_S8 + [_c_int, _vp, _vp, _vp, _vp, _c_i64, _vp, _c_i64]))
 _sig("ttf_dbfun3_profile", _c_int, *(_S8 + [_c_int, _vp, _vp, _vp, _vp, _c_i64, _vp, _c_i64, _c_int,
                                            C.POINTER(C.c_float)]))
