@@ -4,13 +4,17 @@
  *
  * Two families of entry points:
  *
- *  1. Reference-compatible symbols.  Same names and by-reference argument
- *     lists as the tt-fort `bind(c)` subroutines in tt_linalg.f90 (Fortran
- *     default integers, host pointers, column-major arrays).  A Fortran
- *     program that links this library instead of tt_linalg.o gets the GPU
- *     path without source changes.  Workspace arguments (res1, res2, work)
- *     are accepted and ignored: the library keeps its own device scratch.
- *     These calls stage host arrays through PCIe on every call.
+ *  1. Reference-compatible symbols.  The tt-fort `bind(c)` subroutines of
+ *     tt_linalg.f90 (dbfun3, dqr, drow_add, drow_cut, dtransp, dperm1324,
+ *     djac_gen, djac_apply, duchol_fort, dgmresr_hh_fort) under the same
+ *     names with the same by-reference argument lists (Fortran default
+ *     integers, host pointers, column-major arrays), and C versions of the
+ *     local-operator callbacks the reference hands PRIMME and Expokit.  A
+ *     program that links this library instead of tt_linalg.o reaches the
+ *     GPU without source changes.  Workspace arguments (res1, res2, work,
+ *     tau, work1) are accepted and ignored: the library keeps its own
+ *     device scratch.  These calls stage host arrays through PCIe on every
+ *     call.
  *
  *  2. Native `ttf_*` entry points.  Plain int sizes, device pointers,
  *     int status returns (0 = success), blocks of vectors, and the
@@ -196,6 +200,11 @@ int ttf_dsvd_side(int m, int n, const double* A, int64_t lda, char side, double*
  * stored consecutively (A is overwritten): eigenvalues to w (ascending, or
  * descending if desc), eigenvectors to Q.  Replaces the small dsyev calls
  * (tt_eigb.f90:238 for local sizes <= 64; tt_linalg.f90:630). */
+/* Householder QR of A (n x m, ld n) on the device, in place: Q (n x k,
+ * k = min(n, m)) into A's first k columns, R (k x m, ld ldr) upper
+ * trapezoidal; LAPACK's reflector convention (tt_linalg.f90:30-51, dqr).
+ * n < 16384. */
+int ttf_dqr(int n, int m, double* A, double* R, int64_t ldr);
 int ttf_dsyev_small(int n, int count, double* A, double* w, double* Q, int desc);
 /* G (p x q, ldg) = X^T Y for tall X (N x p), Y (N x q); p, q <= 64. */
 int ttf_dgram(int N, int p, int q, const double* X, int64_t ldx, const double* Y, int64_t ldy,
@@ -486,6 +495,73 @@ void dbfun3(const ttf_fint* rx1, const ttf_fint* m, const ttf_fint* rx2,
             const ttf_fint* ra1, const ttf_fint* ra2, const double* phi1,
             const double* A, const double* phi2, const double* x, double* y,
             double* res1, double* res2);
+/* tt_linalg.f90:30-51, dqr bind(c): Householder QR in place (LAPACK's
+ * reflector convention, so Q and R match dgeqrf + dorgqr to rounding).
+ * A (n, m) returns Q in its first min(n, m) columns (and, for m > n, R's
+ * trailing columns in the rest, as dgeqrf leaves them); R is the
+ * reference's R(min(n, m), n): zeroed, then R(1:min(j, n), j) for
+ * j = 1..m written with leading dimension min(n, m).  work, lwork, tau
+ * ignored.  n < 16384. */
+void dqr(const ttf_fint* n, const ttf_fint* m, double* A, double* R, double* work,
+         const ttf_fint* lwork, double* tau);
+/* tt_linalg.f90:54-69, drow_add bind(c): the k rows of B (k, n) appended
+ * under A (m, n), in place (A must hold (m + k) n). */
+void drow_add(const ttf_fint* m, const ttf_fint* n, const ttf_fint* k, double* A, const double* B);
+/* tt_linalg.f90:71-79, drow_cut bind(c): A (m, n) -> its first k rows,
+ * packed (k, n) in place. */
+void drow_cut(const ttf_fint* m, const ttf_fint* n, const ttf_fint* k, double* A);
+/* tt_linalg.f90:82-98, dtransp bind(c): B (m, n) = A (n, m)^T. */
+void dtransp(const ttf_fint* n, const ttf_fint* m, const double* A, double* B);
+/* tt_linalg.f90:101-121, dperm1324 bind(c): B (n1, n3, n2, n4) =
+ * A (n1, n2, n3, n4). */
+void dperm1324(const ttf_fint* n1, const ttf_fint* n2, const ttf_fint* n3, const ttf_fint* n4,
+               const double* A, double* B);
+/* tt_linalg.f90:363-495, djac_gen bind(c): block-Jacobi blocks ('c', 'l'
+ * or 'r') of the local operator with Phi1 (rx1, rx1, ra1), A (ra1, n, n,
+ * ra2), Phi2 (rx2, ra2, rx2); jacs in the reference's layout
+ * (ttf_djac_size doubles).  Blocks up to 128. */
+void djac_gen(const char* ptype, const ttf_fint* rx1, const ttf_fint* n, const ttf_fint* rx2,
+              const ttf_fint* ra1, const ttf_fint* ra2, const double* Phi1, const double* A,
+              const double* Phi2, double* jacs);
+/* tt_linalg.f90:497-533, djac_apply bind(c): y = P x; work1 ignored. */
+void djac_apply(const char* ptype, const ttf_fint* rx1, const ttf_fint* n, const ttf_fint* rx2,
+                const double* jacs, const double* x, double* y, double* work1);
+/* tt_linalg.f90:540-643, duchol_fort bind(c): pivoted incomplete Cholesky
+ * of the Gram matrix of the n items of dA; du (n, numvec), lam (numvec),
+ * real_numvec.  numvec <= 32. */
+void duchol_fort(const char* trans, const ttf_fint* m, const ttf_fint* n, const double* dA,
+                 const ttf_fint* numvec, double* du, double* lam, ttf_fint* real_numvec);
+/* tt_linalg.f90:651-845, dgmresr_hh_fort bind(c): right-preconditioned
+ * restarted GMRES of the local system (Phi1 (rx1, rx1, ra1), A (ra1, n, n,
+ * ra2), Phi2 (rx2, ra2, rx2)); sol (rx1 n rx2) is returned before the
+ * preconditioner is applied, as the reference returns it.  Krylov
+ * dimension at most 63; verb ignored. */
+void dgmresr_hh_fort(const double* Phi1, const double* A, const double* Phi2, const double* rhs,
+                     const ttf_fint* rx1, const ttf_fint* n, const ttf_fint* rx2, const ttf_fint* ra1,
+                     const ttf_fint* ra2, const ttf_fint* nrestart, const double* tol,
+                     const ttf_fint* niters, const char* ptype, const double* jacs, double* sol,
+                     const ttf_fint* verb);
+
+/* ------------------------------- callbacks of the local eigen/exp solvers */
+/* The reference binds one local step's operator for its callbacks with
+ * init_bfun_sizes + init_bfun_main (tt_eigb.f90:29-49; tt_ksl.f90:55-90):
+ * here the interfaces and operator core are uploaded once per step, so
+ * each callback moves only x and y.  x is (rx1, m, rx2), y (ry1, n, ry2);
+ * xsize / ysize are the leading dimensions of blocks of vectors. */
+void ttf_init_bfun_sizes(const ttf_fint* rx1, const ttf_fint* m, const ttf_fint* rx2, const ttf_fint* ry1,
+                         const ttf_fint* n, const ttf_fint* ry2, const ttf_fint* ra1, const ttf_fint* ra2,
+                         const ttf_fint* xsize, const ttf_fint* ysize);
+void ttf_init_bfun_main(const double* phi1, const double* A, const double* phi2);
+void ttf_zinit_bfun_main(const ttf_zdouble* phi1, const ttf_zdouble* A, const ttf_zdouble* phi2);
+/* PRIMME's matrixMatvec(x, y, blockSize, primme) (primme/primme.h:140-141)
+ * as tt_eigb.f90:15-27 binds it: blockSize columns of host x and y, one
+ * block product on the device; the PRIMME parameter block is not read. */
+void ttf_primme_matvec(void* x, void* y, ttf_fint* blockSize, void* primme);
+/* Expokit's matvec(x, y) (expm/expokit.f:2453-2457) and normest's
+ * matvec_transp(x, y) (expm/normest.f90:146-151) as tt_ksl.f90:25-50 binds
+ * them: y = B x and y = B^H x on host complex(8) arrays. */
+void ttf_zmatvec(const ttf_zdouble* x, ttf_zdouble* y);
+void ttf_zmatvec_transp(const ttf_zdouble* x, ttf_zdouble* y);
 
 #ifdef __cplusplus
 }

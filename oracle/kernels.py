@@ -27,7 +27,7 @@ from __future__ import annotations
 import numpy as np
 
 __all__ = [
-    "F", "dtransp", "dperm1324",
+    "F", "dtransp", "dperm1324", "drow_add", "drow_cut",
     "dbfun3", "zbfun3", "zbfun3_transp", "dbfun3_right", "dbfun3_left_intended",
     "dphi_left", "dphi_right", "zphi_left", "zphi_right",
     "dphi2_left", "dphi2_right", "d2d_fullmat", "local_matrix_kron",
@@ -53,6 +53,17 @@ def dtransp(n, m, a):
 def dperm1324(n1, n2, n3, n4, a):
     """tt_linalg.f90:101-119: B(n1,n3,n2,n4) = A(n1,n2,n3,n4)."""
     return _flat(np.transpose(F(a, n1, n2, n3, n4), (0, 2, 1, 3)))
+
+
+def drow_add(m, n, k, A, B):
+    """tt_linalg.f90:54-69: the k rows of B (k, n) appended under A (m, n),
+    giving (m + k, n)."""
+    return _flat(np.vstack([F(A, m, n), F(B, k, n)]))
+
+
+def drow_cut(m, n, k, A):
+    """tt_linalg.f90:71-79: the first k rows of A (m, n), packed (k, n)."""
+    return _flat(F(A, m, n)[:k, :])
 
 
 def bfun3_flops(rx1, m, rx2, ry1, n, ry2, ra1, ra2, complex_=False):

@@ -311,3 +311,32 @@ def test_amen_truncation_rank_properties():
                 assert errs[r - 1] > bound
             # graded singular values: residuals fall with the rank
             assert np.all(np.diff(errs[1:]) < 0)
+
+
+def test_row_add_and_cut_are_inverse_and_index_exactly():
+    rng = np.random.default_rng(15)
+    m, n, k = 5, 4, 3
+    A = rng.standard_normal(m * n)
+    B = rng.standard_normal(k * n)
+    C = K.drow_add(m, n, k, A, B)
+    Cm = K.F(C, m + k, n)
+    for j in range(n):
+        for i in range(m + k):
+            assert Cm[i, j] == (A[i + m * j] if i < m else B[(i - m) + k * j])
+    assert np.array_equal(K.drow_cut(m + k, n, m, C), A)
+    assert np.array_equal(K.drow_cut(m, n, m, A), A)
+
+
+def test_transp_and_perm1324_index_exactly():
+    rng = np.random.default_rng(16)
+    n1, n2, n3, n4 = 2, 3, 4, 5
+    A = rng.standard_normal(n1 * n2 * n3 * n4)
+    B = K.dperm1324(n1, n2, n3, n4, A)
+    for i4 in range(n4):
+        for i3 in range(n3):
+            for i2 in range(n2):
+                for i1 in range(n1):
+                    assert B[i1 + n1 * (i3 + n3 * (i2 + n2 * i4))] == A[i1 + n1 * (i2 + n2 * (i3 + n3 * i4))]
+    M = rng.standard_normal(6 * 7)
+    T = K.dtransp(6, 7, M)
+    assert all(T[j + 7 * i] == M[i + 6 * j] for i in range(6) for j in range(7))

@@ -83,6 +83,50 @@ def build(verbose: bool = True, jobs: int = 8) -> list:
     if verbose:
         print("  linked %s in %.1fs" % (", ".join(os.path.basename(o) for o in outs), time.time() - t0),
               flush=True)
+    outs += build_fortran(verbose=verbose)
+    return outs
+
+
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+FORTRAN = os.path.join(HERE, "fortran")
+
+
+def fortran_commands(kind, outdir, libdir=LIB, rpath=None):
+    """Commands that compile the Fortran module and its boundary check
+    program for one integer kind ('i4' or 'i8': the reference's presets,
+    default INTEGER of 4 or 8 bytes) and link them against the library in
+    libdir (found at run time through rpath, default libdir)."""
+    flags = ["-cpp", "-O2"] + (["-DTTF_FINT64", "-fdefault-integer-8"] if kind == "i8" else [])
+    lib = "ttfort_amd" if kind == "i4" else "ttfort_amd_i8"
+    mod_o = os.path.join(outdir, "ttfort_amd.o")
+    exe = os.path.join(outdir, "check_boundary_%s" % kind)
+    return exe, [
+        [FLANG] + flags + ["-c", os.path.join(FORTRAN, "ttfort_amd.F90"), "-module-dir", outdir, "-o", mod_o],
+        [FLANG] + flags + ["-I", outdir, os.path.join(FORTRAN, "check_boundary.F90"), mod_o, "-L", libdir,
+                           "-l" + lib, "-Wl,-rpath," + (rpath or libdir), "-o", exe],
+    ]
+
+
+def build_fortran(verbose=True):
+    """The Fortran module and boundary check programs, per integer kind, in
+    lib/fortran_<kind>/ (skipped with a message when flang is absent); the
+    programs find the library relative to themselves, so the tree can move."""
+    if not os.path.exists(FLANG):
+        if verbose:
+            print("  flang not found at %s: Fortran module not built" % FLANG, flush=True)
+        return []
+    outs = []
+    for kind in ("i4", "i8"):
+        outdir = os.path.join(LIB, "fortran_%s" % kind)
+        os.makedirs(outdir, exist_ok=True)
+        exe, cmds = fortran_commands(kind, outdir, rpath="$ORIGIN/..")
+        for cmd in cmds:
+            r = subprocess.run(cmd, capture_output=True, text=True, cwd=outdir)
+            if r.returncode != 0:
+                raise RuntimeError("flang failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-4000:]))
+        outs.append(exe)
+    if verbose:
+        print("  built the Fortran module and boundary checks (i4, i8)", flush=True)
     return outs
 
 

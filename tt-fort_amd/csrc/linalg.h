@@ -50,6 +50,13 @@ int svd_vectors(int m, int n, const double* A, int64_t lda, bool want_left, doub
 // such).  S: k device doubles of workspace.
 void orth_factor(int m, int n, const double* A, double* Q, double* R, double* S, hipStream_t st);
 
+// Householder QR with LAPACK's reflector convention (dgeqrf + dorgqr, the
+// reference's dqr, tt_linalg.f90:30-51): A (n x m, ld n) = Q R with Q
+// (n x k, ld n) orthonormal and R (k x m, ld ldr) upper trapezoidal,
+// k = min(n, m).  Panels of up to 32 columns in LDS (n < 16384 rows), the
+// trailing columns and the explicit Q formed by compact-WY GEMMs.
+void householder_qr(int n, int m, const double* A, double* Q, double* R, int64_t ldr, hipStream_t st);
+
 // Completes columns [k0, k) of U (m x k, ldu) to an orthonormal set.
 void complete_columns(int m, int k0, int k, double* U, int64_t ldu, hipStream_t st);
 

@@ -1177,6 +1177,84 @@ void orth_factor(int m, int n, const double* A, double* Q, double* R, double* S,
   gemm_launch(g, st);
 }
 
+namespace {
+// X(n x k) = the first k columns of the n x n identity (ld n)
+__global__ void identity_cols_kernel(int n, int k, double* __restrict__ X) {
+  const int64_t total = (int64_t)n * k;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x)
+    X[e] = (e % n == e / n) ? 1.0 : 0.0;
+}
+
+// R (k x m, ld ldr) = upper triangle of W (n x m, ld n), zero below
+__global__ void upper_copy_kernel(int k, int m, const double* __restrict__ W, int64_t ldw, double* __restrict__ R,
+                                  int64_t ldr) {
+  const int64_t total = (int64_t)k * m;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % k), j = (int)(e / k);
+    R[i + ldr * j] = (i <= j) ? W[i + ldw * j] : 0.0;
+  }
+}
+}  // namespace
+
+void householder_qr(int n, int m, const double* A, double* Q, double* R, int64_t ldr, hipStream_t st) {
+  TTF_REQUIRE(n >= 1 && m >= 1, "householder_qr: invalid sizes %d x %d", n, m);
+  const int k = std::min(n, m);
+  const int lp = n | 1;
+  const int pcmax = std::min(kQrPanel, kSvdLdsDoubles / lp);
+  TTF_REQUIRE(pcmax >= 1, "householder_qr: %d rows exceed the panel LDS budget", n);
+  static std::atomic<bool> attr_set{false};
+  if (!attr_set) {
+    TTF_HIP(hipFuncSetAttribute((const void*)qr_panel_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(sizeof(double) * kSvdLdsDoubles)));
+    attr_set = true;
+  }
+  // workspace: W (n x m) | every panel's V and Y (rows x pc each, <= n x k) | Z (pc x max(m, k))
+  const size_t vy = (size_t)n * k;
+  double* W = (double*)arena(kArenaTrunc2, sizeof(double) * ((size_t)n * m + 2 * vy + (size_t)kQrPanel * std::max(m, k)));
+  double* Vall = W + (size_t)n * m;
+  double* Yall = Vall + vy;
+  double* Z = Yall + vy;
+  TTF_HIP(hipMemcpyAsync(W, A, sizeof(double) * (size_t)n * m, hipMemcpyDeviceToDevice, st));
+  std::vector<int64_t> off;   // panel offsets into Vall / Yall
+  int64_t used = 0;
+  for (int k0 = 0; k0 < k; k0 += pcmax) {
+    const int rows = n - k0, pc = std::min(pcmax, k - k0);
+    double* Vb = Vall + used;
+    double* Yb = Yall + used;
+    off.push_back(used);
+    used += (int64_t)rows * pc;
+    const size_t lds = sizeof(double) * (size_t)pc * (rows | 1);
+    hipLaunchKernelGGL(qr_panel_kernel, dim3(1, 1), dim3(256), lds, st, rows, pc, W + k0 + (int64_t)n * k0,
+                       (int64_t)n, Vb, Yb, (int64_t)0, (int64_t)0);
+    TTF_HIP(hipGetLastError());
+    const int nrest = m - k0 - pc;
+    if (nrest > 0) {
+      // trailing columns: C <- H^T C = C - Y (V^T C)
+      double* Cm = W + k0 + (int64_t)n * (k0 + pc);
+      gemm_launch(colmajor<double>('T', 'N', pc, nrest, rows, 1.0, Vb, rows, Cm, n, 0.0, Z, pc), st);
+      gemm_launch(colmajor<double>('N', 'N', rows, nrest, pc, -1.0, Yb, rows, Z, pc, 1.0, Cm, n), st);
+    }
+  }
+  hipLaunchKernelGGL(upper_copy_kernel, dim3((unsigned)std::min<int64_t>(ceil_div((int64_t)k * m, 256), 8192)),
+                     dim3(256), 0, st, k, m, W, (int64_t)n, R, ldr);
+  TTF_HIP(hipGetLastError());
+  // Q = H_1 ... H_P I(:, 0:k): panels applied last to first, each as
+  // (I - V T V^T) X = X - V (Y^T X) with Y = V T^T, on rows and columns k0:
+  hipLaunchKernelGGL(identity_cols_kernel, dim3((unsigned)std::min<int64_t>(ceil_div((int64_t)n * k, 256), 8192)),
+                     dim3(256), 0, st, n, k, Q);
+  TTF_HIP(hipGetLastError());
+  for (int p = (int)off.size() - 1; p >= 0; --p) {
+    const int k0 = p * pcmax, rows = n - k0, pc = std::min(pcmax, k - k0), cols = k - k0;
+    const double* Vb = Vall + off[p];
+    const double* Yb = Yall + off[p];
+    double* Qs = Q + k0 + (int64_t)n * k0;
+    gemm_launch(colmajor<double>('T', 'N', pc, cols, rows, 1.0, Yb, rows, Qs, n, 0.0, Z, pc), st);
+    gemm_launch(colmajor<double>('N', 'N', rows, cols, pc, -1.0, Vb, rows, Z, pc, 1.0, Qs, n), st);
+  }
+}
+
 int svd_vectors(int m, int n, const double* A, int64_t lda, bool want_left, double* S, double* O,
                 int64_t ldo, std::vector<double>& hs, int* kpos, hipStream_t st) {
   TTF_REQUIRE(m >= 1 && n >= 1, "svd_vectors: invalid sizes %d x %d", m, n);
@@ -1283,6 +1361,18 @@ int ttf_dsvd_side(int m, int n, const double* A, int64_t lda, char side, double*
     TTF_HIP(hipStreamSynchronize(stream()));
     if (kpos) *kpos = kp;
     if (sweeps) *sweeps = sw;
+  });
+}
+
+int ttf_dqr(int n, int m, double* A, double* R, int64_t ldr) {
+  return guarded([&] {
+    TTF_REQUIRE(n >= 1 && m >= 1 && ldr >= std::min(n, m), "ttf_dqr: invalid sizes n=%d m=%d ldr=%lld", n, m,
+                (long long)ldr);
+    hipStream_t st = stream();
+    const int k = std::min(n, m);
+    double* Q = (double*)arena(kArenaTrunc3, sizeof(double) * (size_t)n * k);
+    householder_qr(n, m, A, Q, R, ldr, st);
+    TTF_HIP(hipMemcpyAsync(A, Q, sizeof(double) * (size_t)n * k, hipMemcpyDeviceToDevice, st));
   });
 }
 

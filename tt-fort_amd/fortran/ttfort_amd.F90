@@ -3,10 +3,14 @@
 ! Build with -DTTF_FINT64 for a default-integer-8 program (the reference's
 ! i8 preset) and link libttfort_amd_i8.so; otherwise link libttfort_amd.so.
 !
-!  * dbfun3: the reference's bind(c) symbol (tt_linalg.f90:127-159) with the
-!    same by-reference argument list, so existing callers (GMRES, PRIMME and
-!    Expokit adapters) reach the GPU by linking this library in place of the
-!    reference definition.
+!  * dbfun3, dqr, drow_add, drow_cut, dtransp, dperm1324, djac_gen,
+!    djac_apply, duchol_fort, dgmresr_hh_fort: the reference's bind(c)
+!    symbols (tt_linalg.f90:30-845) with the same by-reference argument
+!    lists, so existing callers reach the GPU by linking this library in
+!    place of the reference definitions.
+!  * init_bfun_sizes, init_bfun_main, primme_matvec, zmatvec,
+!    zmatvec_transp: the callback binding of one local step and the PRIMME /
+!    Expokit callback shapes (tt_eigb.f90:15-49, tt_ksl.f90:20-90).
 !  * tt_eigb_gpu, tt_amen_solve_gpu, ztt_ksl_gpu: the argument lists of
 !    tt_eigb (tt_eigb.f90:110-120), tt_amen_solve (tt_adapt_als.f90:513-525)
 !    and ztt_ksl (tt_ksl.f90:431-445) with the same optional arguments and
@@ -59,6 +63,122 @@ module ttfort_amd
       real(c_double), intent(in) :: phi1(*), A(*), phi2(*), x(*)
       real(c_double), intent(inout) :: y(*), res1(*), res2(*)
     end subroutine dbfun3
+
+    subroutine dqr(n, m, A, R, work, lwork, tau) bind(c, name='dqr')
+      import :: fint, c_double
+      integer(fint), intent(in) :: n, m, lwork
+      real(c_double), intent(inout) :: A(*), R(*), work(*), tau(*)
+    end subroutine dqr
+
+    subroutine drow_add(m, n, k, A, B) bind(c, name='drow_add')
+      import :: fint, c_double
+      integer(fint), intent(in) :: m, n, k
+      real(c_double), intent(inout) :: A(*)
+      real(c_double), intent(in) :: B(*)
+    end subroutine drow_add
+
+    subroutine drow_cut(m, n, k, A) bind(c, name='drow_cut')
+      import :: fint, c_double
+      integer(fint), intent(in) :: m, n, k
+      real(c_double), intent(inout) :: A(*)
+    end subroutine drow_cut
+
+    subroutine dtransp(n, m, A, B) bind(c, name='dtransp')
+      import :: fint, c_double
+      integer(fint), intent(in) :: n, m
+      real(c_double), intent(in) :: A(*)
+      real(c_double), intent(inout) :: B(*)
+    end subroutine dtransp
+
+    subroutine dperm1324(n1, n2, n3, n4, A, B) bind(c, name='dperm1324')
+      import :: fint, c_double
+      integer(fint), intent(in) :: n1, n2, n3, n4
+      real(c_double), intent(in) :: A(*)
+      real(c_double), intent(inout) :: B(*)
+    end subroutine dperm1324
+
+    subroutine djac_gen(ptype, rx1, n, rx2, ra1, ra2, Phi1, A, Phi2, jacs) bind(c, name='djac_gen')
+      import :: fint, c_double, c_char
+      character(kind=c_char), intent(in) :: ptype
+      integer(fint), intent(in) :: rx1, n, rx2, ra1, ra2
+      real(c_double), intent(in) :: Phi1(*), A(*), Phi2(*)
+      real(c_double), intent(inout) :: jacs(*)
+    end subroutine djac_gen
+
+    subroutine djac_apply(ptype, rx1, n, rx2, jacs, x, y, work1) bind(c, name='djac_apply')
+      import :: fint, c_double, c_char
+      character(kind=c_char), intent(in) :: ptype
+      integer(fint), intent(in) :: rx1, n, rx2
+      real(c_double), intent(in) :: jacs(*), x(*)
+      real(c_double), intent(inout) :: y(*), work1(*)
+    end subroutine djac_apply
+
+    subroutine duchol_fort(trans, m, n, dA, numvec, du, lam, real_numvec) bind(c, name='duchol_fort')
+      import :: fint, c_double, c_char
+      character(kind=c_char), intent(in) :: trans
+      integer(fint), intent(in) :: m, n, numvec
+      real(c_double), intent(in) :: dA(*)
+      real(c_double), intent(inout) :: du(*), lam(*)
+      integer(fint), intent(inout) :: real_numvec
+    end subroutine duchol_fort
+
+    subroutine dgmresr_hh_fort(Phi1, A, Phi2, rhs, rx1, n, rx2, ra1, ra2, nrestart, tol, niters, ptype, jacs, &
+                               sol, verb) bind(c, name='dgmresr_hh_fort')
+      import :: fint, c_double, c_char
+      real(c_double), intent(in) :: Phi1(*), A(*), Phi2(*), rhs(*), jacs(*)
+      integer(fint), intent(in) :: rx1, n, rx2, ra1, ra2, nrestart, niters, verb
+      real(c_double), intent(in) :: tol
+      character(kind=c_char), intent(in) :: ptype
+      real(c_double), intent(inout) :: sol(*)
+    end subroutine dgmresr_hh_fort
+
+    ! the callback binding of one local step (tt_eigb.f90:29-49) and the
+    ! PRIMME / Expokit callback shapes
+    subroutine init_bfun_sizes(rx1, m, rx2, ry1, n, ry2, ra1, ra2, xsize, ysize) bind(c, name='ttf_init_bfun_sizes')
+      import :: fint
+      integer(fint), intent(in) :: rx1, m, rx2, ry1, n, ry2, ra1, ra2, xsize, ysize
+    end subroutine init_bfun_sizes
+
+    subroutine init_bfun_main(phi1, A, phi2) bind(c, name='ttf_init_bfun_main')
+      import :: c_double
+      real(c_double), intent(in) :: phi1(*), A(*), phi2(*)
+    end subroutine init_bfun_main
+
+    subroutine zinit_bfun_main(phi1, A, phi2) bind(c, name='ttf_zinit_bfun_main')
+      import :: c_double_complex
+      complex(c_double_complex), intent(in) :: phi1(*), A(*), phi2(*)
+    end subroutine zinit_bfun_main
+
+    subroutine primme_matvec(x, y, k, primme) bind(c, name='ttf_primme_matvec')
+      import :: fint, c_double, c_ptr
+      real(c_double), intent(in) :: x(*)
+      real(c_double), intent(inout) :: y(*)
+      integer(fint), intent(in) :: k
+      type(c_ptr), value :: primme
+    end subroutine primme_matvec
+
+    subroutine zmatvec(x, y) bind(c, name='ttf_zmatvec')
+      import :: c_double_complex
+      complex(c_double_complex), intent(in) :: x(*)
+      complex(c_double_complex), intent(inout) :: y(*)
+    end subroutine zmatvec
+
+    subroutine zmatvec_transp(x, y) bind(c, name='ttf_zmatvec_transp')
+      import :: c_double_complex
+      complex(c_double_complex), intent(in) :: x(*)
+      complex(c_double_complex), intent(inout) :: y(*)
+    end subroutine zmatvec_transp
+
+    integer(c_int) function ttf_my_chop3(n, s, eps) bind(c, name='ttf_my_chop3')
+      import :: c_int, c_double
+      integer(c_int), value :: n
+      real(c_double), intent(in) :: s(*)
+      real(c_double), value :: eps
+    end function ttf_my_chop3
+
+    type(c_ptr) function ttf_version() bind(c, name='ttf_version')
+      import :: c_ptr
+    end function ttf_version
 
     integer(c_int) function ttf_init(device) bind(c, name='ttf_init')
       import :: c_int

@@ -83,6 +83,22 @@ for _nm in ("ttf_dphi2_left", "ttf_dphi2_right"):
     _sig(_nm, _c_int, *([_c_int] * 5 + [_vp, _vp, _vp, _vp]))
 _sig("ttf_d2d_fullmat", _c_int, *([_c_int] * 5 + [_vp, _vp, _vp, _vp]))
 _sig("dbfun3", None, *([_vp] * 15))
+_sig("dqr", None, *([_vp] * 7))
+_sig("drow_add", None, *([_vp] * 5))
+_sig("drow_cut", None, *([_vp] * 4))
+_sig("dtransp", None, *([_vp] * 4))
+_sig("dperm1324", None, *([_vp] * 6))
+_sig("djac_gen", None, *([_vp] * 10))
+_sig("djac_apply", None, *([_vp] * 8))
+_sig("duchol_fort", None, *([_vp] * 8))
+_sig("dgmresr_hh_fort", None, *([_vp] * 16))
+_sig("ttf_init_bfun_sizes", None, *([_vp] * 10))
+_sig("ttf_init_bfun_main", None, _vp, _vp, _vp)
+_sig("ttf_zinit_bfun_main", None, _vp, _vp, _vp)
+_sig("ttf_primme_matvec", None, _vp, _vp, _vp, _vp)
+_sig("ttf_zmatvec", None, _vp, _vp)
+_sig("ttf_zmatvec_transp", None, _vp, _vp)
+_sig("ttf_dqr", _c_int, _c_int, _c_int, _vp, _vp, _c_i64)
 _sig("ttf_dgesvd", _c_int, _c_int, _c_int, _vp, _c_i64, _vp, _c_i64, _vp, _vp, _c_i64,
      C.POINTER(_c_int))
 _sig("ttf_dsvd_side", _c_int, _c_int, _c_int, _vp, _c_i64, C.c_char, _vp, _vp, _c_i64,
