@@ -5,10 +5,14 @@
 Prints one JSON line.  Workloads:
 
   eigb    (default) -- tt_eigb on the Heisenberg XXX chain d=64, 5 lowest
-          eigenpairs, rmax 128, tol 1e-10, rank-8 N(0,1) start (seed 0).
-          W warm-up sweeps, then K timed sweeps of the same run; one step =
-          one sweep.  `value` = algorithmic local-matvec flops of the timed
-          sweeps / their wall time; all cores and interfaces stay in HBM.
+          eigenpairs, rmax 128, tol 1e-10, rank-8 N(0,1) start (seed 0), in
+          two legs: FP64 throughout, and fp32 matrix-core products of the
+          search directions with FP64 refinement.  Each leg: W warm-up
+          sweeps, then K timed sweeps of one run; one step = one sweep.
+          `value` = algorithmic local-matvec flops of the mixed leg's timed
+          sweeps / their wall time; both legs' sweep times, ratio and final
+          FP64 eigen-residuals are in `config`.  All cores and interfaces
+          stay in HBM.
   matvec  -- one block matvec y = Phi_L A_k Phi_R x of the largest local
           problem of that sweep, (r_left, n, r_right, R) = (128, 2, 128, 5),
           block of 8 vectors (the block eigensolver's width), inputs resident
@@ -17,8 +21,9 @@ Prints one JSON line.  Workloads:
 The `roofline` object describes the dominant kernel (the MFMA GEMM of the
 slower large contraction stage at that local shape), timed with HIP events
 on the library stream over 50 launches; `traffic` is its HBM bytes per
-launch from the committed counter summary.  `cpu_baseline` times the oracle
-(numpy + host BLAS) on the same local shape on this host's cores.
+launch from the committed counter summary.  `cpu_baseline` times the
+oracle's sweep (numpy + host BLAS) on a bounded sample of the same workload
+on this host's cores (eigb), or its local matvec (matvec).
 """
 from __future__ import annotations
 
@@ -197,46 +202,138 @@ def dominant_kernel_roofline(lib, shape, nvec, iters=50):
                 stage_tflops=[round(fl[k] * nvec / (stage_ms[k] * 1e-3) / 1e12, 3) for k in range(3)])
 
 
-def run_eigb(args):
-    """Heisenberg XXX d=64, 5 lowest eigenpairs, rmax 128, tol 1e-10, rank-8
-    N(0,1) start: W warm-up sweeps then K timed sweeps of one tt_eigb run."""
+def _eigb_leg(args, mixed):
+    """One tt_eigb run of the benchmark workload: W warm-up sweeps then K
+    timed sweeps, FP64 throughout or with fp32 search-direction products
+    and FP64 refinement; the final FP64 eigen-residuals of the result are
+    then computed in TT format on the host (outside the timed region)."""
     import ttfort
     import mpo
-    lib = ttfort.load()
-    ttfort.check(lib.ttf_init(0), lib)
+    import ttops
     d, B, rmax, tol = args.eigb_d, 5, args.eigb_rmax, 1e-10
     crA, ra, n = mpo.xxz(d)
     crY, ry = mpo.random_block_tt(d, n, 8, B, seed=0)
     nsw = args.warmup + args.steps
-    t0 = time.perf_counter()
-    lam, res, ry_out, st = ttfort.tt_eigb(n, ra, crA, crY, ry, tol, rmax, B, nswp=nsw, verb=args.verb)
-    wall = time.perf_counter() - t0
-    secs = st["sweep_seconds"]
-    flops = st["sweep_flops"]
-    timed = list(range(min(args.warmup, len(secs)), len(secs)))
-    if not timed:
-        timed = list(range(len(secs)))
+    ttfort.set_local_precision(32 if mixed else 64)
+    try:
+        t0 = time.perf_counter()
+        lam, res, ry_out, st = ttfort.tt_eigb(n, ra, crA, crY, ry, tol, rmax, B, nswp=nsw, verb=args.verb)
+        wall = time.perf_counter() - t0
+    finally:
+        ttfort.set_local_precision(64)
+    secs, flops = st["sweep_seconds"], st["sweep_flops"]
+    timed = list(range(min(args.warmup, len(secs)), len(secs))) or list(range(len(secs)))
     t_timed = sum(secs[i] for i in timed)
     f_timed = sum(flops[i] for i in timed)
-    value = f_timed / t_timed / 1e12
+    mv = sum(st["sweep_matvecs"][i] for i in timed)
+    mv32 = sum(st["sweep_matvecs32"][i] for i in timed)
+    t0 = time.perf_counter()
+    resid, G = ttops.block_eig_residuals(crA, ra, n, res, ry_out, lam)
+    t_res = time.perf_counter() - t0
+    leg = dict(ms_per_sweep=round(1e3 * t_timed / len(timed), 2), tflops=round(f_timed / t_timed / 1e12, 4),
+               sweep_seconds=[round(v, 3) for v in secs], matvecs_timed=int(mv), matvecs_fp32_timed=int(mv32),
+               fp64_refreshes=st["refreshes"], lambda_=[float("%.12f" % v) for v in lam],
+               eigen_residuals=[float("%.3e" % v) for v in resid],
+               orthonormality=float("%.2e" % np.abs(G - np.eye(B)).max()),
+               sweep_ermax=[float("%.3e" % v) for v in st["sweep_ermax"]], last_ermax=st["last_ermax"],
+               converged=bool(st["converged"]), reached_tol_at_sweep=next(
+                   (i + 1 for i, e in enumerate(st["sweep_ermax"]) if e < tol), None),
+               max_rank=st["max_rank"], solve_s=round(st["solve_seconds"], 3),
+               trunc_s=round(st["trunc_seconds"], 3), phi_s=round(st["phi_seconds"], 3), wall_s=round(wall, 2),
+               host_residual_s=round(t_res, 1))
+    return leg, (crA, ra, n, res, ry_out, d, B, rmax, tol, len(timed), f_timed, t_timed)
+
+
+def cpu_baseline_sweep(ctx, budget_s):
+    """The oracle's tt_eigb (the reference's sweep restated in numpy: the
+    dbfun3 GEMM chain on host BLAS, ARPACK as the local eigensolver with the
+    reference's tolerance eps2/resid_damp) on the same operator, started
+    from the GPU run's rank-128 result with the block index moved to the
+    middle of the chain, for a bounded number of consecutive backward local
+    steps at the full local size (128, 2, 128).  The host BLAS thread count
+    is calibrated on the local matvec first and the fastest is used."""
+    from oracle import drivers as D
+    from oracle import kernels as K
+    from threadpoolctl import threadpool_limits
+    import scipy.sparse.linalg as sla
+    crA, ra, n, res, ry_out, d, B, rmax, tol, nsteps_gpu, f_gpu, t_gpu = ctx
+    shape = (rmax, 2, rmax, rmax, 2, rmax, 5, 5)
+    N = rmax * 2 * rmax
+    rng = np.random.default_rng(1)
+    P1 = rng.standard_normal((rmax, rmax, 5))
+    P2 = rng.standard_normal((rmax, 5, rmax))
+    Ac = rng.standard_normal((5, 2, 2, 5))
+    p1 = (P1 + P1.transpose(1, 0, 2)).reshape(-1, order="F")
+    p2 = (P2 + P2.transpose(2, 1, 0)).reshape(-1, order="F")
+    A = (Ac + Ac.transpose(3, 2, 1, 0)).reshape(-1, order="F")
+    op = sla.LinearOperator((N, N), dtype=float,
+                            matvec=lambda v: K.dbfun3(*shape, p1, A, p2, np.asarray(v, float).reshape(-1)))
+    v0 = rng.standard_normal(N)
+    # calibration on what the sample runs: an ARPACK restart cycle of local
+    # matvecs (BLAS threads that pay off on one large product can lose
+    # many-fold inside the eigensolver's loop)
+    best = None
+    for t in _blas_threads_options():
+        with threadpool_limits(limits=t, user_api="blas"):
+            t0 = time.perf_counter()
+            try:
+                sla.eigsh(op, k=B, which="SA", v0=v0, ncv=20, maxiter=1, tol=1e-12)
+            except sla.ArpackNoConvergence:
+                pass
+            el = time.perf_counter() - t0
+        if best is None or el < best[1]:
+            best = (t, el)
+    cores = best[0]
+    with threadpool_limits(limits=cores, user_api="blas"):
+        lam, _, _, st = D.tt_eigb(d, n, ra, crA, res, list(ry_out), tol, rmax, B, nswp=1, start_site=d // 2,
+                                  time_budget=budget_s)
+    steps = st["steps"]
+    secs = sum(s[1] for s in steps)
+    mv = sum(s[2] for s in steps)
+    fl = sum(bfun3_stage_flops(*shape)) * mv
+    full = [s for s in steps if s[0] == rmax * 2 * rmax]
+    per_step = (sum(s[1] for s in full) / len(full)) if full else secs / max(len(steps), 1)
+    gpu_per_step = t_gpu / nsteps_gpu / (2 * (d - 1))
+    return dict(value=round(fl / secs / 1e12, 6), unit="TFLOP/s", cores=int(cores), kind="port",
+                sample="%d consecutive backward local steps of the oracle's tt_eigb sweep (local eigensolve to "
+                       "eps2/resid_damp, SVD truncation, interface update) at local size %d, from the GPU "
+                       "result moved to site %d; %.1f s, %d matvecs; host BLAS on %d threads (fastest of %s)"
+                       % (len(steps), rmax * 2 * rmax, d // 2 + 1, secs, mv, cores, _blas_threads_options()),
+                s_per_local_step=round(per_step, 3), gpu_s_per_local_step=round(gpu_per_step, 5),
+                sweep_seconds_estimate=round(per_step * 2 * (d - 1), 1),
+                note="sweep estimate = mean full-size local step x %d steps (an upper estimate: the steps "
+                     "near the chain ends are smaller)" % (2 * (d - 1)))
+
+
+def run_eigb(args):
+    """Heisenberg XXX d=64, 5 lowest eigenpairs, rmax 128, tol 1e-10, rank-8
+    N(0,1) start: the FP64 leg and the mixed-precision leg (fp32 matrix-core
+    search directions, FP64 refinement), each W warm-up sweeps then K timed
+    sweeps of one tt_eigb run, with their final FP64 eigen-residuals."""
+    import ttfort
+    lib = ttfort.load()
+    ttfort.check(lib.ttf_init(0), lib)
+    legs = {}
+    leg64, ctx = _eigb_leg(args, mixed=False)
+    legs["fp64"] = leg64
+    if not args.fp64_only:
+        legs["mixed"], _ = _eigb_leg(args, mixed=True)
+    main = legs.get("mixed", leg64)
+    d, rmax = args.eigb_d, args.eigb_rmax
     roof = dominant_kernel_roofline(lib, (rmax, 2, rmax, rmax, 2, rmax, 5, 5), C2_BLOCK)
-    out = dict(metric="sweep time + local-matvec TFLOP/s (FP64 tt_eigb sweep, Heisenberg XXX d=%d)" % d,
-               value=round(value, 4), unit="TFLOP/s", n_gpus=1, steps=len(timed), warmup=args.warmup,
-               ms_per_step=round(1e3 * t_timed / len(timed), 2), higher_is_better=True, scaling="weak",
-               vs_baseline=None, dtype="f64", data="synthetic",
-               config=dict(workload="tt_eigb Heisenberg XXX d=%d B=5 rmax=%d tol=1e-10 rank-8 start"
-                           % (d, rmax), d=d, B=B, rmax=rmax, tol=tol,
-                           sweep_seconds=[round(s, 3) for s in secs],
-                           sweep_tflops=[round(f / s / 1e12, 3) for f, s in zip(flops, secs)],
-                           matvecs=st["total_matvecs"], max_rank=st["max_rank"],
-                           lambda_=[float("%.12f" % v) for v in lam], last_ermax=st["last_ermax"],
-                           solve_s=round(st["solve_seconds"], 3), trunc_s=round(st["trunc_seconds"], 3),
-                           phi_s=round(st["phi_seconds"], 3), wall_s=round(wall, 2)),
-               roofline=roof)
+    cfg = dict(workload="tt_eigb Heisenberg XXX d=%d B=5 rmax=%d tol=1e-10 rank-8 start, FP64 leg and "
+                        "mixed leg (fp32 search directions + FP64 refinement)" % (d, rmax),
+               d=d, B=5, rmax=rmax, tol=1e-10, legs=legs)
+    if "mixed" in legs:
+        cfg["mixed_speedup_vs_fp64"] = round(leg64["ms_per_sweep"] / legs["mixed"]["ms_per_sweep"], 3)
+        cfg["max_eigen_residual"] = dict(fp64=max(leg64["eigen_residuals"]),
+                                         mixed=max(legs["mixed"]["eigen_residuals"]))
+    out = dict(metric="sweep time + local-matvec TFLOP/s: mixed/learned vs FP64 GPU at equal residual",
+               value=main["tflops"], unit="TFLOP/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+               ms_per_step=main["ms_per_sweep"], higher_is_better=True, scaling="weak", vs_baseline=None,
+               dtype="f32+f64" if "mixed" in legs else "f64", data="synthetic", config=cfg, roofline=roof)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline_matvec((rmax, 2, rmax, rmax, 2, rmax, 5, 5), C2_BLOCK,
-                                                  budget_s=args.cpu_budget)
-        out["cpu_baseline"]["sample"] += " (the oracle's local matvec at the sweep's largest local shape)"
+        out["cpu_baseline"] = cpu_baseline_sweep(ctx, args.cpu_budget)
     return out
 
 
@@ -417,7 +514,8 @@ def main():
     ap.add_argument("--eigb-rmax", type=int, default=128)
     ap.add_argument("--verb", type=int, default=1, help="per-step progress lines on stderr")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--fp64-only", action="store_true", help="eigb: the FP64 leg only")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the local-solver path has no "

@@ -62,11 +62,14 @@ void ttf_clear_error(void);
 /* Library version string. */
 const char* ttf_version(void);
 
-/* Precision of the matvecs inside the iterative local solvers (the local
- * GMRES of ttf_tt_amen_solve): 64 (default, FP64 throughout) or 32 (Krylov
- * directions multiplied on the fp32 matrix cores with fp32 accumulation,
- * every restart residual rhs - B x and the solver's own residual checks in
- * FP64, i.e. FP64 iterative refinement).  Not in the reference. */
+/* Precision of the matvecs inside the iterative local solvers: 64
+ * (default, FP64 throughout) or 32.  With 32, the local GMRES of
+ * ttf_tt_amen_solve multiplies its Krylov directions on the fp32 matrix
+ * cores with fp32 accumulation while every restart residual rhs - B x and
+ * the solver's own residual checks stay FP64 (FP64 iterative refinement);
+ * the block eigensolver of ttf_tt_eigb multiplies its search directions in
+ * fp32 and recomputes A X in FP64 before any stopping decision, which is
+ * always taken on FP64 residuals.  Not in the reference. */
 int ttf_set_local_precision(int bits);
 int ttf_get_local_precision(void);
 
@@ -231,6 +234,11 @@ typedef struct {
   int converged;              /* 1 if a sweep ended with ermax < eps (the
                                  reference's exit, tt_eigb.f90:401-403), 0 if
                                  the run stopped at nswp sweeps */
+  long total_matvecs32;       /* of total_matvecs, those on the fp32 operator
+                                 (ttf_set_local_precision(32)) */
+  long refreshes;             /* FP64 recomputations of A X in the local
+                                 eigensolves (mixed precision) */
+  long sweep_matvecs32[64];   /* fp32 local matvecs of each sweep */
 } ttf_eigb_stats;
 
 /* Block eigensolver in TT format: the B lowest eigenpairs of the symmetric

@@ -55,8 +55,17 @@ def local_eigs(shape, phi1, A, phi2, x0, B, tol, max_full_size):
     return w[idx], V[:, idx], count[0]
 
 
-def tt_eigb(d, n, ra, crA, crY0, ry, eps, rmax, B, nswp=20, max_full_size=2000, verb=0):
-    """tt_eigb.f90:110-436 restated.  Returns (lambda, cores, ry, stats)."""
+def tt_eigb(d, n, ra, crA, crY0, ry, eps, rmax, B, nswp=20, max_full_size=2000, verb=0, start_site=None,
+            max_steps=None, time_budget=None):
+    """tt_eigb.f90:110-436 restated.  Returns (lambda, cores, ry, stats).
+
+    For timing a bounded sample of a sweep (not part of the reference):
+    start_site = k0 (0-based) moves the block index from the last core to
+    core k0 through the backward SVD steps without local solves, so that
+    the sweep proper begins at core k0; max_steps / time_budget (seconds)
+    stop it after that many local steps or once that much time has passed.
+    stats['steps'] then lists (local size, seconds, matvecs) per local step
+    (eigensolve, truncation and interface update)."""
     n = [int(v) for v in n]
     ra = [int(v) for v in ra]
     ry = [int(v) for v in ry]
@@ -92,16 +101,29 @@ def tt_eigb(d, n, ra, crA, crY0, ry, eps, rmax, B, nswp=20, max_full_size=2000, 
     ermax, fvold, total_mv = 0.0, 0.0, 0
     lam = np.zeros(B)
     last = 0
+    steps = []
+    dry = start_site is not None and start_site < d - 1
+    import time as _time
+    t_begin = None
     while swp <= nswp:
         shape = (ry[k], n[k], ry[k + 1], ry[k], n[k], ry[k + 1], ra[k], ra[k + 1])
         N = ry[k] * n[k] * ry[k + 1]
         x0 = K.F(cores[k], N, B)
-        lam, X, mv = local_eigs(shape, phi[k], Acore[k], phi[k + 1], x0, B, eps2 / resid_damp,
-                                max_full_size)
+        if dry and k == start_site:
+            dry = False
+        if dry:
+            X, mv = x0, 0   # moving the block index only
+        else:
+            if t_begin is None:
+                t_begin = _time.perf_counter()
+            t_step = _time.perf_counter()
+            lam, X, mv = local_eigs(shape, phi[k], Acore[k], phi[k + 1], x0, B, eps2 / resid_damp,
+                                    max_full_size)
         total_mv += mv
-        fv = lam.sum()
-        ermax = max(ermax, (fvold - fv) / abs(fv))
-        fvold = fv
+        if not dry:
+            fv = lam.sum()
+            ermax = max(ermax, (fvold - fv) / abs(fv))
+            fvold = fv
         if verb:
             print("swp %d i [%d/%d] N %d ermax %.3e" % (swp, k + 1, d, N, ermax))
         if direction < 0 and k > 0:
@@ -130,6 +152,11 @@ def tt_eigb(d, n, ra, crA, crY0, ry, eps, rmax, B, nswp=20, max_full_size=2000, 
                                      phi[k], Acore[k], cores[k], cores[k])
         else:
             cores[k] = X.reshape(-1, order="F")
+        if t_begin is not None and not dry:
+            steps.append((N, _time.perf_counter() - t_step, mv))
+            if (max_steps is not None and len(steps) >= max_steps) or \
+                    (time_budget is not None and _time.perf_counter() - t_begin >= time_budget):
+                break
         if direction > 0 and k == d - 2:
             direction, k = -1, d - 1
             last = swp
@@ -142,7 +169,7 @@ def tt_eigb(d, n, ra, crA, crY0, ry, eps, rmax, B, nswp=20, max_full_size=2000, 
         else:
             k += direction
     ry[d] = B
-    return lam, cores, ry, dict(sweeps=last, total_matvecs=total_mv)
+    return lam, cores, ry, dict(sweeps=last, total_matvecs=total_mv, steps=steps)
 
 
 def _zqr(A):

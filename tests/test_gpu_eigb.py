@@ -114,3 +114,34 @@ def test_eigb_reference_fixture_d20(ttlib, golden_dir):
         r = np.linalg.norm(Hx - lam[b] * X[:, b]) / np.linalg.norm(X[:, b])
         assert r < 1e-2, (b, r)
     assert st["max_rank"] <= 40
+
+
+@pytest.mark.parametrize("d,rmax", [(10, 160), (12, 320)])
+def test_eigb_mixed_precision_1e10(ttlib, golden_dir, d, rmax):
+    """fp32 search-direction products with FP64 refinement of A X and an
+    FP64 finish: the same 1e-10 relative eigenvalues and the same FP64
+    residuals on full vectors as the FP64 solver, with part of the local
+    matvecs on the fp32 operator."""
+    import ttfort
+    ed = json.load(open(os.path.join(golden_dir, "heisenberg_ed.json")))
+    B = 5
+    cores, ra, n = O.heisenberg_mpo(d)
+    crA = O.flatten_cores(cores)
+    rng = np.random.default_rng(d)
+    crY, ry = _initial(rng, d, n, 4, B)
+    try:
+        ttfort.set_local_precision(32)
+        lam, res, ry_out, st = ttfort.tt_eigb(n, ra, crA, crY, ry, 1e-12, rmax, B, nswp=10)
+    finally:
+        ttfort.set_local_precision(64)
+    ref = np.array(ed[str(d)][:B])
+    assert np.all(np.abs(lam - ref) <= 1e-10 * np.abs(ref)), (lam - ref) / np.abs(ref)
+    assert st["converged"] == 1
+    assert st["total_matvecs32"] > 0, (st["total_matvecs32"], st["total_matvecs"])
+    assert st["refreshes"] > 0
+    X = _block_full(_split(res, n, ry_out), n, ry_out)
+    H = O.mpo_to_dense(cores, ra, n, n)
+    for b in range(B):
+        r = np.linalg.norm(H @ X[:, b] - lam[b] * X[:, b]) / np.linalg.norm(X[:, b])
+        assert r < 1e-8, (b, r)
+    assert np.allclose(X.T @ X, np.eye(B), atol=1e-10)

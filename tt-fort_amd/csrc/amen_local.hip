@@ -290,16 +290,35 @@ void to_float(int64_t n, const double* x, float* y, hipStream_t st) {
   hipLaunchKernelGGL(d2f_kernel, dim3(grid_for(n)), dim3(256), 0, st, n, x, y);
   TTF_HIP(hipGetLastError());
 }
+// columns of a block with leading dimensions: y(:, c) = x(:, c), c < k
+__global__ void d2f_cols_kernel(int64_t rows, int k, const double* __restrict__ x, int64_t ldx,
+                                float* __restrict__ y, int64_t ldy) {
+  const int64_t total = rows * k;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e % rows, c = e / rows;
+    y[i + ldy * c] = (float)x[i + ldx * c];
+  }
+}
+__global__ void f2d_cols_kernel(int64_t rows, int k, const float* __restrict__ x, int64_t ldx,
+                                double* __restrict__ y, int64_t ldy) {
+  const int64_t total = rows * k;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e % rows, c = e / rows;
+    y[i + ldy * c] = (double)x[i + ldx * c];
+  }
+}
 }  // namespace
 
-MixedOp make_mixed(const LocalOp& op, hipStream_t st) {
+MixedOp make_mixed(const LocalOp& op, hipStream_t st, int nvec) {
   const Shape& s = op.s;
+  TTF_REQUIRE(nvec >= 1, "make_mixed: nvec=%d", nvec);
   const int64_t n1 = (int64_t)s.ry1 * s.rx1 * s.ra1, na = (int64_t)s.ra1 * s.n * s.m * s.ra2;
   const int64_t n2 = (int64_t)s.rx2 * s.ra2 * s.ry2;
-  const int64_t nx = (int64_t)s.rx1 * s.m * s.rx2, ny = (int64_t)s.ry1 * s.n * s.ry2;
+  const int64_t nx = (int64_t)s.rx1 * s.m * s.rx2 * nvec, ny = (int64_t)s.ry1 * s.n * s.ry2 * nvec;
   float* base = (float*)arena(kArenaMixed, sizeof(float) * (size_t)(n1 + na + n2 + nx + ny + 64));
   MixedOp mo;
   mo.s = s;
+  mo.nvec = nvec;
   mo.phi1 = base;
   mo.A = mo.phi1 + n1;
   mo.phi2 = mo.A + na;
@@ -316,6 +335,16 @@ void MixedOp::apply(const double* xin, double* yout, hipStream_t st) const {
   to_float(nx, xin, x, st);
   bfun3_chain<float>(s, 1, phi1, A, phi2, x, nx, y, ny, st);
   hipLaunchKernelGGL(f2d_kernel, dim3(grid_for(ny)), dim3(256), 0, st, ny, y, yout);
+  TTF_HIP(hipGetLastError());
+}
+
+void MixedOp::apply_block(int k, const double* xin, int64_t ldx, double* yout, int64_t ldy, hipStream_t st) const {
+  TTF_REQUIRE(k >= 1 && k <= nvec, "MixedOp: %d vectors for staging of %d", k, nvec);
+  const int64_t nx = (int64_t)s.rx1 * s.m * s.rx2, ny = (int64_t)s.ry1 * s.n * s.ry2;
+  hipLaunchKernelGGL(d2f_cols_kernel, dim3(grid_for(nx * k)), dim3(256), 0, st, nx, k, xin, ldx, x, nx);
+  TTF_HIP(hipGetLastError());
+  bfun3_chain<float>(s, k, phi1, A, phi2, x, nx, y, ny, st);
+  hipLaunchKernelGGL(f2d_cols_kernel, dim3(grid_for(ny * k)), dim3(256), 0, st, ny, k, y, ny, yout, ldy);
   TTF_HIP(hipGetLastError());
 }
 

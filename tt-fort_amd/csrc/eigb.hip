@@ -93,6 +93,7 @@ extern "C" int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, con
     // LOBPCG guard vectors beyond the B wanted (TTF_EIG_GUARD overrides)
     const int guard = eig_guard_vectors();
     const int nswp0 = nswp > 0 ? nswp : 20;
+    const bool mixed = local_matvec_bits() == 32;
     const int mfs = max_full_size > 0 ? max_full_size : 2000;
     // operator cores on the device
     std::vector<int64_t> pa(d + 1, 0);
@@ -157,25 +158,35 @@ extern "C" int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, con
     // ---- sweeps
     int k = d - 1, dir = -1, swp = 1;
     double ermax = 0.0, fvold = 0.0;
-    long total_mv = 0;
+    long total_mv = 0, total_mv32 = 0, total_refresh = 0;
     std::vector<double> lam(B, 0.0), hs;
     int last_swp = 0;
     std::vector<double> rec_sec, rec_flops, rec_er, rec_fv;
-    std::vector<long> rec_mv;
+    std::vector<long> rec_mv, rec_mv32;
     // ermax of the last completed sweep and whether it met eps
     double last_er = 0.0;
     bool converged = false;
     Timer tsweep;
     double sweep_flops = 0.0;
-    long sweep_mv = 0;
+    long sweep_mv = 0, sweep_mv32 = 0;
     while (swp <= nswp0) {
       const int N = ry[k] * n[k] * ry[k + 1];
       LocalOp op{Shape{ry[k], m[k], ry[k + 1], ry[k], n[k], ry[k + 1], ra[k], ra[k + 1]}, phi[k],
                  dA + pa[k], phi[k + 1]};
       Timer ts;
       const int dense_max = std::min(mfs - 1, 64);
+      // mixed precision: fp32 search-direction products, FP64 refinement
+      MixedOp mixop;
+      const MixedOp* mix = nullptr;
+      if (mixed) {
+        mixop = make_mixed(op, st, B + guard);
+        mix = &mixop;
+      }
       EigStats es = block_eig(op, B, guard, core[k], lam.data(), eps2 / resid_damp, max_matvecs,
-                              dense_max, st);
+                              dense_max, st, mix);
+      total_mv32 += es.matvecs32;
+      sweep_mv32 += es.matvecs32;
+      total_refresh += es.refreshes;
       t_solve += ts.ms();
       double fv = 0.0;
       for (int b = 0; b < B; ++b) fv += lam[b];
@@ -270,11 +281,13 @@ extern "C" int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, con
         rec_sec.push_back(tsweep.ms() * 1e-3);
         rec_flops.push_back(sweep_flops);
         rec_mv.push_back(sweep_mv);
+        rec_mv32.push_back(sweep_mv32);
         rec_er.push_back(ermax);
         rec_fv.push_back(fvold);
         tsweep = Timer();
         sweep_flops = 0.0;
         sweep_mv = 0;
+        sweep_mv32 = 0;
         last_er = ermax;
         if (ermax < eps) {
           converged = true;
@@ -318,6 +331,8 @@ extern "C" int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, con
       stats->total_matvecs = total_mv;
       stats->last_ermax = last_er;
       stats->converged = converged ? 1 : 0;
+      stats->total_matvecs32 = total_mv32;
+      stats->refreshes = total_refresh;
       int mx = 0;
       for (int q = 0; q <= d; ++q) mx = std::max(mx, ry[q]);
       stats->max_rank = mx;
@@ -332,6 +347,7 @@ extern "C" int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, con
         stats->sweep_matvecs[q] = rec_mv[q];
         stats->sweep_ermax[q] = rec_er[q];
         stats->sweep_lambda_sum[q] = rec_fv[q];
+        stats->sweep_matvecs32[q] = rec_mv32[q];
       }
     }
   });

@@ -191,7 +191,8 @@ void orth_block(int k, const double* G, double* T, int* mask, hipStream_t st) {
 __global__ __launch_bounds__(256) void rr_kernel(int m, int bs, int nx, const double* __restrict__ H,
                                                  const int* __restrict__ maskP_in, const int* __restrict__ maskW,
                                                  double* __restrict__ theta, double* __restrict__ CZ,
-                                                 int* __restrict__ maskP, double* __restrict__ amax) {
+                                                 int* __restrict__ maskP, double* __restrict__ amax,
+                                                 int lower = 0) {
   __shared__ double A[kMaxM][kMaxM + 1], V[kMaxM][kMaxM + 1], Y[kMaxM][kMaxM + 1];
   __shared__ double Z[kMaxM][kMaxM / 2 + 1];
   __shared__ double w[kMaxM], dg[kMaxM], red[16];
@@ -208,11 +209,19 @@ __global__ __launch_bounds__(256) void rr_kernel(int m, int bs, int nx, const do
   auto colok = [&](int i) {
     return i < nx ? 1 : (i < nx + np ? maskP_in[i - nx] : maskW[i - nx - np]);
   };
+  // H(i, j) = S_i^T (A S_j), symmetrised.  With `lower` (A W from the
+  // fp32 operator) each coupling off the diagonal blocks is taken from the
+  // product of the earlier group instead (X before P before W): the X-W
+  // coupling W^T (A X), of the size of the residual, then never comes from
+  // A W, whose rounding would swamp it once the residual is that small.
+  auto grp = [&](int i) { return i < nx ? 0 : (i < nx + np ? 1 : 2); };
   double f2 = 0.0;
   for (int e = tid; e < m * m; e += blockDim.x) {
     int i = e % m, j = e / m;
     bool ok = colok(i) && colok(j);
-    double h = 0.5 * (H[i + m * j] + H[j + m * i]);
+    const int gi = grp(i), gj = grp(j);
+    double h = (!lower || gi == gj) ? 0.5 * (H[i + m * j] + H[j + m * i])
+                                    : (gi > gj ? H[i + m * j] : H[j + m * i]);
     A[i][j] = ok ? h : ((i == j) ? big : 0.0);
     V[i][j] = (i == j) ? 1.0 : 0.0;
     if (ok) f2 += h * h;
@@ -419,7 +428,7 @@ static double eig_prec_delta() {
 }
 
 EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* lambda, double tol,
-                   int max_matvecs, int dense_max, hipStream_t st) {
+                   int max_matvecs, int dense_max, hipStream_t st, const MixedOp* mix) {
   EigStats stats;
   const int N = op.N();
   TTF_REQUIRE(nev >= 1 && nev <= N, "block_eig: nev=%d invalid for N=%d", nev, N);
@@ -507,6 +516,43 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
   int matvecs = bs;
   int cur = 1;
   double anorm = 0.0;
+  TTF_REQUIRE(mix == nullptr || mix->nvec >= bs, "block_eig: fp32 operator holds %d vectors, need %d",
+              mix ? mix->nvec : 0, bs);
+  // mixed precision: A X of the current set is exact (FP64) right after a
+  // refresh, until the next fp32 A W enters the Rayleigh-Ritz update
+  bool fresh = true, force_sync = false;
+  // once an exact residual is within kSwitch of the fp32 rounding level,
+  // the search directions go back to the FP64 operator for the rest of the
+  // solve: the last digits are won in FP64, as by the FP64 solver
+  bool fp64_finish = false;
+  constexpr double kSwitch = 1e-6;
+  double r_ref = -1.0;
+  // FP64 refresh of the current set: A X and A P recomputed in one block
+  // product (X and P are adjacent columns of S, A X and A P of A S), a
+  // Rayleigh-Ritz step on X alone into the other set; P and its validity
+  // carry over (a rotation within span X keeps P orthogonal to it)
+  auto refresh = [&]() {
+    double* X = Sset(cur);
+    double* AX = X + colsz * 3 * bs;
+    op.apply(2 * bs, X, ld, AX, ld, st);
+    matvecs += 2 * bs;
+    ++stats.refreshes;
+    tsgram(N, bs, bs, X, ld, AX, ld, Hm, bs, st);
+    hipLaunchKernelGGL(rr_kernel, dim3(1), dim3(256), 0, st, bs, bs, bs, Hm, maskPb[cur & 1], maskW, theta, CZ,
+                       maskPb[(cur + 1) & 1], amax);
+    double* Xn = Sset(1 - cur);
+    mul_small(N, bs, bs, X, ld, CZ, bs, Xn, ld, 0.0, st);
+    mul_small(N, bs, bs, AX, ld, CZ, bs, Xn + colsz * 3 * bs, ld, 0.0, st);
+    TTF_HIP(hipMemcpyAsync(Xn + colsz * bs, X + colsz * bs, sizeof(double) * colsz * bs, hipMemcpyDeviceToDevice,
+                           st));
+    TTF_HIP(hipMemcpyAsync(Xn + colsz * 4 * bs, X + colsz * 4 * bs, sizeof(double) * colsz * bs,
+                           hipMemcpyDeviceToDevice, st));
+    TTF_HIP(hipMemcpyAsync(maskPb[(cur + 1) & 1], maskPb[cur & 1], sizeof(int) * kMaxM, hipMemcpyDeviceToDevice,
+                           st));
+    cur = 1 - cur;
+    fresh = true;
+    force_sync = true;   // the next residuals are exact: read them
+  };
   std::vector<double> hbuf(2 * kMaxM + 1);
   // the residual norms are read back (a host synchronisation) at the first
   // iterations and then every kSyncEvery: in between, iterations are queued
@@ -535,7 +581,8 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
                        res_part, nrm);
     TTF_HIP(hipGetLastError());
     const bool at_cap = matvecs + bs > max_matvecs;
-    if (it < 3 || it % kSyncEvery == 0 || at_cap) {
+    if (it < 3 || it % kSyncEvery == 0 || at_cap || force_sync) {
+      force_sync = false;
       read_back(hbuf.data(), theta, sizeof(double) * (2 * kMaxM + 1), st);
       anorm = std::max(anorm, hbuf[2 * kMaxM]);   // amax follows nrm in the buffer
       double res2 = 0.0, rmax = 0.0;
@@ -545,7 +592,23 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
       }
       stats.iters = it;
       stats.res = std::sqrt(res2);
-      if (rmax <= tol * std::max(anorm, 1e-300) || at_cap) break;
+      const bool done = rmax <= tol * std::max(anorm, 1e-300) || at_cap;
+      const double rswitch = kSwitch * std::max(anorm, 1e-300);
+      if (mix && !fresh) {
+        // decide on FP64 residuals: refresh at the tolerance or the cap, at
+        // the switch level, and whenever the maintained residuals fell
+        // 1000-fold since the last exact ones (the fp32 errors in A X scale
+        // with the residual at the last refresh)
+        if (done || rmax <= 1e-3 * r_ref || rmax <= rswitch) {
+          refresh();
+          continue;
+        }
+      }
+      if (done) break;
+      if (fresh) {
+        r_ref = rmax;
+        if (mix && rmax <= rswitch) fp64_finish = true;
+      }
     }
     // Wr <- Wr - [X P] ([X P]^T Wr), twice (P is zero before the first update)
     for (int pass = 0; pass < 2; ++pass) {
@@ -556,13 +619,20 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
     tsgram(N, bs, bs, Wr, ld, Wr, ld, Gm, bs, st);
     orth_block(bs, Gm, Tm, maskW, st);
     mul_small(N, bs, bs, Wr, ld, Tm, bs, W, ld, 0.0, st);
-    op.apply(bs, W, ld, AW, ld, st);
+    const bool aw32 = mix && !fp64_finish;
+    if (aw32) {
+      mix->apply_block(bs, W, ld, AW, ld, st);
+      stats.matvecs32 += bs;
+      fresh = false;
+    } else {
+      op.apply(bs, W, ld, AW, ld, st);
+    }
     matvecs += bs;
     // Rayleigh-Ritz on S = [X P W]
     const int m = 3 * bs;
     tsgram(N, m, m, S, ld, AS, ld, Hm, m, st);
     hipLaunchKernelGGL(rr_kernel, dim3(1), dim3(256), 0, st, m, bs, bs, Hm, maskPb[cur & 1], maskW, theta, CZ,
-                       maskPb[(cur + 1) & 1], amax);
+                       maskPb[(cur + 1) & 1], amax, aw32 ? 1 : 0);
     // next set: [X P] = S [C Z] and [AX AP] = AS [C Z], one batched product
     double* Sn = Sset(1 - cur);
     GemmOp<double> g = colmajor<double>('N', 'N', N, 2 * bs, m, 1.0, S, ld, CZ, m, 0.0, Sn, ld);

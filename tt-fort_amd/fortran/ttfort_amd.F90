@@ -40,6 +40,8 @@ module ttfort_amd
     integer(c_long) :: sweep_matvecs(64)
     real(c_double) :: sweep_ermax(64), sweep_lambda_sum(64)
     integer(c_int) :: converged
+    integer(c_long) :: total_matvecs32, refreshes
+    integer(c_long) :: sweep_matvecs32(64)
   end type ttf_eigb_stats
 
   type, bind(c) :: ttf_amen_stats

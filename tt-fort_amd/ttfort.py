@@ -113,7 +113,8 @@ class EigbStats(C.Structure):
                 ("trunc_seconds", C.c_double), ("phi_seconds", C.c_double), ("nrec", C.c_int),
                 ("sweep_seconds", C.c_double * 64), ("sweep_flops", C.c_double * 64),
                 ("sweep_matvecs", C.c_long * 64), ("sweep_ermax", C.c_double * 64),
-                ("sweep_lambda_sum", C.c_double * 64), ("converged", C.c_int)]
+                ("sweep_lambda_sum", C.c_double * 64), ("converged", C.c_int),
+                ("total_matvecs32", C.c_long), ("refreshes", C.c_long), ("sweep_matvecs32", C.c_long * 64)]
 
     def as_dict(self):
         out = {}
