@@ -354,96 +354,136 @@ def _tt_expect(d, n, ra, crA, cr, ry):
 
 def run_ksl(args):
     """Transverse-field Ising chain g=1, d=40, A = -iH, all-up product state
-    zero-padded to fixed TT rank 64, dt 0.05, Krylov order 30: W untimed then
-    K timed ztt_ksl steps (one step = one typ-2 symmetric KSL step; the
-    driver twin takes and returns host cores, so each step includes their
-    upload and download, about 5 MB each way)."""
+    zero-padded to fixed TT rank 64, dt 0.05, Krylov order 30: two legs, the
+    core products of the local exponentials in complex128 and in complex64
+    (Krylov basis, Hessenberg exponential and S-steps complex128 in both);
+    each W untimed then K timed ztt_ksl steps (one step = one typ-2
+    symmetric KSL step; the driver twin takes and returns host cores, so
+    each step includes their upload and download, about 5 MB each way).
+    The complex64 leg's final state is compared with the complex128 one."""
     import ttfort
     import mpo
+    import ttops
     lib = ttfort.load()
     ttfort.check(lib.ttf_init(0), lib)
     d, r, tau, order = 40, 64, 0.05, 30
     crA, ra, n = mpo.tfim_schroedinger(d)
-    cr, ry = mpo.product_state_padded(d, r)
+    cr0, ry0 = mpo.product_state_padded(d, r)
     sz0 = int(ra[0] * n[0] * n[0] * ra[1])
     crH = np.concatenate([1j * crA[:sz0], crA[sz0:]])      # H = i A (A carries -i on core 1)
-    e0 = _tt_expect(d, n, ra, crH, cr, ry)
-    for _ in range(args.warmup):
-        cr, ry, _ = ttfort.ztt_ksl(n, ra, crA, cr, ry, tau, order=order)
-    t0 = time.perf_counter()
-    mv = 0
-    fl = 0.0
-    for _ in range(args.steps):
-        cr, ry, st = ttfort.ztt_ksl(n, ra, crA, cr, ry, tau, order=order)
-        mv += st["matvecs"]
-        fl += st["flops"]
-    wall = time.perf_counter() - t0
-    e1 = _tt_expect(d, n, ra, crH, cr, ry)
-    ms = 1e3 * wall / args.steps
-    return dict(metric="KSL step time + local-matvec TFLOP/s (complex128 ztt_ksl, TFIM d=40 r=64)",
-                value=round(fl / wall / 1e12, 4), unit="TFLOP/s", n_gpus=1, steps=args.steps,
-                warmup=args.warmup, ms_per_step=round(ms, 2), higher_is_better=True, scaling="weak",
-                vs_baseline=None, dtype="c128", data="synthetic",
-                config=dict(workload="ztt_ksl TFIM g=1 d=40 rank 64 dt=0.05 order 30 all-up start",
-                            d=d, r=r, tau=tau, order=order, matvecs=mv,
-                            energy_start=round(e0.real, 10), energy_end=round(e1.real, 10),
-                            host_transfer_bytes_per_step=int(2 * 16 * cr.size), wall_s=round(wall, 2)))
+    e0 = _tt_expect(d, n, ra, crH, cr0, ry0)
+    legs, finals = {}, {}
+    for name, bits in (("c128", 64), ("c64", 32)):
+        cr, ry = cr0, ry0
+        ttfort.set_local_precision(bits)
+        try:
+            for _ in range(args.warmup):
+                cr, ry, _ = ttfort.ztt_ksl(n, ra, crA, cr, ry, tau, order=order)
+            t0 = time.perf_counter()
+            mv = mv32 = 0
+            fl = 0.0
+            for step in range(args.steps):
+                cr, ry, st = ttfort.ztt_ksl(n, ra, crA, cr, ry, tau, order=order)
+                if args.verb:
+                    print("ksl %s step %d: %.3f s, %d matvecs" % (name, step, st["seconds"], st["matvecs"]),
+                          file=sys.stderr, flush=True)
+                mv += st["matvecs"]
+                mv32 += st["matvecs32"]
+                fl += st["flops"]
+            wall = time.perf_counter() - t0
+        finally:
+            ttfort.set_local_precision(64)
+        e1 = _tt_expect(d, n, ra, crH, cr, ry)
+        finals[name] = (cr, ry)
+        legs[name] = dict(ms_per_step=round(1e3 * wall / args.steps, 2), tflops=round(fl / wall / 1e12, 4),
+                          matvecs=mv, matvecs_c64=mv32, energy_start=round(e0.real, 10),
+                          energy_end=round(e1.real, 10), wall_s=round(wall, 2))
+    (ca, ra_), (cb, rb_) = finals["c128"], finals["c64"]
+    A = ttops.split(ca, ra_, n)
+    Bc = ttops.split(cb, rb_, n)
+    diff = ttops.norm(ttops.sub(Bc, A)) / ttops.norm(A)
+    main = legs["c64"]
+    return dict(metric="sweep time + local-matvec TFLOP/s: mixed/learned vs FP64 GPU at equal residual",
+                value=main["tflops"], unit="TFLOP/s", n_gpus=1, steps=args.steps,
+                warmup=args.warmup, ms_per_step=main["ms_per_step"], higher_is_better=True, scaling="weak",
+                vs_baseline=None, dtype="c64+c128", data="synthetic",
+                config=dict(workload="ztt_ksl TFIM g=1 d=40 rank 64 dt=0.05 order 30 all-up start, complex128 "
+                                     "and complex64 core products", d=d, r=r, tau=tau, order=order, legs=legs,
+                            c64_speedup_vs_c128=round(legs["c128"]["ms_per_step"] / main["ms_per_step"], 3),
+                            c64_state_rel_diff_vs_c128=float("%.3e" % diff),
+                            host_transfer_bytes_per_step=int(2 * 16 * cr0.size)))
 
 
 def run_amen(args):
     """Variable-coefficient diffusion -div(a grad u) = 1 in 6D, n=64 per mode,
     a = 1 + 0.5 tanh(sum c_k x_k) (c_k ~ U(-1,1), seed 0) as a rank-3 TT
     coefficient, 7-point stencil TT-matrix of rank 6; rank-1 all-ones rhs,
-    rank-2 N(0,1) start (seed 1), tol 1e-8, max rank 64, block-Jacobi 'c',
-    nrestart 40, niters 2, kick 4.  W untimed solves, then K timed solves to
-    convergence; a step is one solve, and the global FP64 residual of the
-    last result is computed in TT format on the host."""
+    rank-2 N(0,1) start (seed 1), tol 1e-8, max rank 64, nrestart 40,
+    niters 2, kick 4.  Legs: block-Jacobi 'c' in FP64, block-Jacobi 'c' with
+    fp32 Krylov matvecs and FP64 refinement, and the learned preconditioner
+    'm' in FP64 and mixed (when its parameters are present; --prec picks a
+    single preconditioner).  Each leg: W untimed solves, then K timed solves
+    to convergence (a step is one solve); the global FP64 residual of each
+    leg's result is computed in TT format on the host."""
     import ttfort
     import mpo
     import ttops
     lib = ttfort.load()
     ttfort.check(lib.ttf_init(0), lib)
-    if args.prec == "m":
-        import learned
-        if learned.load(args.model) is None:
-            raise SystemExit("bench.py: --prec m needs trained parameters at %s "
-                             "(python tt-fort_amd/learned.py)" % args.model)
-    ttfort.set_local_precision(32 if args.mixed else 64)
     d, n, tol, rmax = 6, 64, 1e-8, 64
     crA, ra, nn, info = mpo.diffusion_7pt(d, n, seed=0, coef_rank=3)
     crY, ry = mpo.ones_tt(d, n)
     crX, rx = mpo.random_tt(d, n, 2, seed=1)
+    precs = [args.prec] if args.prec_given else ["c", "m"]
+    if "m" in precs:
+        import learned
+        if learned.load(args.model) is None:
+            if args.prec_given:
+                raise SystemExit("bench.py: --prec m needs trained parameters at %s "
+                                 "(python tt-fort_amd/learned.py)" % args.model)
+            precs = [p for p in precs if p != "m"]
+    legs = {}
+    for prec in precs:
+        for mixed in ((args.mixed,) if args.prec_given and args.mixed else (False, True)):
+            ttfort.set_local_precision(32 if mixed else 64)
 
-    def solve():
-        return ttfort.tt_amen_solve(nn, ra, crA, ry, crY, rx, crX, tol, kickrank=4, nswp=20, prec=args.prec,
-                                    nrestart=40, niters=2, rmax=rmax, verb=args.verb)
-
-    for _ in range(args.warmup):
-        solve()
-    secs, flops, sweeps = 0.0, 0.0, 0
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        res, rx_out, st = solve()
-        secs += st["seconds"]
-        flops += st["flops"]
-        sweeps += st["sweeps"]
-    wall = time.perf_counter() - t0
-    resid = ttops.residual(crA, ra, nn, res, rx_out, crY, ry)
-    prec_name = "mixed fp32-matvec/FP64-refinement" if args.mixed else "FP64"
-    return dict(metric="sweep time + local-matvec TFLOP/s (%s AMEn solve, 6D variable-coefficient diffusion)"
-                       % prec_name,
-                value=round(flops / secs / 1e12, 4), unit="TFLOP/s", n_gpus=1, steps=args.steps,
-                warmup=args.warmup, ms_per_step=round(1e3 * wall / args.steps, 1), higher_is_better=True,
-                scaling="weak", vs_baseline=None, dtype="f32+f64" if args.mixed else "f64", data="synthetic",
-                config=dict(workload="tt_amen_solve 6D diffusion n=64 coef rank 3 (operator rank 6) tol 1e-8 "
-                                     "rmax 64 prec %s%s" % (args.prec, " mixed" if args.mixed else ""),
-                            d=d, n=n, tol=tol, rmax=rmax, prec=args.prec, sweeps_per_solve=sweeps / args.steps,
-                            ms_per_sweep=round(1e3 * secs / max(sweeps, 1), 1), max_rank=st["max_rank"],
-                            matvecs=st["total_matvecs"], coef_maxerr=info["coef_maxerr"],
-                            global_residual=resid, last_local_residual=st["last_res_max"],
-                            learned_solves=st["learned_solves"], learned_fallbacks=st["learned_fallbacks"],
-                            solve_s=round(st["solve_seconds"], 3), trunc_s=round(st["trunc_seconds"], 3),
-                            phi_s=round(st["phi_seconds"], 3), wall_s=round(wall, 2)))
+            def solve():
+                return ttfort.tt_amen_solve(nn, ra, crA, ry, crY, rx, crX, tol, kickrank=4, nswp=20, prec=prec,
+                                            nrestart=40, niters=2, rmax=rmax, verb=args.verb)
+            try:
+                for _ in range(args.warmup):
+                    solve()
+                secs, flops, sweeps = 0.0, 0.0, 0
+                t0 = time.perf_counter()
+                for _ in range(args.steps):
+                    res, rx_out, st = solve()
+                    secs += st["seconds"]
+                    flops += st["flops"]
+                    sweeps += st["sweeps"]
+                wall = time.perf_counter() - t0
+            finally:
+                ttfort.set_local_precision(64)
+            resid = ttops.residual(crA, ra, nn, res, rx_out, crY, ry)
+            legs["%s_%s" % (prec, "mixed" if mixed else "fp64")] = dict(
+                ms_per_solve=round(1e3 * wall / args.steps, 1), tflops=round(flops / secs / 1e12, 4),
+                sweeps_per_solve=sweeps / args.steps, ms_per_sweep=round(1e3 * secs / max(sweeps, 1), 1),
+                max_rank=st["max_rank"], matvecs=st["total_matvecs"], global_residual=float("%.3e" % resid),
+                last_local_residual=st["last_res_max"], learned_solves=st["learned_solves"],
+                learned_fallbacks=st["learned_fallbacks"], solve_s=round(st["solve_seconds"], 3),
+                trunc_s=round(st["trunc_seconds"], 3), phi_s=round(st["phi_seconds"], 3))
+    base = legs.get("c_fp64") or next(iter(legs.values()))
+    best_key = min(legs, key=lambda k: legs[k]["ms_per_solve"])
+    best = legs[best_key]
+    return dict(metric="sweep time + local-matvec TFLOP/s: mixed/learned vs FP64 GPU at equal residual",
+                value=best["tflops"], unit="TFLOP/s", n_gpus=1, steps=args.steps,
+                warmup=args.warmup, ms_per_step=best["ms_per_solve"], higher_is_better=True,
+                scaling="weak", vs_baseline=None, dtype="f32+f64" if "mixed" in best_key else "f64",
+                data="synthetic",
+                config=dict(workload="tt_amen_solve 6D diffusion n=64 coef rank 3 (operator rank 6) tol 1e-8 rmax 64",
+                            d=d, n=n, tol=tol, rmax=rmax, coef_maxerr=info["coef_maxerr"], legs=legs,
+                            fastest_leg=best_key,
+                            speedup_vs_c_fp64={k: round(base["ms_per_solve"] / v["ms_per_solve"], 3)
+                                               for k, v in legs.items()}))
 
 
 def run_batch(args):
@@ -505,7 +545,8 @@ def main():
     ap.add_argument("--workers", type=int, default=16, help="batch: concurrent worker streams")
     ap.add_argument("--hw-queues", type=int, default=None,
                     help="batch: GPU_MAX_HW_QUEUES for this process (4..32; default: the runtime's)")
-    ap.add_argument("--prec", default="c", help="AMEn local preconditioner: n, c, l, r, m (learned)")
+    ap.add_argument("--prec", default=None,
+                    help="AMEn local preconditioner: n, c, l, r, m (learned); default: legs c and m")
     ap.add_argument("--mixed", action="store_true",
                     help="AMEn: fp32 Krylov matvecs with FP64 refinement in the local GMRES")
     ap.add_argument("--model", default=os.path.join(ROOT, "tt-fort_amd", "models", "mlp_precond_n64.npy"),
@@ -517,6 +558,9 @@ def main():
     ap.add_argument("--fp64-only", action="store_true", help="eigb: the FP64 leg only")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     args = ap.parse_args()
+    args.prec_given = args.prec is not None
+    if args.prec is None:
+        args.prec = "c"
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the local-solver path has no "
               "multi-GPU mode); run with --gpus 1", file=sys.stderr)

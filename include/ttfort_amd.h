@@ -49,6 +49,7 @@ typedef int32_t ttf_fint;
 #endif
 
 typedef struct { double re, im; } ttf_zdouble;
+typedef struct { float re, im; } ttf_zfloat;
 
 /* ------------------------------------------------------------------ runtime */
 /* Select the device and create the library stream (idempotent).  Every other
@@ -69,7 +70,12 @@ const char* ttf_version(void);
  * the solver's own residual checks stay FP64 (FP64 iterative refinement);
  * the block eigensolver of ttf_tt_eigb multiplies its search directions in
  * fp32 and recomputes A X in FP64 before any stopping decision, which is
- * always taken on FP64 residuals.  Not in the reference. */
+ * always taken on FP64 residuals; the local exponentials of ttf_ztt_ksl
+ * multiply the Krylov vectors by a complex64 copy of the local operator
+ * (S-steps, norm estimates, Krylov basis, Hessenberg matrix and its
+ * exponential stay complex128) at local tolerance max(1e-8, 1e-6), the
+ * floor the complex64 rounding leaves the Krylov error estimate.  Not in
+ * the reference. */
 int ttf_set_local_precision(int bits);
 int ttf_get_local_precision(void);
 
@@ -138,6 +144,12 @@ int ttf_zbfun3(int rx1, int m, int rx2, int ry1, int n, int ry2, int ra1, int ra
                int nvec, const ttf_zdouble* phi1, const ttf_zdouble* A,
                const ttf_zdouble* phi2, const ttf_zdouble* x, int64_t ldx,
                ttf_zdouble* y, int64_t ldy);
+/* Complex64 version (complex64 operands and intermediates on the fp32
+ * matrix cores with fp32 accumulation): the reduced-precision matvec of the
+ * KSL local exponentials (tt_ksl.f90:25-32 zmatvec). */
+int ttf_cbfun3(int rx1, int m, int rx2, int ry1, int n, int ry2, int ra1, int ra2,
+               int nvec, const ttf_zfloat* phi1, const ttf_zfloat* A, const ttf_zfloat* phi2,
+               const ttf_zfloat* x, int64_t ldx, ttf_zfloat* y, int64_t ldy);
 /* y = B^H x for the zbfun3 operator B (ttals.f90:85-108, zbfun3_transp);
  * x has shape (ry1,n,ry2), y has shape (rx1,m,rx2). */
 int ttf_zbfun3_transp(int rx1, int m, int rx2, int ry1, int n, int ry2, int ra1, int ra2,
@@ -448,6 +460,8 @@ typedef struct {
   long krylov_steps;  /* ZGEXPV integration steps */
   double seconds;     /* wall time of the call */
   double flops;       /* algorithmic real flops of those operator applications */
+  long matvecs32;     /* of matvecs, the core products in complex64
+                         (ttf_set_local_precision(32)) */
 } ttf_ksl_stats;
 
 /* One KSL time step for dY/dt = A Y in TT format, argument layout of

@@ -64,6 +64,22 @@ def test_sbfun3_fp32(ttlib, shape):
         assert np.linalg.norm(got - ref) <= 2e-5 * np.linalg.norm(ref) + 1e-6 * _scale(shape, t)
 
 
+@pytest.mark.parametrize("shape", SHAPES[2:])
+def test_cbfun3_complex64(ttlib, shape):
+    """The complex64 chain (fp32 matrix cores, fp32 accumulation) against the
+    oracle's complex128 zbfun3: 2e-5 relative, the FP32 bar."""
+    import ttfort
+    rng = np.random.default_rng(13)
+    t = _rand(rng, shape, 2, cplx=True)
+    nx = shape[0] * shape[1] * shape[2]
+    ny = shape[3] * shape[4] * shape[5]
+    y = ttfort.bfun3(shape, t["phi1"], t["A"], t["phi2"], t["x"], nvec=2, dtype=np.complex64)
+    for v in range(2):
+        ref = K.zbfun3(*shape, t["phi1"], t["A"], t["phi2"], t["x"][v * nx:(v + 1) * nx])
+        got = y[v * ny:(v + 1) * ny]
+        assert np.linalg.norm(got - ref) <= 2e-5 * np.linalg.norm(ref) + 4e-6 * _scale(shape, t)
+
+
 @pytest.mark.parametrize("shape", SHAPES[:6])
 def test_zbfun3_and_transp(ttlib, shape):
     import ttfort

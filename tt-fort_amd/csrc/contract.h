@@ -37,6 +37,11 @@ void bfun3_stage12(const Shape& s, int nvec, const T* A, const T* phi2, const T*
 void zbfun3_chain(const Shape& s, int nvec, const hipDoubleComplex* phi1, const hipDoubleComplex* A,
                   const hipDoubleComplex* phi2, const hipDoubleComplex* x, int64_t ldx,
                   hipDoubleComplex* y, int64_t ldy, hipStream_t st);
+// The same chain in complex64 (complex64 operands and intermediates, fp32
+// matrix cores with fp32 accumulation).
+void cbfun3_chain(const Shape& s, int nvec, const hipFloatComplex* phi1, const hipFloatComplex* A,
+                  const hipFloatComplex* phi2, const hipFloatComplex* x, int64_t ldx, hipFloatComplex* y,
+                  int64_t ldy, hipStream_t st);
 void zbfun3_transp_chain(const Shape& s, int nvec, const hipDoubleComplex* phi1,
                          const hipDoubleComplex* A, const hipDoubleComplex* phi2,
                          const hipDoubleComplex* x, int64_t ldx, hipDoubleComplex* y, int64_t ldy,

@@ -117,40 +117,63 @@ typedef hipDoubleComplex zc;
 static const zc kZ1 = {1.0, 0.0};
 static const zc kZ0 = {0.0, 0.0};
 
-static ZGemmOp zop() {
-  ZGemmOp o{};
-  o.nb0 = 1; o.nbatch = 1; o.alpha = kZ1; o.beta = kZ0;
+template <typename R>
+static ZGemmOpT<R> zop() {
+  ZGemmOpT<R> o{};
+  o.nb0 = 1; o.nbatch = 1;
+  o.alpha = CplxT<R>::make(R(1), R(0));
+  o.beta = CplxT<R>::make(R(0), R(0));
   return o;
 }
 
-void zbfun3_chain(const Shape& s, int nvec, const zc* phi1, const zc* A, const zc* phi2,
-                  const zc* x, int64_t ldx, zc* y, int64_t ldy, hipStream_t st) {
+static void zlaunch(const ZGemmOp& g, hipStream_t st) { zgemm_launch(g, st); }
+static void zlaunch(const CGemmOp& g, hipStream_t st) { cgemm_launch(g, st); }
+
+// The complex chain in either precision (complex128 or complex64 operands,
+// intermediates and accumulation).
+template <typename R>
+static void zbfun3_chain_t(const Shape& s, int nvec, const typename CplxT<R>::type* phi1,
+                           const typename CplxT<R>::type* A, const typename CplxT<R>::type* phi2,
+                           const typename CplxT<R>::type* x, int64_t ldx, typename CplxT<R>::type* y, int64_t ldy,
+                           hipStream_t st) {
+  typedef typename CplxT<R>::type C_t;
   if (nvec <= 0) return;
   const int64_t sz1 = (int64_t)s.rx1 * s.m * s.ra2 * s.ry2;
   const int64_t sz2 = (int64_t)s.rx1 * s.ra1 * s.n * s.ry2;
-  zc* T1 = (zc*)arena(kArenaBfun, sizeof(zc) * (size_t)(sz1 + sz2) * nvec);
-  zc* T2 = T1 + sz1 * nvec;
-  ZGemmOp g1 = zop();
+  C_t* T1 = (C_t*)arena(kArenaBfun, sizeof(C_t) * (size_t)(sz1 + sz2) * nvec);
+  C_t* T2 = T1 + sz1 * nvec;
+  ZGemmOpT<R> g1 = zop<R>();
   g1.M = s.rx1 * s.m; g1.N = s.ra2 * s.ry2; g1.K = s.rx2;
   g1.A = x; g1.sAi = 1; g1.sAk = (int64_t)s.rx1 * s.m; g1.sAb0 = ldx;
   g1.B = phi2; g1.sBk = 1; g1.sBj = s.rx2;
   g1.C = T1; g1.sCi = 1; g1.sCj = (int64_t)s.rx1 * s.m; g1.sCb0 = sz1;
   g1.nb0 = nvec; g1.nbatch = nvec;
-  zgemm_launch(g1, st);
-  ZGemmOp g2 = zop();
+  zlaunch(g1, st);
+  ZGemmOpT<R> g2 = zop<R>();
   g2.M = s.rx1; g2.N = s.ra1 * s.n; g2.K = s.m * s.ra2;
   g2.A = T1; g2.sAi = 1; g2.sAk = s.rx1; g2.sAb0 = (int64_t)s.rx1 * s.m * s.ra2; g2.sAb1 = sz1;
   g2.B = A; g2.sBk = (int64_t)s.ra1 * s.n; g2.sBj = 1;
   g2.C = T2; g2.sCi = 1; g2.sCj = s.rx1; g2.sCb0 = (int64_t)s.rx1 * s.ra1 * s.n; g2.sCb1 = sz2;
   g2.nb0 = s.ry2; g2.nbatch = s.ry2 * nvec;
-  zgemm_launch(g2, st);
-  ZGemmOp g3 = zop();
+  zlaunch(g2, st);
+  ZGemmOpT<R> g3 = zop<R>();
   g3.M = s.ry1; g3.N = s.n * s.ry2; g3.K = s.rx1 * s.ra1;
   g3.A = phi1; g3.sAi = 1; g3.sAk = s.ry1;
   g3.B = T2; g3.sBk = 1; g3.sBj = (int64_t)s.rx1 * s.ra1; g3.sBb0 = sz2;
   g3.C = y; g3.sCi = 1; g3.sCj = s.ry1; g3.sCb0 = ldy;
   g3.nb0 = nvec; g3.nbatch = nvec;
-  zgemm_launch(g3, st);
+  zlaunch(g3, st);
+}
+
+void zbfun3_chain(const Shape& s, int nvec, const zc* phi1, const zc* A, const zc* phi2,
+                  const zc* x, int64_t ldx, zc* y, int64_t ldy, hipStream_t st) {
+  zbfun3_chain_t<double>(s, nvec, phi1, A, phi2, x, ldx, y, ldy, st);
+}
+
+void cbfun3_chain(const Shape& s, int nvec, const hipFloatComplex* phi1, const hipFloatComplex* A,
+                  const hipFloatComplex* phi2, const hipFloatComplex* x, int64_t ldx, hipFloatComplex* y,
+                  int64_t ldy, hipStream_t st) {
+  zbfun3_chain_t<float>(s, nvec, phi1, A, phi2, x, ldx, y, ldy, st);
 }
 
 // y = B^H x (ttals.f90:85-108): x is (c1,i,c2), y is (b1,j,b2).
@@ -164,21 +187,21 @@ void zbfun3_transp_chain(const Shape& s, int nvec, const zc* phi1, const zc* A, 
   const int64_t sz2 = (int64_t)s.rx1 * s.m * s.ra2 * s.ry2;
   zc* R1 = (zc*)arena(kArenaBfun, sizeof(zc) * (size_t)(sz1 + sz2) * nvec);
   zc* R2 = R1 + sz1 * nvec;
-  ZGemmOp g1 = zop();
+  ZGemmOp g1 = zop<double>();
   g1.M = s.rx1 * s.ra1; g1.N = s.n * s.ry2; g1.K = s.ry1;
   g1.A = phi1; g1.sAi = s.ry1; g1.sAk = 1; g1.conjA = 1;
   g1.B = x; g1.sBk = 1; g1.sBj = s.ry1; g1.sBb0 = ldx;
   g1.C = R1; g1.sCi = 1; g1.sCj = (int64_t)s.rx1 * s.ra1; g1.sCb0 = sz1;
   g1.nb0 = nvec; g1.nbatch = nvec;
   zgemm_launch(g1, st);
-  ZGemmOp g2 = zop();
+  ZGemmOp g2 = zop<double>();
   g2.M = s.rx1; g2.N = s.m * s.ra2; g2.K = s.ra1 * s.n;
   g2.A = R1; g2.sAi = 1; g2.sAk = s.rx1; g2.sAb0 = (int64_t)s.rx1 * s.ra1 * s.n; g2.sAb1 = sz1;
   g2.B = A; g2.sBk = 1; g2.sBj = (int64_t)s.ra1 * s.n; g2.conjB = 1;
   g2.C = R2; g2.sCi = 1; g2.sCj = s.rx1; g2.sCb0 = (int64_t)s.rx1 * s.m * s.ra2; g2.sCb1 = sz2;
   g2.nb0 = s.ry2; g2.nbatch = s.ry2 * nvec;
   zgemm_launch(g2, st);
-  ZGemmOp g3 = zop();
+  ZGemmOp g3 = zop<double>();
   g3.M = s.rx1 * s.m; g3.N = s.rx2; g3.K = s.ra2 * s.ry2;
   g3.A = R2; g3.sAi = 1; g3.sAk = (int64_t)s.rx1 * s.m; g3.sAb0 = sz2;
   g3.B = phi2; g3.sBk = s.rx2; g3.sBj = 1; g3.conjB = 1;
@@ -246,20 +269,20 @@ void zphi_left_chain(const Shape& s, const zc* phi1, const zc* A, const zc* x, c
   const int64_t sz2 = (int64_t)s.rx1 * s.m * s.ra2 * s.ry2;
   zc* R1 = (zc*)arena(kArenaBfun, sizeof(zc) * (size_t)(sz1 + sz2));
   zc* R2 = R1 + sz1;
-  ZGemmOp g1 = zop();
+  ZGemmOp g1 = zop<double>();
   g1.M = s.rx1 * s.ra1; g1.N = s.n * s.ry2; g1.K = s.ry1;
   g1.A = phi1; g1.sAi = s.ry1; g1.sAk = 1;
   g1.B = y; g1.sBk = 1; g1.sBj = s.ry1; g1.conjB = 1;
   g1.C = R1; g1.sCi = 1; g1.sCj = (int64_t)s.rx1 * s.ra1;
   zgemm_launch(g1, st);
-  ZGemmOp g2 = zop();
+  ZGemmOp g2 = zop<double>();
   g2.M = s.rx1; g2.N = s.m * s.ra2; g2.K = s.ra1 * s.n;
   g2.A = R1; g2.sAi = 1; g2.sAk = s.rx1; g2.sAb0 = (int64_t)s.rx1 * s.ra1 * s.n;
   g2.B = A; g2.sBk = 1; g2.sBj = (int64_t)s.ra1 * s.n;
   g2.C = R2; g2.sCi = 1; g2.sCj = s.rx1; g2.sCb0 = (int64_t)s.rx1 * s.m * s.ra2;
   g2.nb0 = s.ry2; g2.nbatch = s.ry2;
   zgemm_launch(g2, st);
-  ZGemmOp g3 = zop();
+  ZGemmOp g3 = zop<double>();
   g3.M = s.ry2; g3.N = s.rx2; g3.K = s.rx1 * s.m;
   g3.A = R2; g3.sAi = (int64_t)s.rx1 * s.m * s.ra2; g3.sAk = 1; g3.sAb0 = (int64_t)s.rx1 * s.m;
   g3.B = x; g3.sBk = 1; g3.sBj = (int64_t)s.rx1 * s.m;
@@ -274,20 +297,20 @@ void zphi_right_chain(const Shape& s, const zc* phi2, const zc* A, const zc* x, 
   const int64_t sz2 = (int64_t)s.rx1 * s.ra1 * s.n * s.ry2;
   zc* T1 = (zc*)arena(kArenaBfun, sizeof(zc) * (size_t)(sz1 + sz2));
   zc* T2 = T1 + sz1;
-  ZGemmOp g1 = zop();
+  ZGemmOp g1 = zop<double>();
   g1.M = s.rx1 * s.m; g1.N = s.ra2 * s.ry2; g1.K = s.rx2;
   g1.A = x; g1.sAi = 1; g1.sAk = (int64_t)s.rx1 * s.m;
   g1.B = phi2; g1.sBk = 1; g1.sBj = s.rx2;
   g1.C = T1; g1.sCi = 1; g1.sCj = (int64_t)s.rx1 * s.m;
   zgemm_launch(g1, st);
-  ZGemmOp g2 = zop();
+  ZGemmOp g2 = zop<double>();
   g2.M = s.rx1; g2.N = s.ra1 * s.n; g2.K = s.m * s.ra2;
   g2.A = T1; g2.sAi = 1; g2.sAk = s.rx1; g2.sAb0 = (int64_t)s.rx1 * s.m * s.ra2;
   g2.B = A; g2.sBk = (int64_t)s.ra1 * s.n; g2.sBj = 1;
   g2.C = T2; g2.sCi = 1; g2.sCj = s.rx1; g2.sCb0 = (int64_t)s.rx1 * s.ra1 * s.n;
   g2.nb0 = s.ry2; g2.nbatch = s.ry2;
   zgemm_launch(g2, st);
-  ZGemmOp g3 = zop();
+  ZGemmOp g3 = zop<double>();
   g3.M = s.rx1 * s.ra1; g3.N = s.ry1; g3.K = s.n * s.ry2;
   g3.A = T2; g3.sAi = 1; g3.sAk = (int64_t)s.rx1 * s.ra1;
   g3.B = y; g3.sBk = s.ry1; g3.sBj = 1; g3.conjB = 1;

@@ -371,7 +371,9 @@ template size_t gemm_workspace<float>(const GemmOp<float>&);
 template void gemm_launch_ws<double>(GemmOp<double>, double*, size_t, hipStream_t);
 template void gemm_launch_ws<float>(GemmOp<float>, float*, size_t, hipStream_t);
 
-void zgemm_launch(ZGemmOp op, hipStream_t st) {
+template <typename R>
+static void zgemm_launch_t(ZGemmOpT<R> op, hipStream_t st) {
+  typedef typename CplxT<R>::type C_t;
   if (op.M <= 0 || op.N <= 0 || op.nbatch <= 0) return;
   if (op.nb0 <= 0) op.nb0 = 1;
   TTF_REQUIRE(op.nbatch <= 65535, "zgemm: batch count %d exceeds the grid limit", op.nbatch);
@@ -385,27 +387,29 @@ void zgemm_launch(ZGemmOp op, hipStream_t st) {
     const int ks = (int)std::min<int64_t>({8, op.K / 48, ceil_div(64, tiles * op.nbatch)});
     if (ks > 1) {
       op.ksplit = ks;
-      op.work = (hipDoubleComplex*)arena(kArenaSplitK, sizeof(hipDoubleComplex) * (size_t)ks * op.nbatch *
-                                                           (size_t)op.M * op.N);
+      op.work = (C_t*)arena(kArenaSplitK, sizeof(C_t) * (size_t)ks * op.nbatch * (size_t)op.M * op.N);
     }
   }
   dim3 grid((unsigned)tiles, (unsigned)op.nbatch, (unsigned)op.ksplit);
   // 32x32 tiles come with few work-groups, each one's K loop latency-bound
   // on its global loads: a 32-deep K chunk halves the trips
   if (c.bm == 64)
-    hipLaunchKernelGGL((zgemm_mfma_kernel<64, 64>), grid, dim3(256), 0, st, op);
+    hipLaunchKernelGGL((zgemm_mfma_kernel<R, 64, 64>), grid, dim3(256), 0, st, op);
   else if (op.K > 16)
-    hipLaunchKernelGGL((zgemm_mfma_kernel<32, 32, 32>), grid, dim3(256), 0, st, op);
+    hipLaunchKernelGGL((zgemm_mfma_kernel<R, 32, 32, 32>), grid, dim3(256), 0, st, op);
   else
-    hipLaunchKernelGGL((zgemm_mfma_kernel<32, 32>), grid, dim3(256), 0, st, op);
+    hipLaunchKernelGGL((zgemm_mfma_kernel<R, 32, 32>), grid, dim3(256), 0, st, op);
   TTF_HIP(hipGetLastError());
   if (op.ksplit > 1) {
     const int64_t total = (int64_t)op.M * op.N * op.nbatch;
-    hipLaunchKernelGGL(zgemm_splitk_reduce, dim3((unsigned)std::min<int64_t>(ceil_div(total, 256), 4096)), dim3(256),
-                       0, st, op);
+    hipLaunchKernelGGL(zgemm_splitk_reduce<R>, dim3((unsigned)std::min<int64_t>(ceil_div(total, 256), 4096)),
+                       dim3(256), 0, st, op);
     TTF_HIP(hipGetLastError());
   }
 }
+
+void zgemm_launch(ZGemmOp op, hipStream_t st) { zgemm_launch_t<double>(op, st); }
+void cgemm_launch(CGemmOp op, hipStream_t st) { zgemm_launch_t<float>(op, st); }
 
 long gemm_tuned_shapes() {
   std::lock_guard<std::mutex> lk(g_tune_mu);

@@ -270,34 +270,54 @@ __global__ void gemm_splitk_reduce(GemmOp<T> op) {
 }
 
 // ---------------------------------------------------------------- complex
-// Complex double GEMM: real and imaginary planes are staged separately in
-// LDS and each complex multiply-add is four real MFMAs.  conjA / conjB
-// conjugate the operand on load.
-struct ZGemmOp {
+// Complex GEMM (complex128 on the FP64 matrix cores, complex64 on the FP32
+// ones): real and imaginary planes are staged separately in LDS and each
+// complex multiply-add is four real MFMAs.  conjA / conjB conjugate the
+// operand on load.
+template <typename R> struct CplxT;
+template <> struct CplxT<double> {
+  typedef hipDoubleComplex type;
+  typedef double2 vec;
+  __device__ __host__ static type make(double re, double im) { return make_hipDoubleComplex(re, im); }
+};
+template <> struct CplxT<float> {
+  typedef hipFloatComplex type;
+  typedef float2 vec;
+  __device__ __host__ static type make(float re, float im) { return make_hipFloatComplex(re, im); }
+};
+
+template <typename R>
+struct ZGemmOpT {
+  typedef typename CplxT<R>::type C_t;
   int M, N, K;
-  const hipDoubleComplex* A; int64_t sAi, sAk, sAb0, sAb1;
-  const hipDoubleComplex* B; int64_t sBk, sBj, sBb0, sBb1;
-  hipDoubleComplex* C;       int64_t sCi, sCj, sCb0, sCb1;
+  const C_t* A; int64_t sAi, sAk, sAb0, sAb1;
+  const C_t* B; int64_t sBk, sBj, sBb0, sBb1;
+  C_t* C;       int64_t sCi, sCj, sCb0, sCb1;
   int nb0, nbatch;
   int conjA, conjB;
-  hipDoubleComplex alpha, beta;
+  C_t alpha, beta;
   // split-K (ksplit > 1): slice z of K goes to work + (z nbatch + b) M N,
   // column-major and without alpha / beta; zgemm_splitk_reduce finishes C
   int ksplit;
-  hipDoubleComplex* work;
+  C_t* work;
 };
+typedef ZGemmOpT<double> ZGemmOp;   // complex128
+typedef ZGemmOpT<float> CGemmOp;    // complex64
 
-template <int BM, int BN, int BK = 16>
-__global__ __launch_bounds__(256) void zgemm_mfma_kernel(ZGemmOp op) {
+template <typename R, int BM, int BN, int BK = 16>
+__global__ __launch_bounds__(256) void zgemm_mfma_kernel(ZGemmOpT<R> op) {
   static_assert(BM * BK % 256 == 0 && BN * BK % 256 == 0 && BK % 4 == 0, "zgemm tile");
+  typedef typename CplxT<R>::type C_t;
+  typedef typename CplxT<R>::vec V2;
+  typedef typename Acc4<R>::type acc_t;
   constexpr int PAD = 2;
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int FM = WM / 16, FN = WN / 16;
   constexpr int LA = BM * BK / 256;
   constexpr int LB = BN * BK / 256;
 
-  __shared__ double Ar[BK][BM + PAD], Ai[BK][BM + PAD];
-  __shared__ double Br[BK][BN + PAD], Bi[BK][BN + PAD];
+  __shared__ R Ar[BK][BM + PAD], Ai[BK][BM + PAD];
+  __shared__ R Br[BK][BN + PAD], Bi[BK][BN + PAD];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -305,15 +325,15 @@ __global__ __launch_bounds__(256) void zgemm_mfma_kernel(ZGemmOp op) {
   const int tile = xcd_remap(blockIdx.x, tilesM * tilesN);
   const int tm = tile / tilesN, tn = tile % tilesN;
   const int b = blockIdx.y, b0 = b % op.nb0, b1 = b / op.nb0;
-  const hipDoubleComplex* __restrict__ A = op.A + b0 * op.sAb0 + b1 * op.sAb1;
-  const hipDoubleComplex* __restrict__ B = op.B + b0 * op.sBb0 + b1 * op.sBb1;
+  const C_t* __restrict__ A = op.A + b0 * op.sAb0 + b1 * op.sAb1;
+  const C_t* __restrict__ B = op.B + b0 * op.sBb0 + b1 * op.sBb1;
   const bool a_ifast = (op.sAi == 1), b_jfast = (op.sBj == 1);
   const int ks = op.ksplit > 1 ? op.ksplit : 1;
   const int kchunk = ((op.K + ks - 1) / ks + BK - 1) / BK * BK;
   const int kbeg = blockIdx.z * kchunk, kend = min(op.K, kbeg + kchunk);
-  const double sa = op.conjA ? -1.0 : 1.0, sb = op.conjB ? -1.0 : 1.0;
+  const R sa = op.conjA ? R(-1) : R(1), sb = op.conjB ? R(-1) : R(1);
 
-  double2 ra[LA], rb[LB];
+  V2 ra[LA], rb[LB];
   int ai[LA], ak[LA], bk[LB], bj[LB];
 #pragma unroll
   for (int q = 0; q < LA; ++q) {
@@ -330,25 +350,33 @@ __global__ __launch_bounds__(256) void zgemm_mfma_kernel(ZGemmOp op) {
     for (int q = 0; q < LA; ++q) {
       int gi = tm * BM + ai[q], gk = k0 + ak[q];
       if (gi < op.M && gk < kend) {
-        hipDoubleComplex v = A[gi * op.sAi + gk * op.sAk];
-        ra[q] = make_double2(v.x, sa * v.y);
-      } else ra[q] = make_double2(0.0, 0.0);
+        C_t v = A[gi * op.sAi + gk * op.sAk];
+        ra[q].x = v.x;
+        ra[q].y = sa * v.y;
+      } else {
+        ra[q].x = R(0);
+        ra[q].y = R(0);
+      }
     }
 #pragma unroll
     for (int q = 0; q < LB; ++q) {
       int gj = tn * BN + bj[q], gk = k0 + bk[q];
       if (gj < op.N && gk < kend) {
-        hipDoubleComplex v = B[gk * op.sBk + gj * op.sBj];
-        rb[q] = make_double2(v.x, sb * v.y);
-      } else rb[q] = make_double2(0.0, 0.0);
+        C_t v = B[gk * op.sBk + gj * op.sBj];
+        rb[q].x = v.x;
+        rb[q].y = sb * v.y;
+      } else {
+        rb[q].x = R(0);
+        rb[q].y = R(0);
+      }
     }
   };
 
-  d4_t cr[FM][FN], ci[FM][FN];
+  acc_t cr[FM][FN], ci[FM][FN];
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < FN; ++j) { cr[i][j] = d4_t{0, 0, 0, 0}; ci[i][j] = d4_t{0, 0, 0, 0}; }
+    for (int j = 0; j < FN; ++j) { cr[i][j] = acc_t{0, 0, 0, 0}; ci[i][j] = acc_t{0, 0, 0, 0}; }
 
   if (kbeg < kend) load(kbeg);
   for (int k0 = kbeg; k0 < kend; k0 += BK) {
@@ -361,7 +389,7 @@ __global__ __launch_bounds__(256) void zgemm_mfma_kernel(ZGemmOp op) {
     if (k0 + BK < kend) load(k0 + BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
-      double ar[FM], ai_[FM], nai[FM], br[FN], bi[FN];
+      R ar[FM], ai_[FM], nai[FM], br[FN], bi[FN];
 #pragma unroll
       for (int i = 0; i < FM; ++i) {
         int r = kk + (lane >> 4), c = wm * WM + i * 16 + (lane & 15);
@@ -384,68 +412,70 @@ __global__ __launch_bounds__(256) void zgemm_mfma_kernel(ZGemmOp op) {
     }
   }
   if (ks > 1) {   // raw partial sums of this K slice
-    hipDoubleComplex* __restrict__ Wz = op.work + ((int64_t)blockIdx.z * op.nbatch + b) * op.M * op.N;
+    C_t* __restrict__ Wz = op.work + ((int64_t)blockIdx.z * op.nbatch + b) * op.M * op.N;
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
       for (int j = 0; j < FN; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int gi = tm * BM + wm * WM + i * 16 + mfma_row<double>(lane, r);
+          const int gi = tm * BM + wm * WM + i * 16 + mfma_row<R>(lane, r);
           const int gj = tn * BN + wn * WN + j * 16 + (lane & 15);
-          if (gi < op.M && gj < op.N) Wz[gi + (int64_t)gj * op.M] = make_hipDoubleComplex(cr[i][j][r], ci[i][j][r]);
+          if (gi < op.M && gj < op.N) Wz[gi + (int64_t)gj * op.M] = CplxT<R>::make(cr[i][j][r], ci[i][j][r]);
         }
     return;
   }
-  hipDoubleComplex* __restrict__ C = op.C + b0 * op.sCb0 + b1 * op.sCb1;
-  const bool beta0 = (op.beta.x == 0.0 && op.beta.y == 0.0);
+  C_t* __restrict__ C = op.C + b0 * op.sCb0 + b1 * op.sCb1;
+  const bool beta0 = (op.beta.x == R(0) && op.beta.y == R(0));
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
     for (int j = 0; j < FN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        int gi = tm * BM + wm * WM + i * 16 + mfma_row<double>(lane, r);
+        int gi = tm * BM + wm * WM + i * 16 + mfma_row<R>(lane, r);
         int gj = tn * BN + wn * WN + j * 16 + (lane & 15);
         if (gi < op.M && gj < op.N) {
-          hipDoubleComplex* c = C + gi * op.sCi + gj * op.sCj;
-          double xr = cr[i][j][r], xi = ci[i][j][r];
-          double vr = op.alpha.x * xr - op.alpha.y * xi;
-          double vi = op.alpha.x * xi + op.alpha.y * xr;
+          C_t* c = C + gi * op.sCi + gj * op.sCj;
+          R xr = cr[i][j][r], xi = ci[i][j][r];
+          R vr = op.alpha.x * xr - op.alpha.y * xi;
+          R vi = op.alpha.x * xi + op.alpha.y * xr;
           if (!beta0) {
-            hipDoubleComplex o = *c;
+            C_t o = *c;
             vr += op.beta.x * o.x - op.beta.y * o.y;
             vi += op.beta.x * o.y + op.beta.y * o.x;
           }
-          *c = make_hipDoubleComplex(vr, vi);
+          *c = CplxT<R>::make(vr, vi);
         }
       }
 }
 
 // C = alpha (sum of the ksplit partial slices, in slice order) + beta C.
-static __global__ void zgemm_splitk_reduce(ZGemmOp op) {
+template <typename R>
+__global__ void zgemm_splitk_reduce(ZGemmOpT<R> op) {
+  typedef typename CplxT<R>::type C_t;
   const int64_t mn = (int64_t)op.M * op.N, total = mn * op.nbatch;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int b = (int)(t / mn);
     const int64_t e = t - (int64_t)b * mn;
     const int gj = (int)(e / op.M), gi = (int)(e - (int64_t)gj * op.M);
-    double xr = 0.0, xi = 0.0;
+    R xr = R(0), xi = R(0);
     for (int z = 0; z < op.ksplit; ++z) {
-      const hipDoubleComplex v = op.work[((int64_t)z * op.nbatch + b) * mn + e];
+      const C_t v = op.work[((int64_t)z * op.nbatch + b) * mn + e];
       xr += v.x;
       xi += v.y;
     }
     const int b0 = b % op.nb0, b1 = b / op.nb0;
-    hipDoubleComplex* c = op.C + b0 * op.sCb0 + b1 * op.sCb1 + gi * op.sCi + gj * op.sCj;
-    double vr = op.alpha.x * xr - op.alpha.y * xi;
-    double vi = op.alpha.x * xi + op.alpha.y * xr;
-    if (op.beta.x != 0.0 || op.beta.y != 0.0) {
-      const hipDoubleComplex o = *c;
+    C_t* c = op.C + b0 * op.sCb0 + b1 * op.sCb1 + gi * op.sCi + gj * op.sCj;
+    R vr = op.alpha.x * xr - op.alpha.y * xi;
+    R vi = op.alpha.x * xi + op.alpha.y * xr;
+    if (op.beta.x != R(0) || op.beta.y != R(0)) {
+      const C_t o = *c;
       vr += op.beta.x * o.x - op.beta.y * o.y;
       vi += op.beta.x * o.y + op.beta.y * o.x;
     }
-    *c = make_hipDoubleComplex(vr, vi);
+    *c = CplxT<R>::make(vr, vi);
   }
 }
 
@@ -469,6 +499,7 @@ size_t gemm_workspace(const GemmOp<T>& op);
 template <typename T>
 void gemm_launch_ws(GemmOp<T> op, T* ws, size_t ws_cap, hipStream_t st);
 void zgemm_launch(ZGemmOp op, hipStream_t st);
+void cgemm_launch(CGemmOp op, hipStream_t st);
 
 // Convenience constructor for a column-major GEMM with optional transposes:
 // C (M x N, leading dim ldc) = alpha * op(A) op(B) + beta * C.

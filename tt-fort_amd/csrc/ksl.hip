@@ -412,6 +412,26 @@ __global__ void zfill_kernel(int64_t n, zc v, zc* x) {
 }
 
 // B (cols x rows) = A^T for the column-major rows x cols A (no conjugation)
+// complex128 <-> complex64 (round to nearest / exact widening)
+__global__ void z2c_kernel(int64_t n, const zc* __restrict__ x, hipFloatComplex* __restrict__ y) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    y[e] = make_hipFloatComplex((float)x[e].x, (float)x[e].y);
+}
+__global__ void c2z_kernel(int64_t n, const hipFloatComplex* __restrict__ x, zc* __restrict__ y) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    y[e] = make_hipDoubleComplex((double)x[e].x, (double)x[e].y);
+}
+void to_c64(int64_t n, const zc* x, hipFloatComplex* y, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(z2c_kernel, dim3(grid_for(n)), dim3(256), 0, st, n, x, y);
+  TTF_HIP(hipGetLastError());
+}
+void to_c128(int64_t n, const hipFloatComplex* x, zc* y, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(c2z_kernel, dim3(grid_for(n)), dim3(256), 0, st, n, x, y);
+  TTF_HIP(hipGetLastError());
+}
+
 __global__ void ztranspose_kernel(int rows, int cols, const zc* __restrict__ A, zc* __restrict__ B) {
   const int64_t total = (int64_t)rows * cols;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
@@ -686,6 +706,44 @@ void zqr(int rows, int cols, const zc* A, zc* Q, zc* R, hipStream_t st) {
 
 // The S-step operator of bond k (zsfun_matvec, tt_ksl.f90:130-136) and its
 // adjoint: Sy (ry1 x ry2) = phi1 (ry1 x rx1 ra) [Sx (rx1 x rx2) phi2 (rx2 x ra ry2)].
+// Local tolerance of the exponentials whose core products are complex64.
+constexpr double kC64Tol = 1e-6;
+
+// complex64 copy of one local operator (the KSL local exponential's matvec
+// in reduced precision): interfaces and core rounded once per local step,
+// each product rounds x, runs the chain on the fp32 matrix cores and widens
+// y back; the Krylov basis, Hessenberg matrix and exponential stay
+// complex128.
+struct CMixedOp {
+  Shape s;
+  hipFloatComplex *phi1 = nullptr, *A = nullptr, *phi2 = nullptr, *x = nullptr, *y = nullptr;
+  void apply(const zc* xin, zc* yout, hipStream_t st) const {
+    const int64_t nx = (int64_t)s.rx1 * s.m * s.rx2, ny = (int64_t)s.ry1 * s.n * s.ry2;
+    to_c64(nx, xin, x, st);
+    cbfun3_chain(s, 1, phi1, A, phi2, x, nx, y, ny, st);
+    to_c128(ny, y, yout, st);
+  }
+};
+
+CMixedOp make_cmixed(const Shape& s, const zc* phi1, const zc* A, const zc* phi2, hipStream_t st) {
+  const int64_t n1 = (int64_t)s.ry1 * s.rx1 * s.ra1, na = (int64_t)s.ra1 * s.n * s.m * s.ra2;
+  const int64_t n2 = (int64_t)s.rx2 * s.ra2 * s.ry2;
+  const int64_t nx = (int64_t)s.rx1 * s.m * s.rx2, ny = (int64_t)s.ry1 * s.n * s.ry2;
+  hipFloatComplex* base =
+      (hipFloatComplex*)arena(kArenaMixed, sizeof(hipFloatComplex) * (size_t)(n1 + na + n2 + nx + ny + 64));
+  CMixedOp mo;
+  mo.s = s;
+  mo.phi1 = base;
+  mo.A = mo.phi1 + n1;
+  mo.phi2 = mo.A + na;
+  mo.x = mo.phi2 + n2;
+  mo.y = mo.x + nx;
+  to_c64(n1, phi1, mo.phi1, st);
+  to_c64(na, A, mo.A, st);
+  to_c64(n2, phi2, mo.phi2, st);
+  return mo;
+}
+
 struct SOp {
   int rx1, ry1, ra, rx2, ry2;
   const zc* phi1;
@@ -775,8 +833,11 @@ extern "C" int ttf_ztt_ksl(int d, const int* n, const int* m, const int* ra, con
     zc* St = bS.get((size_t)R * R);
     zc* phit = bphit.get((size_t)R * R * ramax);
     zc* work = bwork.get(core_cap + (size_t)R * R * ramax);
-    long nmult = 0, nsteps = 0, nsolves = 0;
+    long nmult = 0, nsteps = 0, nsolves = 0, nmult32 = 0;
     double flops = 0.0;   // algorithmic real flops of the Krylov operator applications
+    // complex64 local matvecs in the local exponentials (the S-steps, the
+    // norm estimates and everything but the core products stay complex128)
+    const bool c64 = local_matvec_bits() == 32;
     // initial left-to-right QR and left interfaces (tt_ksl.f90:538-556)
     for (int i = 0; i < d - 1; ++i) {
       const int rows = ry[i] * n[i], cols = ry[i + 1];
@@ -800,7 +861,17 @@ extern "C" int ttf_ztt_ksl(int d, const int* n, const int* m, const int* ra, con
       ZOp op = [&, s, p1, Ai, p2, N](const zc* x, zc* y) { zbfun3_chain(s, 1, p1, Ai, p2, x, N, y, N, st); };
       ZOp adj = [&, s, p1, Ai, p2, N](const zc* x, zc* y) { zbfun3_transp_chain(s, 1, p1, Ai, p2, x, N, y, N, st); };
       const double anorm = usenrm == 2 ? 1.0 : std::max(hager_norm1(N, op, adj, st), 1e-300);
-      ExpInfo info = zexp_mv(N, order, t, v, w, tol, anorm, op, st);
+      ExpInfo info;
+      if (c64) {
+        // the Krylov error estimate cannot fall below the complex64 rounding
+        // of the products, so these exponentials run at tolerance kC64Tol
+        const CMixedOp mo = make_cmixed(s, p1, Ai, p2, st);
+        ZOp op32 = [&, mo](const zc* x, zc* y) { mo.apply(x, y, st); };
+        info = zexp_mv(N, order, t, v, w, std::max(tol, kC64Tol), anorm, op32, st);
+        nmult32 += info.nmult;
+      } else {
+        info = zexp_mv(N, order, t, v, w, tol, anorm, op, st);
+      }
       nmult += info.nmult;
       nsteps += info.nstep;
       ++nsolves;
@@ -891,6 +962,7 @@ extern "C" int ttf_ztt_ksl(int d, const int* n, const int* m, const int* ra, con
       stats->matvecs = nmult;
       stats->krylov_steps = nsteps;
       stats->flops = flops;
+      stats->matvecs32 = nmult32;
       stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
   });
@@ -934,3 +1006,17 @@ int ttf_zqr(int rows, int cols, const ttf_zdouble* A, ttf_zdouble* Q, ttf_zdoubl
 }
 
 }  // extern "C"
+
+extern "C" int ttf_cbfun3(int rx1, int m, int rx2, int ry1, int n, int ry2, int ra1, int ra2, int nvec,
+                          const ttf_zfloat* phi1, const ttf_zfloat* A, const ttf_zfloat* phi2, const ttf_zfloat* x,
+                          int64_t ldx, ttf_zfloat* y, int64_t ldy) {
+  return guarded([&] {
+    TTF_REQUIRE(rx1 > 0 && m > 0 && rx2 > 0 && ry1 > 0 && n > 0 && ry2 > 0 && ra1 > 0 && ra2 > 0,
+                "ttf_cbfun3: invalid sizes");
+    TTF_REQUIRE(ldx >= (int64_t)rx1 * m * rx2 || nvec <= 1, "ttf_cbfun3: ldx too small");
+    TTF_REQUIRE(ldy >= (int64_t)ry1 * n * ry2 || nvec <= 1, "ttf_cbfun3: ldy too small");
+    cbfun3_chain(Shape{rx1, m, rx2, ry1, n, ry2, ra1, ra2}, nvec, (const hipFloatComplex*)phi1,
+                 (const hipFloatComplex*)A, (const hipFloatComplex*)phi2, (const hipFloatComplex*)x, ldx,
+                 (hipFloatComplex*)y, ldy, stream());
+  });
+}

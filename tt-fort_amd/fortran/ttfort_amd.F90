@@ -56,6 +56,7 @@ module ttfort_amd
   type, bind(c) :: ttf_ksl_stats
     integer(c_long) :: nsolves, matvecs, krylov_steps
     real(c_double) :: seconds, flops
+    integer(c_long) :: matvecs32
   end type ttf_ksl_stats
 
   interface

@@ -73,7 +73,7 @@ _sig("ttf_timer_start", _c_int)
 _sig("ttf_timer_stop", _c_int, C.POINTER(C.c_float))
 _sig("ttf_set_autotune", _c_int, _c_int)
 _sig("ttf_autotune_count", C.c_long)
-for _nm in ("ttf_dbfun3", "ttf_dbfun3_noalloc", "ttf_sbfun3", "ttf_zbfun3", "ttf_zbfun3_transp"):
+for _nm in ("ttf_dbfun3", "ttf_dbfun3_noalloc", "ttf_sbfun3", "ttf_zbfun3", "ttf_zbfun3_transp", "ttf_cbfun3"):
     _sig(_nm, _c_int, *(_S8 + [_c_int, _vp, _vp, _vp, _vp, _c_i64, _vp, _c_i64]))
 _sig("ttf_dbfun3_profile", _c_int, *(_S8 + [_c_int, _vp, _vp, _vp, _vp, _c_i64, _vp, _c_i64, _c_int,
                                            C.POINTER(C.c_float)]))
@@ -158,7 +158,7 @@ _sig("ttf_tt_amen_solve", _c_int, _c_int, _IP, _IP, _IP, _IP, _DP, _DP, _DP, _IP
 
 class KslStats(C.Structure):
     _fields_ = [("nsolves", C.c_long), ("matvecs", C.c_long), ("krylov_steps", C.c_long), ("seconds", C.c_double),
-                ("flops", C.c_double)]
+                ("flops", C.c_double), ("matvecs32", C.c_long)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -297,7 +297,7 @@ def bfun3(shape, phi1, A, phi2, x, nvec=1, dtype=np.float64, transp=False):
     dx = _dev(x, dt)
     dy = DeviceArray((ny * nvec,), dt)
     fn = {np.dtype(np.float64): lib.ttf_dbfun3, np.dtype(np.float32): lib.ttf_sbfun3,
-          np.dtype(np.complex128): lib.ttf_zbfun3}[dt]
+          np.dtype(np.complex128): lib.ttf_zbfun3, np.dtype(np.complex64): lib.ttf_cbfun3}[dt]
     if transp:
         fn = lib.ttf_zbfun3_transp
     check(fn(*shape, nvec, dphi1.ptr, dA.ptr, dphi2.ptr, dx.ptr, nx, dy.ptr, ny), lib)
