@@ -268,6 +268,30 @@ int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, const double* 
                 int kickrank, int nswp, int max_full_size, int verb, double* result,
                 int64_t result_cap, int64_t* result_size, ttf_eigb_stats* stats);
 
+typedef struct {
+  int sweeps;           /* sweeps performed */
+  double last_err_max;  /* largest relative core change of the last sweep */
+  int converged;        /* 1 if that was below eps (tt_adapt_als.f90:441-447) */
+  int max_rank;         /* largest rank of y at exit */
+  double seconds;       /* wall time of the call */
+  double flops;         /* algorithmic flops of the local projections */
+} ttf_mvk4_stats;
+
+/* Approximate TT-matrix by TT-vector product y ~ A x by alternating
+ * projections with enrichment (AMEn-mv), argument layout of tt_mvk4
+ * (tt_adapt_als.f90:67-500): d modes, A (ra_k, n_k, m_k, ra_k+1) cores in
+ * crA, x (rx_k, m_k, rx_k+1) cores in crX, the initial y0 (ry_k, n_k,
+ * ry_k+1) in crY0 with ranks ry (in/out), eps the accuracy; the truncation
+ * is my_chop3 at eps / sqrt(d), the enrichment min(kickrank, n ry) pivoted
+ * Cholesky directions of A_k Phi_k+1 x_k per backward step.  rmax sets the
+ * rank capacity (the reference uses it only to size work arrays): a rank
+ * beyond max(rmax, max ry0) is an error.  nswp <= 0 means 20.  The cores of
+ * y (the reference's tt_adapt_als::result_core) go to `result`.  Host
+ * pointers; device-resident run. */
+int ttf_tt_mvk4(int d, const int* n, const int* m, const int* rx, const int* ra, const double* crA,
+                const double* crX, const double* crY0, int* ry, double eps, int rmax, int kickrank, int nswp,
+                int verb, double* result, int64_t result_cap, int64_t* result_size, ttf_mvk4_stats* stats);
+
 /* Many independent tt_eigb problems solved concurrently on one GPU (no
  * counterpart in the reference, which is not re-entrant: tt_eigb.f90:6-8).
  * Chain c has d[c] modes; its mode sizes (n = m) are n_all[sum_{c'<c} d],
@@ -313,6 +337,12 @@ int ttf_residual_rank_rule(int k, const double* errs, double res_new, double eps
 int ttf_amen_truncation_rank(int dir, int rx1, int n, int rx2, int ra1, int ra2, const double* phi1,
                              const double* A, const double* phi2, const double* x, const double* rhs,
                              double res_new, double eps2, int* rank, double* errs);
+
+/* The Frobenius chop of the typed solvers' truncation (svd.f90:86-111):
+ * the smallest rank r with sum_{k>r} s_k^2 < (tol ||s||)^2, found by the
+ * reference's scan from the end; rmax > 0 caps r first (its tail counted
+ * as discarded), tol < 0 skips the tolerance.  Host arrays; needs no GPU. */
+int ttf_chop(int n, const double* s, double tol, int rmax);
 
 /* ------------------------------------------------- AMEn building blocks */
 /* Block-Jacobi preconditioner of one local step (tt_linalg.f90:363-495,
@@ -385,6 +415,25 @@ int ttf_tt_amen_solve(int d, const int* n, const int* m, const int* ry, const in
                       const double* crY, const double* crX0, int* rx, double eps, int kickrank, int nswp,
                       int verb, char prec, int nrestart, int niters, int rmax, double* result,
                       int64_t result_cap, int64_t* result_size, ttf_amen_stats* stats);
+/* AMEn solve of A x = y with an auxiliary residual TT z, the typed solver
+ * dtt_amen_solve (ttamen.f90:29-252) on flat arrays (its type(dtt)
+ * arguments are not C-interoperable): A (ra_k, n_k, n_k, ra_k+1) cores in
+ * crA, y (ry_k, n_k, ry_k+1) in crY, the initial x in crX0 with ranks rx
+ * (in/out; the solution's ranks on return); defaults of the reference:
+ * kickrank 4, nswp 20, local_prec 'n', local_iters 2, local_restart 40,
+ * trunc_norm 1 (truncation by local residual; 0: Frobenius chop),
+ * max_full_size 50 (dense local solves below that local size, at most 128).
+ * z starts from N(0,1) cores drawn from `seed` (the reference uses its
+ * Fortran random number generator).  rmax <= 0 means a rank capacity of
+ * 256; ranks may reach rmax + kickrank between truncations, and beyond that
+ * the call fails.  The cores of x go to `result`.  Host pointers;
+ * device-resident run.  stats: learned_* stay zero. */
+int ttf_dtt_amen_solve(int d, const int* n, const int* ra, const double* crA, const int* ry, const double* crY,
+                       double tol, int* rx, const double* crX0, int kickrank, int nswp, char local_prec,
+                       int local_iters, int local_restart, int trunc_norm, int max_full_size, int verb, int rmax,
+                       unsigned seed, double* result, int64_t result_cap, int64_t* result_size,
+                       ttf_amen_stats* stats);
+
 
 /* ------------------------------------- learned block preconditioner (MLP) */
 /* No counterpart in the reference: a replacement for the block-Jacobi 'c'

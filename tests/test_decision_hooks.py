@@ -93,3 +93,25 @@ def test_residual_rank_rule_on_the_threshold(hooks):
     errs = np.concatenate([[-1.0], np.full(9, 1.0), np.full(10, 1e-3)])
     assert hooks.residual_rank_rule(errs, 1e-2, 1e-6) == 10 == K.residual_rank(lambda c: errs[c], 20, 1e-2, 1e-6)
     assert hooks.residual_rank_rule(np.array([-1.0]), 0.0, 0.0) == 1
+
+
+def test_frobenius_chop_matches_oracle(hooks):
+    """ttf_chop (svd.f90:86-111, the typed solver's Frobenius truncation)
+    against the oracle's chop, with and without a rank cap, on random
+    singular values and on values whose tails sit exactly on the bound (the
+    reference keeps a value whose addition reaches the bound: '<')."""
+    rng = np.random.default_rng(23)
+    for _ in range(3000):
+        n = int(rng.integers(1, 60))
+        s = np.sort(10.0 ** rng.uniform(-14, 0, n))[::-1]
+        tol = 10.0 ** rng.uniform(-12, -0.5)
+        rmax = int(rng.integers(1, n + 1)) if rng.random() < 0.3 else None
+        assert hooks.chop(s, tol, rmax) == K.chop(s, tol, rmax)[0], (s, tol, rmax)
+    # ||s||^2 = 4 and tol 0.5 (both exact): bound 1.0; the tail 1.0 is not < 1.0
+    s = np.array([1.0, 1.0, 1.0, 1.0])
+    assert hooks.chop(s, 0.5) == K.chop(s, 0.5)[0] == 4
+    assert hooks.chop(s, 0.51) == K.chop(s, 0.51)[0] == 3
+    # bound 1.5625 (tol 0.25, ||s||^2 = 25): tails 0, 1 dropped, 5 kept
+    s = np.array([4.0, 2.0, 2.0, 1.0, 0.0])
+    assert hooks.chop(s, 0.25) == K.chop(s, 0.25)[0] == 3
+    assert hooks.chop(s, None, 2) == K.chop(s, None, 2)[0] == 2

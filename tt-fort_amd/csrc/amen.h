@@ -31,6 +31,11 @@ void jac_gen(char type, int rx1, int n, int rx2, int ra1, int ra2, const double*
 void jac_apply(char type, int rx1, int n, int rx2, const double* jacs, const double* x, double* y,
                hipStream_t st);
 
+// In-place inverse of one n x n column-major matrix (n <= 128), Gauss-Jordan
+// with partial pivoting (the dense local solve of ttamen.f90:138-146); a
+// singular matrix is an error.
+void dense_inverse(int n, double* M, hipStream_t st);
+
 struct GmresStats {
   int it = 0;         // restart cycles completed (0-based index of the last)
   int j = 0;          // Krylov dimension of the last cycle

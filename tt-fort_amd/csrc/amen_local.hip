@@ -252,6 +252,24 @@ void jac_gen(char type, int rx1, int n, int rx2, int ra1, int ra2, const double*
     TTF_REQUIRE(hinfo[b] == 0, "djac_gen: block %lld is singular (zero pivot %d)", (long long)b, hinfo[b]);
 }
 
+void dense_inverse(int n, double* M, hipStream_t st) {
+  TTF_REQUIRE(n >= 1 && n <= kInvMaxN, "dense_inverse: order %d outside 1..%d", n, kInvMaxN);
+  int* info = (int*)arena(kArenaSmall, sizeof(int) * 4);
+  TTF_HIP(hipMemsetAsync(info, 0, sizeof(int), st));
+  static std::atomic<bool> attr{false};
+  if (!attr) {
+    TTF_HIP(hipFuncSetAttribute((const void*)batched_inverse_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(sizeof(double) * kInvMaxN * (kInvMaxN + 1))));
+    attr = true;
+  }
+  hipLaunchKernelGGL(batched_inverse_kernel, dim3(1), dim3(256), sizeof(double) * (size_t)n * (n + 1), st, n, M,
+                     info);
+  TTF_HIP(hipGetLastError());
+  int h = 0;
+  read_back(&h, info, sizeof(int), st);
+  TTF_REQUIRE(h == 0, "dense_inverse: singular matrix (zero pivot %d)", h);
+}
+
 void jac_apply(char type, int rx1, int n, int rx2, const double* jacs, const double* x, double* y,
                hipStream_t st) {
   type = norm_ptype(type);

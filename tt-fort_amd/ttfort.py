@@ -141,6 +141,7 @@ class AmenStats(C.Structure):
 
 
 _sig("ttf_my_chop3", _c_int, _c_int, _DP, C.c_double)
+_sig("ttf_chop", _c_int, _c_int, _DP, C.c_double, _c_int)
 _sig("ttf_residual_rank_rule", _c_int, _c_int, _DP, C.c_double, C.c_double)
 _sig("ttf_amen_truncation_rank", _c_int, _c_int, *([_c_int] * 5), _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_double,
      _IP, _DP)
@@ -152,8 +153,22 @@ _sig("ttf_dgmresr", _c_int, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, 
 _sig("ttf_duchol", _c_int, C.c_char, _c_int, _c_int, _vp, _c_int, _vp, _DP, _IP)
 _sig("ttf_dbfun3_left", _c_int, *_S8, _vp, _vp, _vp, _vp)
 _sig("ttf_dbfun3_right", _c_int, *_S8, _vp, _vp, _vp, _vp)
+_sig("ttf_dtt_amen_solve", _c_int, _c_int, _IP, _IP, _DP, _IP, _DP, C.c_double, _IP, _DP, _c_int, _c_int, C.c_char,
+     _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, C.c_uint, _DP, _c_i64, C.POINTER(_c_i64), C.POINTER(AmenStats))
 _sig("ttf_tt_amen_solve", _c_int, _c_int, _IP, _IP, _IP, _IP, _DP, _DP, _DP, _IP, C.c_double, _c_int, _c_int,
      _c_int, C.c_char, _c_int, _c_int, _c_int, _DP, _c_i64, C.POINTER(_c_i64), C.POINTER(AmenStats))
+
+
+class Mvk4Stats(C.Structure):
+    _fields_ = [("sweeps", C.c_int), ("last_err_max", C.c_double), ("converged", C.c_int), ("max_rank", C.c_int),
+                ("seconds", C.c_double), ("flops", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_sig("ttf_tt_mvk4", _c_int, _c_int, _IP, _IP, _IP, _IP, _DP, _DP, _DP, _IP, C.c_double, _c_int, _c_int, _c_int,
+     _c_int, _DP, _c_i64, C.POINTER(_c_i64), C.POINTER(Mvk4Stats))
 
 
 class KslStats(C.Structure):
@@ -460,6 +475,37 @@ def my_chop3(s, eps):
     return int(lib.ttf_my_chop3(s.size, _dp(s), eps))
 
 
+def chop(s, tol=None, rmax=None):
+    """The Frobenius chop (svd.f90:86-111) as the typed solver applies it."""
+    lib = load()
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    return int(lib.ttf_chop(s.size, _dp(s), -1.0 if tol is None else tol, 0 if rmax is None else rmax))
+
+
+def dtt_amen_solve(n, ra, crA, ry, crY, tol, rx, crX0, kickrank=4, nswp=20, local_prec="n", local_iters=2,
+                   local_restart=40, trunc_norm=1, max_full_size=50, verb=0, rmax=0, seed=0):
+    """Run ttf_dtt_amen_solve (the typed dtt_amen_solve on flat arrays).
+    Returns (result cores flat, rx out, stats dict)."""
+    lib = load()
+    d = len(n)
+    n_ = np.ascontiguousarray(n, dtype=np.int32)
+    ra_ = np.ascontiguousarray(ra, dtype=np.int32)
+    ry_ = np.ascontiguousarray(ry, dtype=np.int32)
+    rx_ = np.ascontiguousarray(rx, dtype=np.int32).copy()
+    crA_ = np.ascontiguousarray(crA, dtype=np.float64)
+    crY_ = np.ascontiguousarray(crY, dtype=np.float64)
+    crX_ = np.ascontiguousarray(crX0, dtype=np.float64)
+    R = (rmax if rmax > 0 else 256) + kickrank
+    cap = int(sum(R * int(v) * R for v in n) + 16)
+    res = np.zeros(cap)
+    size = C.c_int64()
+    st = AmenStats()
+    check(lib.ttf_dtt_amen_solve(d, _ip(n_), _ip(ra_), _dp(crA_), _ip(ry_), _dp(crY_), tol, _ip(rx_), _dp(crX_),
+                                 kickrank, nswp, local_prec.encode(), local_iters, local_restart, trunc_norm,
+                                 max_full_size, verb, rmax, seed, _dp(res), cap, C.byref(size), C.byref(st)), lib)
+    return res[:size.value].copy(), rx_, st.as_dict()
+
+
 def residual_rank_rule(errs, res_new, eps2):
     """The AMEn residual-rank scan on given candidate residuals errs[c],
     c = 1..k-1 (errs[0] unused; k = len(errs))."""
@@ -561,6 +607,30 @@ def tt_amen_solve(n, ra, crA, ry, crY, rx, crX0, eps, kickrank=4, nswp=20, verb=
                                 eps, kickrank, nswp, verb, prec.encode(), nrestart, niters, rmax, _dp(res), cap,
                                 C.byref(size), C.byref(st)), lib)
     return res[:size.value].copy(), rx_, st.as_dict()
+
+
+def tt_mvk4(n, m, rx, ra, crA, crX, ry, crY0, eps, rmax=0, kickrank=5, nswp=20, verb=0):
+    """Run ttf_tt_mvk4 with the reference's argument layout (host arrays).
+
+    Returns (result cores flat, ry out, stats dict)."""
+    lib = load()
+    d = len(n)
+    n_ = np.ascontiguousarray(n, dtype=np.int32)
+    m_ = np.ascontiguousarray(m, dtype=np.int32)
+    rx_ = np.ascontiguousarray(rx, dtype=np.int32)
+    ra_ = np.ascontiguousarray(ra, dtype=np.int32)
+    ry_ = np.ascontiguousarray(ry, dtype=np.int32).copy()
+    crA_ = np.ascontiguousarray(crA, dtype=np.float64)
+    crX_ = np.ascontiguousarray(crX, dtype=np.float64)
+    crY_ = np.ascontiguousarray(crY0, dtype=np.float64)
+    R = max(rmax, int(max(ry)))
+    cap = int(sum(R * int(v) * (R + kickrank) for v in n) + 16)
+    res = np.zeros(cap)
+    size = C.c_int64()
+    st = Mvk4Stats()
+    check(lib.ttf_tt_mvk4(d, _ip(n_), _ip(m_), _ip(rx_), _ip(ra_), _dp(crA_), _dp(crX_), _dp(crY_), _ip(ry_), eps,
+                          rmax, kickrank, nswp, verb, _dp(res), cap, C.byref(size), C.byref(st)), lib)
+    return res[:size.value].copy(), ry_, st.as_dict()
 
 
 def zexpm(H, t):
