@@ -18,10 +18,11 @@ Prints one JSON line.  Workloads:
           block of 8 vectors (the block eigensolver's width), inputs resident
           in HBM.
 
-The `roofline` object describes the dominant kernel (the MFMA GEMM of the
-slower large contraction stage at that local shape), timed with HIP events
-on the library stream over 50 launches; `traffic` is its HBM bytes per
-launch from the committed counter summary.  `cpu_baseline` times the
+The `roofline` object describes the dominant kernel, the FP64 local matvec
+at that local shape (one fused launch of the three contraction stages plus
+the zero fill of y), timed with HIP events on the library stream over 50
+calls; `traffic` is its HBM bytes per call from the committed counter
+summary.  `cpu_baseline` times the
 oracle's sweep (numpy + host BLAS) on a bounded sample of the same workload
 on this host's cores (eigb), or its local matvec (matvec).
 """
@@ -49,19 +50,20 @@ MI355X_HBM_GBS = 8000.0
 C2_LOCAL_SHAPE = (128, 2, 128, 128, 2, 128, 5, 5)
 C2_BLOCK = 8
 
-# HBM bytes per launch of the dominant GEMM, from two rocprofv3 --pmc passes
-# (FETCH_SIZE, WRITE_SIZE) scaled by the same counters of a known-size copy.
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "bfun3_r128_block8_hbm_traffic.md")
+# HBM bytes per call of the FP64 local matvec, from two rocprofv3 --pmc
+# passes (FETCH_SIZE, WRITE_SIZE) scaled by the same counters of a
+# known-size copy.
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "bfun3_fused_r128_block8_hbm_traffic.md")
 
 
-def measured_traffic(stage):
-    """HBM bytes per launch of bfun3 stage `stage` (1-based) at the C2 local
-    shape, block 8, as recorded in the committed counter summary; None when
-    the summary has no figure for it."""
+def measured_traffic():
+    """HBM bytes per call of the FP64 local matvec at the C2 local shape,
+    block 8, as recorded in the committed counter summary; None when the
+    summary has no figure."""
     try:
         with open(TRAFFIC_PROFILE) as f:
             for line in f:
-                if line.startswith("traffic_bytes_stage%d =" % stage):
+                if line.startswith("traffic_bytes ="):
                     return int(line.split("=")[1])
     except OSError:
         pass
@@ -168,38 +170,38 @@ def run_matvec(args):
 
 
 def dominant_kernel_roofline(lib, shape, nvec, iters=50):
-    """Per-stage HIP-event timings of the bfun3 contraction on the library
-    stream (each stage launched `iters` times back to back between two
-    events); the roofline reports the stage-1 MFMA GEMM."""
+    """HIP-event timing of the FP64 local matvec as the library runs it (at
+    this shape the fused one-launch bfun3 kernel and its zero fill of y),
+    `iters` calls back to back between two events on the library stream.
+    achieved = algorithmic flops of the three stages per call / mean call
+    time; traffic = HBM bytes per call from the committed counter summary
+    (against the algorithmic bytes phi1 + phi2 + A + block (x + y))."""
     import ctypes
     import ttfort
     rx1, m, rx2, ry1, n, ry2, ra1, ra2 = shape
-    N = rx1 * m * rx2
+    nx, ny = rx1 * m * rx2, ry1 * n * ry2
     rng = np.random.default_rng(3)
     d = dict(phi1=ttfort.DeviceArray.from_numpy(rng.standard_normal(ry1 * rx1 * ra1)),
              A=ttfort.DeviceArray.from_numpy(rng.standard_normal(ra1 * n * m * ra2)),
              phi2=ttfort.DeviceArray.from_numpy(rng.standard_normal(rx2 * ra2 * ry2)),
-             x=ttfort.DeviceArray.from_numpy(rng.standard_normal(N * nvec)),
-             y=ttfort.DeviceArray((N * nvec,)))
-    st = (ctypes.c_float * 3)()
-    ttfort.check(lib.ttf_dbfun3_profile(*shape, nvec, d["phi1"].ptr, d["A"].ptr, d["phi2"].ptr,
-                                        d["x"].ptr, N, d["y"].ptr, N, 5, st), lib)
-    ttfort.check(lib.ttf_dbfun3_profile(*shape, nvec, d["phi1"].ptr, d["A"].ptr, d["phi2"].ptr,
-                                        d["x"].ptr, N, d["y"].ptr, N, iters, st), lib)
-    stage_ms = [float(v) for v in st]
-    fl = bfun3_stage_flops(*shape)
-    # stage 1 (x * phi2) is one GEMM launch; stage 3 (phi1 * T2) does the same
-    # flops, split-K into a GEMM and a reduction launch, and stage 2 is the
-    # small panel product -- stage 1 is the kernel the roofline reports
-    dom = 0
-    achieved = fl[dom] * nvec / (stage_ms[dom] * 1e-3) / 1e12
-    traffic = measured_traffic(dom + 1) if (shape, nvec) == (C2_LOCAL_SHAPE, C2_BLOCK) else None
+             x=ttfort.DeviceArray.from_numpy(rng.standard_normal(nx * nvec)),
+             y=ttfort.DeviceArray((ny * nvec,)))
+    ms = ctypes.c_float()
+    fused = ctypes.c_int()
+    for it in (5, iters):
+        ttfort.check(lib.ttf_dbfun3_chain_profile(*shape, nvec, d["phi1"].ptr, d["A"].ptr, d["phi2"].ptr,
+                                                  d["x"].ptr, nx, d["y"].ptr, ny, it, ctypes.byref(ms),
+                                                  ctypes.byref(fused)), lib)
+    flops = sum(bfun3_stage_flops(*shape)) * nvec
+    achieved = flops / (ms.value * 1e-3) / 1e12
+    alg_bytes = 8 * (ry1 * rx1 * ra1 + rx2 * ra2 * ry2 + ra1 * n * m * ra2 + nvec * (nx + ny))
+    traffic = measured_traffic() if (shape, nvec) == (C2_LOCAL_SHAPE, C2_BLOCK) else None
+    name = ("bfun3_fused_kernel<3> + zero fill of y" if fused.value else "bfun3 GEMM chain (three stages)")
     return dict(bound="mfma", achieved=round(achieved, 3), peak=MI355X_PEAK_F64_TFLOPS,
                 unit="TFLOP/s", frac=round(achieved / MI355X_PEAK_F64_TFLOPS, 4), traffic=traffic,
-                kernel="gemm_mfma_kernel<double,...> of bfun3 stage 1 at (r,n,r,R)=(%d,%d,%d,%d), "
-                       "block %d" % (rx1, m, rx2, ra2, nvec),
-                stage_ms=[round(v, 5) for v in stage_ms],
-                stage_tflops=[round(fl[k] * nvec / (stage_ms[k] * 1e-3) / 1e12, 3) for k in range(3)])
+                kernel="%s: y = Phi1 A Phi2 x at (r,n,r,R)=(%d,%d,%d,%d), block %d, %.1f MFLOP per call"
+                       % (name, rx1, m, rx2, ra2, nvec, flops / 1e6),
+                us_per_call=round(ms.value * 1e3, 3), algorithmic_bytes=alg_bytes)
 
 
 def _eigb_leg(args, mixed):

@@ -135,6 +135,15 @@ int ttf_dbfun3_profile(int rx1, int m, int rx2, int ry1, int n, int ry2, int ra1
                        int nvec, const double* phi1, const double* A, const double* phi2,
                        const double* x, int64_t ldx, double* y, int64_t ldy, int iters,
                        float* stage_ms);
+/* Times the whole FP64 dbfun3 as the library runs it (the fused one-launch
+ * kernel and its zero fill where the shape fits, else the GEMM chain):
+ * `iters` back-to-back calls between two events on the library stream,
+ * after one call that settles any tuning; *ms receives the mean
+ * milliseconds per call and *fused (optional) 1 if the fused kernel ran.
+ * Synchronous; y is overwritten. */
+int ttf_dbfun3_chain_profile(int rx1, int m, int rx2, int ry1, int n, int ry2, int ra1, int ra2, int nvec,
+                             const double* phi1, const double* A, const double* phi2, const double* x,
+                             int64_t ldx, double* y, int64_t ldy, int iters, float* ms, int* fused);
 /* Same contraction in FP32 on the FP32 matrix cores (mixed-precision path). */
 int ttf_sbfun3(int rx1, int m, int rx2, int ry1, int n, int ry2, int ra1, int ra2,
                int nvec, const float* phi1, const float* A, const float* phi2,
@@ -251,6 +260,10 @@ typedef struct {
   long refreshes;             /* FP64 recomputations of A X in the local
                                  eigensolves (mixed precision) */
   long sweep_matvecs32[64];   /* fp32 local matvecs of each sweep */
+  long block_prec_solves;     /* local eigensolves preconditioned by the shifted
+                                 diagonal blocks of the local matrix */
+  long block_prec_fallbacks;  /* local eigensolves whose shifted blocks were not
+                                 positive definite (diagonal preconditioner) */
 } ttf_eigb_stats;
 
 /* Block eigensolver in TT format: the B lowest eigenpairs of the symmetric
@@ -343,6 +356,17 @@ int ttf_amen_truncation_rank(int dir, int rx1, int n, int rx2, int ra1, int ra2,
  * reference's scan from the end; rmax > 0 caps r first (its tail counted
  * as discarded), tol < 0 skips the tolerance.  Host arrays; needs no GPU. */
 int ttf_chop(int n, const double* s, double tol, int rmax);
+
+/* Shifted block preconditioner of the local eigensolver: the inverses of
+ * M_k - sigma I for the ry2 diagonal blocks M_k (order ry1 n <= 256) of the
+ * local matrix over the left interface and the mode, i.e. the 'l' blocks
+ * of djac_gen (tt_linalg.f90:434-462) shifted by sigma; no counterpart in
+ * the reference, whose PRIMME call preconditions nothing.  inv receives
+ * (ry1 n)^2 ry2 doubles (layout of ttf_djac_gen 'l'); *positive_definite
+ * is 0 (and inv undefined) when some shifted block is not positive
+ * definite.  Device pointers. */
+int ttf_eig_block_prec(int ry1, int n, int ry2, int ra1, int ra2, const double* phi1, const double* A,
+                       const double* phi2, double sigma, double* inv, int* positive_definite);
 
 /* ------------------------------------------------- AMEn building blocks */
 /* Block-Jacobi preconditioner of one local step (tt_linalg.f90:363-495,

@@ -20,6 +20,11 @@ SHAPES = [
     (40, 2, 40, 40, 2, 40, 5, 5),
     (64, 3, 65, 63, 3, 65, 4, 6),
     (128, 2, 128, 128, 2, 128, 5, 5),
+    # the fused kernel's edges: an odd b1 range split in two (the second half
+    # one row), c2 tiles cut short, three-column c2 tiles
+    (5, 2, 7, 3, 2, 9, 2, 3),
+    (96, 2, 96, 96, 2, 96, 5, 5),
+    (21, 3, 11, 22, 3, 13, 3, 2),
 ]
 
 
@@ -38,7 +43,7 @@ def _scale(shape, t):
 
 
 @pytest.mark.parametrize("shape", SHAPES)
-@pytest.mark.parametrize("nvec", [1, 5])
+@pytest.mark.parametrize("nvec", [1, 5, 8])
 def test_dbfun3(ttlib, shape, nvec):
     import ttfort
     rng = np.random.default_rng(10)
@@ -229,7 +234,8 @@ for nvec in (8, 4):
         worst = max(worst, float(np.abs(y[v * ny:(v + 1) * ny] - ref).max() / scale))
 print(json.dumps({"rel": worst}))
 '''
-    env = dict(os.environ, TTF_TUNE_FILE=str(tune))
+    # the GEMM chain, not the fused kernel, at this shape
+    env = dict(os.environ, TTF_TUNE_FILE=str(tune), TTF_BFUN3_FUSED="0")
     r = subprocess.run([sys.executable, "-c", code, root], env=env, capture_output=True, text=True,
                        timeout=600)
     assert r.returncode == 0, r.stderr[-3000:]
@@ -238,3 +244,52 @@ print(json.dumps({"rel": worst}))
     # the pinned stage-1 entries were used: no new stage-1 line was tuned and appended
     lines = [ln.strip() for ln in tune.read_text().splitlines() if ln.strip()]
     assert [ln for ln in lines if ln.startswith("0 256 640 128 ")] == pinned, lines
+
+
+@pytest.mark.parametrize("nvec", [1, 8, 16])
+def test_dbfun3_fused_matches_gemm_chain(ttlib, tmp_path, nvec):
+    """At the (128, 2, 128, R = 5) local shape the library runs the fused
+    one-launch kernel; the GEMM chain (TTF_BFUN3_FUSED=0, in a child
+    process) gives the same products to 1e-13 of the problem scale, and two
+    fused calls give identical bits (the two halves of the b1 range meet by
+    commutative additions onto zero)."""
+    import json
+    import os
+    import subprocess
+    import sys
+    import ttfort
+    shape = (128, 2, 128, 128, 2, 128, 5, 5)
+    rng = np.random.default_rng(30 + nvec)
+    t = _rand(rng, shape, nvec)
+    for k in ("phi1", "A", "phi2", "x"):
+        np.save(tmp_path / (k + ".npy"), t[k])
+    lib = ttfort.load()
+    d = {k: ttfort.DeviceArray.from_numpy(t[k]) for k in ("phi1", "A", "phi2", "x")}
+    nx = ny = 128 * 2 * 128
+    dy = ttfort.DeviceArray((ny * nvec,))
+    ms = ttfort.C.c_float()
+    fused = ttfort.C.c_int()
+    ttfort.check(lib.ttf_dbfun3_chain_profile(*shape, nvec, d["phi1"].ptr, d["A"].ptr, d["phi2"].ptr,
+                                              d["x"].ptr, nx, dy.ptr, ny, 3, ttfort.C.byref(ms),
+                                              ttfort.C.byref(fused)), lib)
+    assert fused.value == 1
+    y1 = dy.numpy().copy()
+    y2 = ttfort.bfun3(shape, t["phi1"], t["A"], t["phi2"], t["x"], nvec=nvec)
+    assert np.array_equal(y1, y2)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = r'''
+import os, sys
+root, d, nvec = sys.argv[1], sys.argv[2], int(sys.argv[3])
+sys.path[:0] = [root, os.path.join(root, "tt-fort_amd")]
+import numpy as np
+import ttfort
+t = {k: np.load(os.path.join(d, k + ".npy")) for k in ("phi1", "A", "phi2", "x")}
+y = ttfort.bfun3((128, 2, 128, 128, 2, 128, 5, 5), t["phi1"], t["A"], t["phi2"], t["x"], nvec=nvec)
+np.save(os.path.join(d, "y_chain.npy"), y)
+'''
+    env = dict(os.environ, TTF_BFUN3_FUSED="0")
+    r = subprocess.run([sys.executable, "-c", code, root, str(tmp_path), str(nvec)], env=env,
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-3000:]
+    yc = np.load(tmp_path / "y_chain.npy")
+    assert np.abs(y1 - yc).max() <= 1e-13 * _scale(shape, t)

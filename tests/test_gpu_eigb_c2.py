@@ -38,7 +38,11 @@ def test_eigb_heisenberg_d64_rmax128(ttlib):
     lam, res, ry_out, st = ttfort.tt_eigb(n, ra, crA, crY, ry, tol, rmax, B, nswp=nswp)
     print("lambda", lam.tolist())
     print("sweep ermax", st["sweep_ermax"], "lambda sums", st["sweep_lambda_sum"], "converged", st["converged"])
+    print("block-preconditioned solves", st["block_prec_solves"], "fallbacks", st["block_prec_fallbacks"])
     assert st["max_rank"] == rmax
+    # the middle local problems (order 256 blocks) take the shifted block
+    # preconditioner
+    assert st["block_prec_solves"] > 0
     assert st["nrec"] == st["sweeps"]
     assert st["last_ermax"] == st["sweep_ermax"][-1]
     assert st["converged"] == (1 if st["last_ermax"] < tol else 0)

@@ -32,7 +32,8 @@ CFLAGS = [
 # sources compiled once; refabi.hip is compiled per Fortran integer kind
 SHARED_SOURCES = ["runtime.hip", "gemm.hip", "contract.hip", "linalg.hip", "lobpcg.hip", "eigb.hip",
                   "amen_local.hip", "amen.hip", "ksl.hip", "mlp.hip",
-                  "eigb_batch.hip", "sdv_io.hip", "noalloc.hip", "mvk4.hip", "amen_typed.hip"]
+                  "eigb_batch.hip", "sdv_io.hip", "noalloc.hip", "mvk4.hip", "amen_typed.hip",
+                  "bfun3_fused.hip"]
 KIND_SOURCES = ["refabi.hip"]
 
 

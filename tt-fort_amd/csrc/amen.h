@@ -36,6 +36,13 @@ void jac_apply(char type, int rx1, int n, int rx2, const double* jacs, const dou
 // singular matrix is an error.
 void dense_inverse(int n, double* M, hipStream_t st);
 
+// In-place inverse of nblk symmetric positive definite nb x nb blocks
+// (column-major, consecutive; nb <= 256): Gauss-Jordan without pivoting for
+// nb <= 128, else a 2 x 2 split through the Schur complement of the leading
+// half.  Returns false (the blocks then hold no inverse) when a pivot is not
+// positive, i.e. when some block is not positive definite.
+bool spd_block_inverse(int nb, int64_t nblk, double* M, hipStream_t st);
+
 struct GmresStats {
   int it = 0;         // restart cycles completed (0-based index of the last)
   int j = 0;          // Krylov dimension of the last cycle

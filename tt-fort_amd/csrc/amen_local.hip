@@ -100,22 +100,23 @@ __global__ void jac_blocks_kernel(char type, int rx1, int n, int rx2, int ra1, i
   }
 }
 
-// In-place inverse of `count` n x n column-major matrices (n <= 128) by
-// Gauss-Jordan elimination with partial (row) pivoting in LDS, one
-// work-group per matrix; the row interchanges are undone as column
-// interchanges at the end.  A zero pivot leaves the block's info flag set
-// and the block's entries non-finite.
+// In-place inverse of `count` n x n column-major matrices (n <= 128; matrix
+// b at M + b stride, leading dimension ldm) by Gauss-Jordan elimination with
+// partial (row) pivoting in LDS, one work-group per matrix; the row
+// interchanges are undone as column interchanges at the end.  A zero pivot
+// leaves the block's info flag set (the first such step) and the block's
+// entries non-finite.
 constexpr int kInvMaxN = 128;
 
-__global__ __launch_bounds__(256) void batched_inverse_kernel(int n, double* __restrict__ M,
-                                                              int* __restrict__ info) {
+__global__ __launch_bounds__(256) void batched_inverse_kernel(int n, double* __restrict__ M, int64_t ldm,
+                                                              int64_t stride, int* __restrict__ info) {
   extern __shared__ double a[];   // column-major, ld = n + 1
   __shared__ int piv[kInvMaxN];
   __shared__ double rmax[256];
   __shared__ int rarg[256];
   const int ld = n + 1, tid = threadIdx.x, nt = blockDim.x;
-  double* g = M + (int64_t)blockIdx.x * n * n;
-  for (int e = tid; e < n * n; e += nt) a[(e % n) + ld * (e / n)] = g[e];
+  double* g = M + (int64_t)blockIdx.x * stride;
+  for (int e = tid; e < n * n; e += nt) a[(e % n) + ld * (e / n)] = g[(e % n) + ldm * (e / n)];
   __syncthreads();
   for (int k = 0; k < n; ++k) {
     // pivot row: largest |a(i, k)|, i >= k (first index among ties)
@@ -146,7 +147,7 @@ __global__ __launch_bounds__(256) void batched_inverse_kernel(int n, double* __r
       }
     __syncthreads();
     const double pv = a[k + ld * k];
-    if (pv == 0.0 && tid == 0) info[blockIdx.x] = k + 1;
+    if (tid == 0 && pv == 0.0 && info[blockIdx.x] == 0) info[blockIdx.x] = k + 1;
     const double inv = 1.0 / pv;
     __syncthreads();
     // row k scaled, with its pivot position becoming 1/pivot
@@ -175,7 +176,81 @@ __global__ __launch_bounds__(256) void batched_inverse_kernel(int n, double* __r
       }
     __syncthreads();
   }
-  for (int e = tid; e < n * n; e += nt) g[e] = a[(e % n) + ld * (e / n)];
+  for (int e = tid; e < n * n; e += nt) g[(e % n) + ldm * (e / n)] = a[(e % n) + ld * (e / n)];
+}
+
+// In-place inverse of symmetric positive definite n x n blocks (n <= 128;
+// block b at M + b stride, leading dimension ldm) by Gauss-Jordan
+// elimination without pivoting, one work-group of 256 per block with the
+// block in registers: thread t owns the 8 x 8 tile at rows 8 (t % 16) and
+// columns 8 (t / 16) of the block padded to order 128 with the identity.
+// Step k broadcasts column k and row k through LDS, double-buffered so that
+// each step has one barrier.  The pivots are those of the LDL^T
+// factorisation, all positive exactly when the block is positive definite;
+// a pivot that is not positive sets info[b] (the block then holds no
+// inverse).
+__global__ __launch_bounds__(256) void spd_inverse_tiled_kernel(int n, double* __restrict__ M, int64_t ldm,
+                                                                int64_t stride, int* __restrict__ info) {
+  __shared__ double colb[2][128];
+  __shared__ double rowb[2][128];
+  const int t = threadIdx.x, ti = t & 15, tj = t >> 4;
+  double* g = M + (int64_t)blockIdx.x * stride;
+  double a[8][8];
+#pragma unroll
+  for (int s = 0; s < 8; ++s)
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int i = 8 * ti + r, j = 8 * tj + s;
+      a[r][s] = (i < n && j < n) ? g[i + ldm * j] : (i == j ? 1.0 : 0.0);
+    }
+  bool bad = false;
+  const int nkb = (n + 7) / 8;
+  for (int kb = 0; kb < nkb; ++kb) {
+    const bool rowk = ti == kb, colk = tj == kb;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int k = 8 * kb + c;
+      if (k < n) {   // uniform
+        const int buf = k & 1;
+        if (colk) {
+#pragma unroll
+          for (int r = 0; r < 8; ++r) colb[buf][8 * ti + r] = a[r][c];
+        }
+        if (rowk) {
+#pragma unroll
+          for (int s = 0; s < 8; ++s) rowb[buf][8 * tj + s] = a[c][s];
+        }
+        __syncthreads();
+        const double pv = rowb[buf][k];
+        bad = bad || !(pv > 0.0);
+        const double inv = 1.0 / pv;
+        double f[8], h[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) f[r] = colb[buf][8 * ti + r];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) h[s] = rowb[buf][8 * tj + s] * inv;
+        // a(i, j) - a(i, k) a(k, j) / pv off row and column k; row k scaled
+        // by 1 / pv; column k -a(i, k) / pv; the pivot 1 / pv
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+#pragma unroll
+          for (int s = 0; s < 8; ++s) {
+            double v = a[r][s] - f[r] * h[s];
+            if (s == c && colk) v = -f[r] * inv;
+            if (r == c && rowk) v = (s == c && colk) ? inv : h[s];
+            a[r][s] = v;
+          }
+      }
+    }
+  }
+  if (t == 0 && bad) info[blockIdx.x] = 1;
+#pragma unroll
+  for (int s = 0; s < 8; ++s)
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int i = 8 * ti + r, j = 8 * tj + s;
+      if (i < n && j < n) g[i + ldm * j] = a[r][s];
+    }
 }
 
 // y(blk) = J(blk) x(blk) for all blocks: block b has size bs, matrix at
@@ -244,7 +319,8 @@ void jac_gen(char type, int rx1, int n, int rx2, int ra1, int ra2, const double*
                                 (int)(sizeof(double) * kInvMaxN * (kInvMaxN + 1))));
     attr = true;
   }
-  hipLaunchKernelGGL(batched_inverse_kernel, dim3((unsigned)nblk), dim3(256), lds, st, bs, jacs, info);
+  hipLaunchKernelGGL(batched_inverse_kernel, dim3((unsigned)nblk), dim3(256), lds, st, bs, jacs, (int64_t)bs,
+                     (int64_t)bs * bs, info);
   TTF_HIP(hipGetLastError());
   std::vector<int> hinfo(nblk);
   read_back(hinfo.data(), info, sizeof(int) * (size_t)nblk, st);
@@ -263,11 +339,74 @@ void dense_inverse(int n, double* M, hipStream_t st) {
     attr = true;
   }
   hipLaunchKernelGGL(batched_inverse_kernel, dim3(1), dim3(256), sizeof(double) * (size_t)n * (n + 1), st, n, M,
-                     info);
+                     (int64_t)n, (int64_t)n * n, info);
   TTF_HIP(hipGetLastError());
   int h = 0;
   read_back(&h, info, sizeof(int), st);
   TTF_REQUIRE(h == 0, "dense_inverse: singular matrix (zero pivot %d)", h);
+}
+
+namespace {
+// A21 = A12^T in each nb x nb block (h1 leading rows and columns).
+__global__ void mirror_upper_kernel(int nb, int h1, int64_t nblk, double* __restrict__ M) {
+  const int h2 = nb - h1;
+  const int64_t per = (int64_t)h2 * h1, total = per * nblk;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = e / per, r = e % per;
+    const int i = h1 + (int)(r % h2), j = (int)(r / h2);
+    double* Mb = M + b * (int64_t)nb * nb;
+    Mb[i + (int64_t)nb * j] = Mb[j + (int64_t)nb * i];
+  }
+}
+}  // namespace
+
+bool spd_block_inverse(int nb, int64_t nblk, double* M, hipStream_t st) {
+  TTF_REQUIRE(nb >= 1 && nb <= 2 * kInvMaxN, "spd_block_inverse: order %d outside 1..%d", nb, 2 * kInvMaxN);
+  TTF_REQUIRE(nblk >= 1 && nblk <= 0x7fffffff, "spd_block_inverse: %lld blocks", (long long)nblk);
+  const int h1 = nb <= kInvMaxN ? nb : nb / 2, h2 = nb - h1;
+  const int64_t nb2 = (int64_t)nb * nb, hh = (int64_t)h1 * h2;
+  char* ws = (char*)arena(kArenaEigPrec2, sizeof(double) * (size_t)(hh * nblk) + sizeof(int) * (size_t)nblk);
+  double* T = (double*)ws;
+  int* info = (int*)(ws + sizeof(double) * (size_t)(hh * nblk));
+  TTF_HIP(hipMemsetAsync(info, 0, sizeof(int) * (size_t)nblk, st));
+  auto inv = [&](int n, double* base) {
+    hipLaunchKernelGGL(spd_inverse_tiled_kernel, dim3((unsigned)nblk), dim3(256), 0, st, n, base, (int64_t)nb, nb2,
+                       info);
+    TTF_HIP(hipGetLastError());
+  };
+  auto batched = [&](GemmOp<double> g, int64_t sa, int64_t sb, int64_t sc) {
+    g.nb0 = (int)nblk;
+    g.nbatch = (int)nblk;
+    g.sAb0 = sa;
+    g.sBb0 = sb;
+    g.sCb0 = sc;
+    gemm_launch(g, st);
+  };
+  if (h2 == 0) {
+    inv(nb, M);
+  } else {
+    // [A11 A12; A21 A22] with A21 = A12^T: S = A22 - A21 A11^-1 A12 is
+    // positive definite exactly when the block is (given A11 is), and
+    // A^-1 = [A11^-1 + T S^-1 T^T, -T S^-1; (-T S^-1)^T, S^-1], T = A11^-1 A12
+    double* A11 = M;
+    double* A12 = M + (int64_t)nb * h1;
+    double* A21 = M + h1;
+    double* A22 = M + h1 + (int64_t)nb * h1;
+    inv(h1, A11);
+    batched(colmajor<double>('N', 'N', h1, h2, h1, 1.0, A11, nb, A12, nb, 0.0, T, h1), nb2, nb2, hh);
+    batched(colmajor<double>('N', 'N', h2, h2, h1, -1.0, A21, nb, T, h1, 1.0, A22, nb), nb2, hh, nb2);
+    inv(h2, A22);
+    batched(colmajor<double>('N', 'N', h1, h2, h2, -1.0, T, h1, A22, nb, 0.0, A12, nb), hh, nb2, nb2);
+    batched(colmajor<double>('N', 'T', h1, h1, h2, -1.0, A12, nb, T, h1, 1.0, A11, nb), nb2, hh, nb2);
+    hipLaunchKernelGGL(mirror_upper_kernel, dim3(grid_for(hh * nblk)), dim3(256), 0, st, nb, h1, nblk, M);
+    TTF_HIP(hipGetLastError());
+  }
+  std::vector<int> hinfo(nblk);
+  read_back(hinfo.data(), info, sizeof(int) * (size_t)nblk, st);
+  for (int64_t b = 0; b < nblk; ++b)
+    if (hinfo[b] != 0) return false;
+  return true;
 }
 
 void jac_apply(char type, int rx1, int n, int rx2, const double* jacs, const double* x, double* y,

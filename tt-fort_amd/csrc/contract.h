@@ -18,6 +18,15 @@ inline double bfun3_flops(const Shape& s) {
          2.0 * s.ry1 * s.n * s.ry2 * s.rx1 * s.ra1;
 }
 
+// The whole FP64 chain in one launch (bfun3_fused.hip) when the shape fits
+// its tiles (n <= 16, ry1 <= 128, ra1 n <= 16, m ra2 <= 64,
+// ra2 (16 / n) <= 64, rx1 m <= 256, LDS within 160 KB) and TTF_BFUN3_FUSED
+// is not 0; false leaves y untouched and the caller runs the GEMM chain.
+// x and y must not overlap.
+bool bfun3_fused(const Shape& s, int nvec, const double* phi1, const double* A, const double* phi2,
+                 const double* x, int64_t ldx, double* y, int64_t ldy, hipStream_t st);
+bool bfun3_fused_fits(const Shape& s, int nvec);
+
 template <typename T>
 void bfun3_chain(const Shape& s, int nvec, const T* phi1, const T* A, const T* phi2, const T* x,
                  int64_t ldx, T* y, int64_t ldy, hipStream_t st);

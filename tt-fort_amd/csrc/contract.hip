@@ -9,6 +9,8 @@
 //   A (a1,i,j,a2) = (ra1,n,m,ra2)  phi1 (c1,b1,a1)  phi2 (b2,a2,c2)
 #include "contract.h"
 
+#include <type_traits>
+
 namespace ttf {
 
 // ------------------------------------------------------------------ bfun3
@@ -61,10 +63,21 @@ void bfun3_stage12(const Shape& s, int nvec, const T* A, const T* phi2, const T*
   gemm_launch(bfun3_op2(s, nvec, A, (const T*)T1, T2), st);
 }
 
+// true when the column blocks [x, x + ldx nvec) and [y, y + ldy nvec) overlap
+template <typename T>
+static bool overlaps(const T* x, int64_t ldx, const T* y, int64_t ldy, int nvec) {
+  const T* xe = x + ldx * nvec;
+  const T* ye = y + ldy * nvec;
+  return x < ye && y < xe;
+}
+
 template <typename T>
 void bfun3_chain(const Shape& s, int nvec, const T* phi1, const T* A, const T* phi2, const T* x,
                  int64_t ldx, T* y, int64_t ldy, hipStream_t st) {
   if (nvec <= 0) return;
+  if constexpr (std::is_same<T, double>::value) {
+    if (!overlaps(x, ldx, y, ldy, nvec) && bfun3_fused(s, nvec, phi1, A, phi2, x, ldx, y, ldy, st)) return;
+  }
   const int64_t sz1 = (int64_t)s.rx1 * s.m * s.ra2 * s.ry2;
   const int64_t sz2 = (int64_t)s.rx1 * s.ra1 * s.n * s.ry2;
   T* T1 = (T*)arena(kArenaBfun, sizeof(T) * (size_t)(sz1 + sz2) * nvec);
@@ -89,6 +102,10 @@ template <typename T>
 void bfun3_chain_ws(const Shape& s, int nvec, const T* phi1, const T* A, const T* phi2, const T* x,
                     int64_t ldx, T* y, int64_t ldy, T* ws, size_t ws_cap, hipStream_t st) {
   if (nvec <= 0) return;
+  // the fused kernel keeps its intermediates in LDS and allocates nothing
+  if constexpr (std::is_same<T, double>::value) {
+    if (!overlaps(x, ldx, y, ldy, nvec) && bfun3_fused(s, nvec, phi1, A, phi2, x, ldx, y, ldy, st)) return;
+  }
   const size_t need = bfun3_workspace(s, nvec);
   TTF_REQUIRE(ws != nullptr && need <= ws_cap, "bfun3: workspace of %zu elements exceeds the %zu given", need,
               ws_cap);
@@ -546,6 +563,30 @@ int ttf_dbfun3_profile(int rx1, int m, int rx2, int ry1, int n, int ry2, int ra1
       TTF_HIP(hipEventElapsedTime(&t, e0, e1));
       stage_ms[k] = t / (float)iters;
     }
+    TTF_HIP(hipEventDestroy(e0));
+    TTF_HIP(hipEventDestroy(e1));
+  });
+}
+
+int ttf_dbfun3_chain_profile(int rx1, int m, int rx2, int ry1, int n, int ry2, int ra1, int ra2, int nvec,
+                             const double* phi1, const double* A, const double* phi2, const double* x,
+                             int64_t ldx, double* y, int64_t ldy, int iters, float* ms, int* fused) {
+  return guarded([&] {
+    Shape s = mkshape(rx1, m, rx2, ry1, n, ry2, ra1, ra2);
+    TTF_REQUIRE(iters > 0 && nvec > 0 && ms != nullptr, "ttf_dbfun3_chain_profile: bad arguments");
+    hipStream_t st = stream();
+    if (fused) *fused = bfun3_fused_fits(s, nvec) && !overlaps(x, ldx, (const double*)y, ldy, nvec) ? 1 : 0;
+    bfun3_chain<double>(s, nvec, phi1, A, phi2, x, ldx, y, ldy, st);   // settles any tuning
+    hipEvent_t e0, e1;
+    TTF_HIP(hipEventCreate(&e0));
+    TTF_HIP(hipEventCreate(&e1));
+    TTF_HIP(hipEventRecord(e0, st));
+    for (int it = 0; it < iters; ++it) bfun3_chain<double>(s, nvec, phi1, A, phi2, x, ldx, y, ldy, st);
+    TTF_HIP(hipEventRecord(e1, st));
+    TTF_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    TTF_HIP(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / (float)iters;
     TTF_HIP(hipEventDestroy(e0));
     TTF_HIP(hipEventDestroy(e1));
   });

@@ -158,7 +158,7 @@ extern "C" int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, con
     // ---- sweeps
     int k = d - 1, dir = -1, swp = 1;
     double ermax = 0.0, fvold = 0.0;
-    long total_mv = 0, total_mv32 = 0, total_refresh = 0;
+    long total_mv = 0, total_mv32 = 0, total_refresh = 0, block_solves = 0, block_fallbacks = 0;
     std::vector<double> lam(B, 0.0), hs;
     int last_swp = 0;
     std::vector<double> rec_sec, rec_flops, rec_er, rec_fv;
@@ -187,6 +187,8 @@ extern "C" int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, con
       total_mv32 += es.matvecs32;
       sweep_mv32 += es.matvecs32;
       total_refresh += es.refreshes;
+      block_solves += es.block_prec > 0 ? 1 : 0;
+      block_fallbacks += es.block_prec < 0 ? 1 : 0;
       t_solve += ts.ms();
       double fv = 0.0;
       for (int b = 0; b < B; ++b) fv += lam[b];
@@ -333,6 +335,8 @@ extern "C" int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, con
       stats->converged = converged ? 1 : 0;
       stats->total_matvecs32 = total_mv32;
       stats->refreshes = total_refresh;
+      stats->block_prec_solves = block_solves;
+      stats->block_prec_fallbacks = block_fallbacks;
       int mx = 0;
       for (int q = 0; q <= d; ++q) mx = std::max(mx, ry[q]);
       stats->max_rank = mx;

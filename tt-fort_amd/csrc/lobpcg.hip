@@ -17,6 +17,7 @@
 // space, orthogonal to the new Ritz vectors; then [X P] <- S [C Z].
 // Dropped (zero) columns carry a large diagonal penalty in the projected
 // matrix so they can never produce a spurious Ritz value.
+#include "amen.h"
 #include "solvers.h"
 #include "jacobi.h"
 #include "wave.h"
@@ -397,6 +398,32 @@ __global__ void random_cols(int N, int c0, int c1, double* X, int64_t ld, unsign
   }
 }
 
+// Diagonal blocks of the local matrix over the left interface and the mode
+// (the 'l' blocks of tt_linalg.f90:434-462), shifted: block b2 < r2 has
+// order nb = r1 n and entries
+//   M((c1, i), (b1, j)) = sum_a1 Phi1(c1, b1, a1) sum_a2 A(a1, i, j, a2) Phi2(b2, a2, b2)
+// less sigma on the diagonal; element (row, col) of block b2 at
+// M[row + nb (col + nb b2)], row = c1 + r1 i, col = b1 + r1 j.
+__global__ void eig_lblocks_kernel(int r1, int n, int r2, int ra1, int ra2, const double* __restrict__ Phi1,
+                                   const double* __restrict__ A, const double* __restrict__ Phi2, double sigma,
+                                   double* __restrict__ M) {
+  const int64_t nb = (int64_t)r1 * n, total = nb * nb * r2;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t row = e % nb, col = (e / nb) % nb, b2 = e / (nb * nb);
+    const int c1 = (int)(row % r1), i = (int)(row / r1), b1 = (int)(col % r1), j = (int)(col / r1);
+    double s = 0.0;
+    for (int a1 = 0; a1 < ra1; ++a1) {
+      double t = 0.0;
+      for (int a2 = 0; a2 < ra2; ++a2)
+        t += A[a1 + (int64_t)ra1 * (i + (int64_t)n * (j + (int64_t)n * a2))] *
+             Phi2[b2 + (int64_t)r2 * (a2 + (int64_t)ra2 * b2)];
+      s += Phi1[c1 + (int64_t)r1 * (b1 + (int64_t)r1 * a1)] * t;
+    }
+    M[e] = row == col ? s - sigma : s;
+  }
+}
+
 int grid_for(int64_t n) { return (int)std::min<int64_t>(std::max<int64_t>(ceil_div(n, 256), 1), 8192); }
 
 void copy_cols(const double* src, int64_t lds, double* dst, int64_t ldd, int rows, int ncols,
@@ -426,6 +453,37 @@ static double eig_prec_delta() {
   }();
   return v;
 }
+
+// Inverses of the shifted 'l' blocks of op into inv ((ry1 n)^2 ry2 doubles);
+// false when some shifted block is not positive definite.
+static bool shifted_lblock_inverse(const LocalOp& op, double sigma, double* inv, hipStream_t st) {
+  const int64_t nb = (int64_t)op.s.ry1 * op.s.n;
+  hipLaunchKernelGGL(eig_lblocks_kernel, dim3(grid_for(nb * nb * op.s.ry2)), dim3(256), 0, st, op.s.ry1, op.s.n,
+                     op.s.ry2, op.s.ra1, op.s.ra2, op.phi1, op.A, op.phi2, sigma, inv);
+  TTF_HIP(hipGetLastError());
+  return spd_block_inverse((int)nb, op.s.ry2, inv, st);
+}
+
+// The shifted block preconditioner of the LOBPCG residuals: W = (M_k -
+// sigma)^-1 r on the rows of each right interface index k, M_k the 'l'
+// diagonal blocks (order ry1 n <= 256), sigma = theta_0 - delta max|theta|
+// below the start block's lowest Ritz value theta_0 (delta = 0.01;
+// TTF_EIG_BLOCK_SHIFT overrides, TTF_EIG_BLOCK=0 turns it off).  When
+// sigma lies below the local matrix's lowest eigenvalue, every M_k - sigma
+// is positive definite (Cauchy interlacing); when some block is not, the
+// blocks are formed again with 5 delta, and after that the solve falls back
+// to the diagonal preconditioner.
+static double eig_block_shift() {
+  static const double v = [] {
+    const char* off = std::getenv("TTF_EIG_BLOCK");
+    if (off && std::atoi(off) == 0) return 0.0;
+    const char* e = std::getenv("TTF_EIG_BLOCK_SHIFT");
+    return e ? std::max(0.0, std::atof(e)) : 0.01;
+  }();
+  return v;
+}
+constexpr int kBlockPrecMaxOrder = 256;
+constexpr size_t kBlockPrecMaxBytes = size_t(1) << 30;
 
 EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* lambda, double tol,
                    int max_matvecs, int dense_max, hipStream_t st, const MixedOp* mix) {
@@ -513,6 +571,27 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
     mul_small(N, bs, bs, AX, ld, CZ, bs, Xn + colsz * 3 * bs, ld, 0.0, st);
     TTF_HIP(hipMemsetAsync(maskPb[1], 0, sizeof(int) * kMaxM, st));
   }
+  // the shifted block preconditioner, from the start block's Ritz values
+  const double* Binv = nullptr;
+  const int pnb = op.s.ry1 * op.s.n, pnblk = op.s.ry2;
+  {
+    const double delta = eig_block_shift();
+    const size_t pbytes = sizeof(double) * (size_t)pnb * pnb * pnblk;
+    if (delta > 0.0 && pnb >= 2 && pnb <= kBlockPrecMaxOrder && pbytes <= kBlockPrecMaxBytes) {
+      std::vector<double> h0(2 * kMaxM + 1);
+      read_back(h0.data(), theta, sizeof(double) * (2 * kMaxM + 1), st);
+      const double scale = std::max(std::fabs(h0[0]), h0[2 * kMaxM]);
+      double* Mb = (double*)arena(kArenaEigPrec, pbytes);
+      // the shift delta, then 5 delta when a block was not positive definite
+      stats.block_prec = -1;
+      for (double dl : {delta, 5.0 * delta})
+        if (shifted_lblock_inverse(op, h0[0] - dl * scale, Mb, st)) {
+          Binv = Mb;
+          stats.block_prec = 1;
+          break;
+        }
+    }
+  }
   int matvecs = bs;
   int cur = 1;
   double anorm = 0.0;
@@ -563,7 +642,7 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
   // the optional diagonal preconditioner (TTF_EIG_PREC = clamp delta > 0)
   const double prec_delta = eig_prec_delta();
   double* Dg = nullptr;
-  if (prec_delta > 0.0) {
+  if (prec_delta > 0.0 && !Binv) {
     Dg = (double*)arena(kArenaSolver2, sizeof(double) * (size_t)N);
     localdiag_launch(op.s.ry1, op.s.n, op.s.ry2, op.s.ra1, op.s.ra2, op.phi1, op.A, op.phi2, Dg, st);
   }
@@ -575,8 +654,19 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
     double* AS = S + colsz * 3 * bs;
     double* AX = AS, *AW = AS + colsz * 2 * bs;
     double* res_part = (double*)arena(kArenaGram, res_bytes);
-    hipLaunchKernelGGL(residual_kernel, dim3(res_grid), dim3(256), 0, st, N, bs, X, AX, ld, theta, Wr,
+    // the preconditioned residual goes to Wr; with the block preconditioner
+    // the plain residual is staged in the W slot, free until the SVQB
+    hipLaunchKernelGGL(residual_kernel, dim3(res_grid), dim3(256), 0, st, N, bs, X, AX, ld, theta, Binv ? W : Wr,
                        res_part, Dg, prec_delta);
+    if (Binv) {
+      GemmOp<double> g = colmajor<double>('N', 'N', pnb, bs, pnb, 1.0, Binv, pnb, W, ld, 0.0, Wr, ld);
+      g.nb0 = pnblk;
+      g.nbatch = pnblk;
+      g.sAb0 = (int64_t)pnb * pnb;
+      g.sBb0 = pnb;
+      g.sCb0 = pnb;
+      gemm_launch(g, st);
+    }
     hipLaunchKernelGGL(residual_norms, dim3((unsigned)ceil_div(16 * bs, 256)), dim3(256), 0, st, bs, res_grid,
                        res_part, nrm);
     TTF_HIP(hipGetLastError());
@@ -652,3 +742,18 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
 }
 
 }  // namespace ttf
+
+// ---------------------------------------------------------------- C ABI (test hook)
+extern "C" int ttf_eig_block_prec(int ry1, int n, int ry2, int ra1, int ra2, const double* phi1, const double* A,
+                                  const double* phi2, double sigma, double* inv, int* positive_definite) {
+  using namespace ttf;
+  return guarded([&] {
+    TTF_REQUIRE(ry1 >= 1 && n >= 1 && ry2 >= 1 && ra1 >= 1 && ra2 >= 1 && phi1 && A && phi2 && inv &&
+                    positive_definite,
+                "ttf_eig_block_prec: invalid arguments");
+    TTF_REQUIRE((int64_t)ry1 * n <= kBlockPrecMaxOrder, "ttf_eig_block_prec: block order %lld exceeds %d",
+                (long long)ry1 * n, kBlockPrecMaxOrder);
+    LocalOp op{Shape{ry1, n, ry2, ry1, n, ry2, ra1, ra2}, phi1, A, phi2};
+    *positive_definite = shifted_lblock_inverse(op, sigma, inv, stream()) ? 1 : 0;
+  });
+}

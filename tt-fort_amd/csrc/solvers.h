@@ -46,6 +46,8 @@ struct EigStats {
   double res = 0.0;      // 2-norm of the residual norms of the wanted pairs
   double anorm = 0.0;    // operator norm estimate (largest |Ritz value|)
   int dense = 0;         // 1 if the dense path was used
+  int block_prec = 0;    // 1 shifted block preconditioner, -1 tried and not
+                         // positive definite (diagonal used), 0 not tried
 };
 
 // Lowest `nev` eigenpairs of the symmetric local operator.  X (N x nev, ld N,

@@ -106,6 +106,8 @@ enum Arena {
   kArenaSvdLong2,    // long-side singular vectors: short-side vectors
   kArenaMixed,       // fp32 operator copy and vector staging of the mixed path
   kArenaTsqr,        // tall-skinny QR: row chunks and stacked R factors
+  kArenaEigPrec,     // shifted block preconditioner of the local eigensolver
+  kArenaEigPrec2,    // its Schur-complement workspace and pivot flags
   kArenaCount
 };
 void* arena(Arena a, size_t bytes);

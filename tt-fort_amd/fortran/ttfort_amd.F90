@@ -42,6 +42,7 @@ module ttfort_amd
     integer(c_int) :: converged
     integer(c_long) :: total_matvecs32, refreshes
     integer(c_long) :: sweep_matvecs32(64)
+    integer(c_long) :: block_prec_solves, block_prec_fallbacks
   end type ttf_eigb_stats
 
   type, bind(c) :: ttf_amen_stats

@@ -77,6 +77,8 @@ for _nm in ("ttf_dbfun3", "ttf_dbfun3_noalloc", "ttf_sbfun3", "ttf_zbfun3", "ttf
     _sig(_nm, _c_int, *(_S8 + [_c_int, _vp, _vp, _vp, _vp, _c_i64, _vp, _c_i64]))
 _sig("ttf_dbfun3_profile", _c_int, *(_S8 + [_c_int, _vp, _vp, _vp, _vp, _c_i64, _vp, _c_i64, _c_int,
                                            C.POINTER(C.c_float)]))
+_sig("ttf_dbfun3_chain_profile", _c_int, *(_S8 + [_c_int, _vp, _vp, _vp, _vp, _c_i64, _vp, _c_i64, _c_int,
+                                                 C.POINTER(C.c_float), C.POINTER(C.c_int)]))
 for _nm in ("ttf_dphi_left", "ttf_dphi_right", "ttf_zphi_left", "ttf_zphi_right"):
     _sig(_nm, _c_int, *(_S8 + [_vp, _vp, _vp, _vp, _vp]))
 for _nm in ("ttf_dphi2_left", "ttf_dphi2_right"):
@@ -114,7 +116,8 @@ class EigbStats(C.Structure):
                 ("sweep_seconds", C.c_double * 64), ("sweep_flops", C.c_double * 64),
                 ("sweep_matvecs", C.c_long * 64), ("sweep_ermax", C.c_double * 64),
                 ("sweep_lambda_sum", C.c_double * 64), ("converged", C.c_int),
-                ("total_matvecs32", C.c_long), ("refreshes", C.c_long), ("sweep_matvecs32", C.c_long * 64)]
+                ("total_matvecs32", C.c_long), ("refreshes", C.c_long), ("sweep_matvecs32", C.c_long * 64),
+                ("block_prec_solves", C.c_long), ("block_prec_fallbacks", C.c_long)]
 
     def as_dict(self):
         out = {}
@@ -145,6 +148,8 @@ _sig("ttf_chop", _c_int, _c_int, _DP, C.c_double, _c_int)
 _sig("ttf_residual_rank_rule", _c_int, _c_int, _DP, C.c_double, C.c_double)
 _sig("ttf_amen_truncation_rank", _c_int, _c_int, *([_c_int] * 5), _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_double,
      _IP, _DP)
+_sig("ttf_eig_block_prec", _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, C.c_double, _vp,
+     C.POINTER(C.c_int))
 _sig("ttf_djac_gen", _c_int, C.c_char, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp)
 _sig("ttf_djac_size", _c_i64, C.c_char, _c_int, _c_int, _c_int)
 _sig("ttf_djac_apply", _c_int, C.c_char, _c_int, _c_int, _c_int, _vp, _vp, _vp)
@@ -535,6 +540,19 @@ def djac_gen(ptype, rx1, n, rx2, ra1, ra2, phi1, A, phi2):
     dj = DeviceArray((max(sz, 1),))
     check(lib.ttf_djac_gen(ptype.encode(), rx1, n, rx2, ra1, ra2, d1.ptr, dA.ptr, d2.ptr, dj.ptr), lib)
     return dj.numpy()[:sz]
+
+
+def eig_block_prec(ry1, n, ry2, ra1, ra2, phi1, A, phi2, sigma):
+    """Inverses of the shifted 'l' blocks through ttf_eig_block_prec: returns
+    (inverses flattened as ttf_djac_gen 'l', positive_definite)."""
+    lib = load()
+    nb = ry1 * n
+    d1, dA, d2 = (DeviceArray.from_numpy(np.ascontiguousarray(v, dtype=np.float64)) for v in (phi1, A, phi2))
+    dj = DeviceArray((nb * nb * ry2,))
+    pd = C.c_int(0)
+    check(lib.ttf_eig_block_prec(ry1, n, ry2, ra1, ra2, d1.ptr, dA.ptr, d2.ptr, float(sigma), dj.ptr,
+                                 C.byref(pd)), lib)
+    return dj.numpy(), bool(pd.value)
 
 
 def djac_apply(ptype, rx1, n, rx2, jacs, x):
