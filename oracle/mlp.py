@@ -19,7 +19,9 @@ Conventions (shared with the device kernels):
     W3 (256 x 64); the last term is the point-Jacobi solution of the block
     (residual over diagonal, 0 where the diagonal is 0), so the layers learn
     the correction the block's off-diagonals make to it; weights rounded to
-    bfloat16 (round to nearest even), float32 arithmetic;
+    bfloat16 (round to nearest even), float32 arithmetic in training; at
+    inference the layer inputs are rounded to bfloat16 too (the bf16 matrix
+    cores, fp32 accumulation);
   * correction z_b = y_b * rn_b / s_b;
   * training target y*_b = (B_b^-1 r_b) * s_b / rn_b; loss
     L = sum (y - y*)^2 / (M * n) over a batch of M blocks; Adam.
@@ -130,17 +132,21 @@ def block_solve_targets(rx1, n, rx2, ra1, ra2, phi1, A, phi2, v):
     return T
 
 
-def forward(theta, X, round_weights=True, dtype=np.float32):
-    """Network output y (64 x M) and the cache for backward."""
+def forward(theta, X, round_weights=True, dtype=np.float32, round_acts=False):
+    """Network output y (64 x M) and the cache for backward.  With
+    round_acts, every layer's input is rounded to bfloat16 before its
+    product (the inference form on the bf16 matrix cores); the point-Jacobi
+    term always uses the unrounded features."""
     p = unflatten(np.asarray(theta, dtype=np.float64))
     W = {k: (bf16_round(p[k]) if round_weights else p[k]).astype(dtype) for k in ("W1", "W2", "W3")}
     b = {k: p[k].astype(dtype) for k in ("b1", "b2", "b3")}
     X = X.astype(dtype)
-    z1 = W["W1"].T @ X + b["b1"][:, None]
+    rnd = (lambda a: bf16_round(a).astype(dtype)) if round_acts else (lambda a: a)
+    z1 = W["W1"].T @ rnd(X) + b["b1"][:, None]
     h1 = np.maximum(z1, 0)
-    z2 = W["W2"].T @ h1 + b["b2"][:, None]
+    z2 = W["W2"].T @ rnd(h1) + b["b2"][:, None]
     h2 = np.maximum(z2, 0)
-    y = W["W3"].T @ h2 + b["b3"][:, None] + jacobi_term(X)
+    y = W["W3"].T @ rnd(h2) + b["b3"][:, None] + jacobi_term(X)
     return y, dict(X=X, z1=z1, h1=h1, z2=z2, h2=h2, W=W)
 
 

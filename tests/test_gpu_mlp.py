@@ -43,9 +43,21 @@ def test_params_round_trip_and_forward(ttlib):
     ttfort.mlp_set_params(theta)
     assert np.array_equal(ttfort.mlp_get_params(), theta)
     X = rng.standard_normal((256, 300)).astype(np.float32)
+    # inference runs on the bf16 matrix cores: bf16 weights and layer inputs,
+    # fp32 accumulation.  Against the oracle with the same roundings the
+    # differences are fp32 summation order plus the rare activation whose
+    # bf16 rounding flips on it (one bf16 step, 2^-8, in one input of one
+    # output): 2e-3 of the largest output bounds both
     Y = ttfort.mlp_forward(X)
-    Yr, _ = ML.forward(theta, X, round_weights=True, dtype=np.float64)
-    assert np.abs(Y - Yr).max() <= 1e-5 * np.abs(Yr).max()
+    Yr, _ = ML.forward(theta, X, round_weights=True, dtype=np.float64, round_acts=True)
+    assert np.abs(Y - Yr).max() <= 2e-3 * np.abs(Yr).max()
+    # and the bf16 activations move the output by a few bf16 steps against
+    # the fp32-activation network
+    Yf, _ = ML.forward(theta, X, round_weights=True, dtype=np.float64)
+    assert np.abs(Y - Yf).max() <= 3e-2 * np.abs(Yf).max()
+    # the edge tiles (M and N not multiples of 64) agree with the full batch
+    Y7 = ttfort.mlp_forward(np.asfortranarray(X[:, :7]))
+    assert np.array_equal(Y7, Y[:, :7])
 
 
 def test_loss_grad_match_oracle(ttlib):
@@ -113,8 +125,9 @@ def test_collected_pairs_and_training(ttlib):
     # and halves (at least) the loss of the untrained network, which is the
     # point-Jacobi solve of each block
     yo, _ = ML.forward(t["theta"], X, round_weights=True, dtype=np.float64)
-    yd = ttfort.mlp_forward(X)
-    assert np.abs(yd - yo).max() <= 1e-4 * np.abs(yo).max()
+    yb, _ = ML.forward(t["theta"], X, round_weights=True, dtype=np.float64, round_acts=True)
+    yd = ttfort.mlp_forward(X)   # bf16 inference
+    assert np.abs(yd - yb).max() <= 2e-3 * np.abs(yb).max()
     y0, _ = ML.forward(t["theta0"], X, round_weights=True, dtype=np.float64)
     assert ML.loss(yo, T) < 0.5 * ML.loss(y0, T), (ML.loss(yo, T), ML.loss(y0, T), t["hist"])
     assert t["hist"][-1] < t["hist"][0]

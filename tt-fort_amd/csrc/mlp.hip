@@ -190,6 +190,93 @@ __global__ __launch_bounds__(256) void layer_kernel(int M, int K, int N, const f
       }
 }
 
+// The inference form of layer_kernel on gfx950's bf16 matrix cores: the
+// same Y = act(W^T X + b) (+ the point-Jacobi term with SKIP), with the
+// activations X rounded to bf16 (round to nearest even) and the products
+// accumulated in fp32 by v_mfma_f32_16x16x32_bf16.  K = 256 for every layer
+// of the network.  Lane l feeds row (or column) l % 16 and the eight
+// consecutive k = 8 (l / 16) .. +7 of each 32-deep step to both operands,
+// so the A and B elements it supplies meet at the same k; fragments are
+// read straight from global memory (weights 16 bytes, activations 32 bytes
+// per lane), all eight k-steps' loads issued before the products.
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+constexpr int kBfK = 256;
+static_assert(kMlpIn == kBfK && kMlpHid == kBfK, "the bf16 layers assume K = 256");
+
+template <bool RELU, bool SKIP>
+__global__ __launch_bounds__(256) void layer_bf16_kernel(int M, int N, const float* __restrict__ X,
+                                                         const uint16_t* __restrict__ W,
+                                                         const float* __restrict__ bias, float* __restrict__ Y,
+                                                         const float* __restrict__ X0) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wn = wave & 1, wm = wave >> 1;
+  const int n0 = blockIdx.x * 64 + wn * 32, m0 = blockIdx.y * 64 + wm * 32;
+  const int g = lane >> 4, c = lane & 15;
+  f4_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
+  const uint16_t* wa[2];
+  const float* xb[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int n = min(n0 + 16 * i + c, N - 1);   // rows past N are computed and not stored
+    wa[i] = W + 8 * g + (int64_t)kBfK * n;
+  }
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int m = min(m0 + 16 * j + c, M - 1);
+    xb[j] = X + 8 * g + (int64_t)kBfK * m;
+  }
+  uint4 wr[kBfK / 32][2];
+  float4 xr[kBfK / 32][2][2];
+#pragma unroll
+  for (int s = 0; s < kBfK / 32; ++s) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) wr[s][i] = *reinterpret_cast<const uint4*>(wa[i] + 32 * s);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      xr[s][j][0] = *reinterpret_cast<const float4*>(xb[j] + 32 * s);
+      xr[s][j][1] = *reinterpret_cast<const float4*>(xb[j] + 32 * s + 4);
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < kBfK / 32; ++s) {
+    bf8_t a[2], b[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) a[i] = __builtin_bit_cast(bf8_t, wr[s][i]);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const float4 lo = xr[s][j][0], hi = xr[s][j][1];
+      b[j] = bf8_t{(__bf16)lo.x, (__bf16)lo.y, (__bf16)lo.z, (__bf16)lo.w,
+                   (__bf16)hi.x, (__bf16)hi.y, (__bf16)hi.z, (__bf16)hi.w};
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gn = n0 + i * 16 + mfma_row<float>(lane, r);
+        const int gm = m0 + j * 16 + c;
+        if (gn < N && gm < M) {
+          float val = acc[i][j][r] + bias[gn];
+          if (RELU) val = fmaxf(val, 0.f);
+          if (SKIP) {
+            const float dg = X0[gn + (int64_t)kMlpIn * gm];
+            if (dg != 0.f) val += X0[3 * kMlpBlock + gn + (int64_t)kMlpIn * gm] / dg;
+          }
+          Y[gn + (int64_t)N * gm] = val;
+        }
+      }
+}
+
 // z(b1, i, b2) = Y(i, b) * scale(b), one wave per block
 __global__ __launch_bounds__(256) void scatter_kernel(int rx1, int n, int rx2, const float* __restrict__ Y,
                                                       const float* __restrict__ scale, double* __restrict__ z) {
@@ -309,6 +396,25 @@ void forward(int M, const float* X, float* H1, float* H2, float* Y, hipStream_t 
   layer<false, true>(M, kMlpHid, kMlpOut, H2, w + kMlpOffW3, p + kMlpOffB3, Y, X, st);
 }
 
+template <bool RELU, bool SKIP>
+void layer_bf16(int M, int N, const float* X, const uint16_t* W, const float* b, float* Y, const float* X0,
+                hipStream_t st) {
+  dim3 grid((unsigned)((N + 63) / 64), (unsigned)((M + 63) / 64));
+  hipLaunchKernelGGL((layer_bf16_kernel<RELU, SKIP>), grid, dim3(256), 0, st, M, N, X, W, b, Y, X0);
+  TTF_HIP(hipGetLastError());
+}
+
+// Inference: the same network with bf16 activations on the bf16 matrix
+// cores (training keeps the fp32 forward above, whose activations the
+// backward pass differentiates)
+void forward_infer(int M, const float* X, float* H1, float* H2, float* Y, hipStream_t st) {
+  const uint16_t* w = g_model.wbf;
+  const float* p = g_model.master;
+  layer_bf16<true, false>(M, kMlpHid, X, w + kMlpOffW1, p + kMlpOffB1, H1, nullptr, st);
+  layer_bf16<true, false>(M, kMlpHid, H1, w + kMlpOffW2, p + kMlpOffB2, H2, nullptr, st);
+  layer_bf16<false, true>(M, kMlpOut, H2, w + kMlpOffW3, p + kMlpOffB3, Y, X, st);
+}
+
 // g <- gradient of the batch loss; dY must already hold dL/dY; dH2, dH1 work
 void backward(int M, const float* X, const float* H1, const float* H2, const float* dY, float* dH2, float* dH1,
               float* g, hipStream_t st) {
@@ -401,7 +507,7 @@ void mlp_precond(int rx1, int n, int rx2, int ra1, int ra2, const double* phi1, 
   const int nb = (int)nb64;
   MlpWork w = work_for(nb, false);
   features(rx1, n, rx2, ra1, ra2, phi1, A, phi2, v, nb, 0, 1, w.X, w.scale, nullptr, st);
-  forward(nb, w.X, w.H1, w.H2, w.Y, st);
+  forward_infer(nb, w.X, w.H1, w.H2, w.Y, st);
   hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, st, rx1, n, rx2, w.Y, w.scale, z);
   TTF_HIP(hipGetLastError());
 }
@@ -478,7 +584,7 @@ int ttf_mlp_forward(int M, const float* X, float* Y) {
     TTF_REQUIRE(M >= 0, "ttf_mlp_forward: M=%d", M);
     if (M == 0) return;
     MlpWork w = work_for(M, false);
-    forward(M, X, w.H1, w.H2, Y, stream());
+    forward_infer(M, X, w.H1, w.H2, Y, stream());
   });
 }
 

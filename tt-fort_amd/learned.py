@@ -7,7 +7,7 @@ set of seeded coefficient draws of the variable-coefficient diffusion
 operator; the network is then trained on the GPU by the library's own
 forward, backward and Adam kernels (ttf_mlp_train).
 
-    python tt-fort_amd/learned.py [--draws 200] [--per-draw 500] [--epochs 40]
+    python tt-fort_amd/learned.py [--draws 60] [--per-draw 2000] [--epochs 40] [--rmax 64]
 
 writes the parameters to models/mlp_precond_n64.npy next to this file.
 """
@@ -85,8 +85,10 @@ def load(path=MODEL_PATH):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--draws", type=int, default=200)
-    ap.add_argument("--per-draw", type=int, default=500)
+    ap.add_argument("--draws", type=int, default=60)
+    ap.add_argument("--per-draw", type=int, default=2000)
+    ap.add_argument("--rmax", type=int, default=64, help="rank cap of the AMEn runs the pairs come from")
+    ap.add_argument("--d", type=int, default=6)
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--batch", type=int, default=1024)
     ap.add_argument("--lr", type=float, default=1e-3)
@@ -94,7 +96,7 @@ def main():
     args = ap.parse_args()
     lib = ttfort.load()
     ttfort.check(lib.ttf_init(0), lib)
-    X, T, info = collect_pairs(args.draws, args.per_draw, verb=1)
+    X, T, info = collect_pairs(args.draws, args.per_draw, d=args.d, rmax=args.rmax, verb=1)
     print("collected %d pairs from %d draws in %.1f s" % (info["pairs"], args.draws, info["seconds"]), flush=True)
     theta, hist, secs = train(X, T, args.epochs, args.batch, args.lr)
     print("trained %d epochs in %.2f s: loss %s" % (args.epochs, secs, " ".join("%.4g" % h for h in hist)),
