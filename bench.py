@@ -349,8 +349,10 @@ def _tt_expect(d, n, ra, crA, cr, ry):
         y = cr[py:py + ry[k] * n[k] * ry[k + 1]].reshape((ry[k], n[k], ry[k + 1]), order="F")
         pa += a.size
         py += y.size
-        L = np.einsum("xay,xiX,aijA,yjY->XAY", L, y.conj(), a, y, optimize=True)
-        N = np.einsum("xy,xiX,yiY->XY", N, y.conj(), y, optimize=True)
+        T = np.tensordot(L, y, axes=([2], [0]))             # (x, a, j, Y)
+        T = np.tensordot(T, a, axes=([1, 2], [0, 2]))       # (x, Y, i, A)
+        L = np.tensordot(y.conj(), T, axes=([0, 1], [0, 2])).transpose(0, 2, 1)   # (X, A, Y)
+        N = np.tensordot(y.conj(), np.tensordot(N, y, axes=([1], [0])), axes=([0, 1], [0, 1]))
     return complex(L.reshape(-1)[0] / N.reshape(-1)[0])
 
 
