@@ -105,15 +105,17 @@ __global__ void jac_blocks_kernel(char type, int rx1, int n, int rx2, int ra1, i
 // partial (row) pivoting in LDS, one work-group per matrix; the row
 // interchanges are undone as column interchanges at the end.  A zero pivot
 // leaves the block's info flag set (the first such step) and the block's
-// entries non-finite.
+// entries non-finite.  With todo, only the blocks whose todo flag is set.
 constexpr int kInvMaxN = 128;
 
 __global__ __launch_bounds__(256) void batched_inverse_kernel(int n, double* __restrict__ M, int64_t ldm,
-                                                              int64_t stride, int* __restrict__ info) {
+                                                              int64_t stride, int* __restrict__ info,
+                                                              const int* __restrict__ todo) {
   extern __shared__ double a[];   // column-major, ld = n + 1
   __shared__ int piv[kInvMaxN];
   __shared__ double rmax[256];
   __shared__ int rarg[256];
+  if (todo && todo[blockIdx.x] == 0) return;   // uniform: a block another kernel inverted
   const int ld = n + 1, tid = threadIdx.x, nt = blockDim.x;
   double* g = M + (int64_t)blockIdx.x * stride;
   for (int e = tid; e < n * n; e += nt) a[(e % n) + ld * (e / n)] = g[(e % n) + ldm * (e / n)];
@@ -187,12 +189,13 @@ __global__ __launch_bounds__(256) void batched_inverse_kernel(int n, double* __r
 // Step k broadcasts column k and row k through LDS, double-buffered so that
 // each step has one barrier.  The pivots are those of the LDL^T
 // factorisation, all positive exactly when the block is positive definite;
-// a pivot that is not positive sets info[b] (the block then holds no
-// inverse).
+// a pivot that is not positive sets info[b] and leaves the block as it
+// was.  A block whose info[b] is already set is skipped.
 __global__ __launch_bounds__(256) void spd_inverse_tiled_kernel(int n, double* __restrict__ M, int64_t ldm,
                                                                 int64_t stride, int* __restrict__ info) {
   __shared__ double colb[2][128];
   __shared__ double rowb[2][128];
+  if (info[blockIdx.x] != 0) return;   // uniform
   const int t = threadIdx.x, ti = t & 15, tj = t >> 4;
   double* g = M + (int64_t)blockIdx.x * stride;
   double a[8][8];
@@ -243,7 +246,11 @@ __global__ __launch_bounds__(256) void spd_inverse_tiled_kernel(int n, double* _
       }
     }
   }
-  if (t == 0 && bad) info[blockIdx.x] = 1;
+  // every thread read the same pivots, so `bad` is the same everywhere
+  if (bad) {
+    if (t == 0) info[blockIdx.x] = 1;
+    return;
+  }
 #pragma unroll
   for (int s = 0; s < 8; ++s)
 #pragma unroll
@@ -251,6 +258,19 @@ __global__ __launch_bounds__(256) void spd_inverse_tiled_kernel(int n, double* _
       const int i = 8 * ti + r, j = 8 * tj + s;
       if (i < n && j < n) g[i + ldm * j] = a[r][s];
     }
+}
+
+// flag[b] = 1 unless block b (n x n, consecutive blocks of n^2) is exactly
+// symmetric with a positive diagonal; one work-group per block.
+__global__ __launch_bounds__(256) void sym_check_kernel(int n, const double* __restrict__ M, int* __restrict__ flag) {
+  const double* g = M + (int64_t)blockIdx.x * n * n;
+  bool bad = false;
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int i = e % n, j = e / n;
+    if (i < j) bad = bad || g[i + (int64_t)n * j] != g[j + (int64_t)n * i];
+    else if (i == j) bad = bad || !(g[i + (int64_t)n * i] > 0.0);
+  }
+  if (bad) flag[blockIdx.x] = 1;   // every writer stores the same value
 }
 
 // y(blk) = J(blk) x(blk) for all blocks: block b has size bs, matrix at
@@ -310,8 +330,9 @@ void jac_gen(char type, int rx1, int n, int rx2, int ra1, int ra2, const double*
   hipLaunchKernelGGL(jac_blocks_kernel, dim3(grid_for(total)), dim3(256), 0, st, type, rx1, n, rx2, ra1, ra2,
                      Phi1, A, Phi2, jacs);
   TTF_HIP(hipGetLastError());
-  int* info = (int*)arena(kArenaSmall, sizeof(int) * (size_t)nblk);
-  TTF_HIP(hipMemsetAsync(info, 0, sizeof(int) * (size_t)nblk, st));
+  int* info = (int*)arena(kArenaSmall, sizeof(int) * (size_t)nblk * 2);
+  int* other = info + nblk;   // blocks left to the pivoting kernel
+  TTF_HIP(hipMemsetAsync(info, 0, sizeof(int) * (size_t)nblk * 2, st));
   const size_t lds = sizeof(double) * (size_t)bs * (bs + 1);
   static std::atomic<bool> attr{false};   // concurrent first calls only repeat the call
   if (!attr) {
@@ -319,8 +340,17 @@ void jac_gen(char type, int rx1, int n, int rx2, int ra1, int ra2, const double*
                                 (int)(sizeof(double) * kInvMaxN * (kInvMaxN + 1))));
     attr = true;
   }
+  // Exactly symmetric blocks with positive pivots (the blocks of symmetric
+  // operators: identical products in identical order make B(i, j) and
+  // B(j, i) the same double) go through the register-tiled inverse without
+  // pivoting; the others, and any that turns out not positive definite,
+  // through Gauss-Jordan with partial pivoting.  Both are the inverse to
+  // rounding, which is all dgesv's solve promises.
+  hipLaunchKernelGGL(sym_check_kernel, dim3((unsigned)nblk), dim3(256), 0, st, bs, jacs, other);
+  hipLaunchKernelGGL(spd_inverse_tiled_kernel, dim3((unsigned)nblk), dim3(256), 0, st, bs, jacs, (int64_t)bs,
+                     (int64_t)bs * bs, other);
   hipLaunchKernelGGL(batched_inverse_kernel, dim3((unsigned)nblk), dim3(256), lds, st, bs, jacs, (int64_t)bs,
-                     (int64_t)bs * bs, info);
+                     (int64_t)bs * bs, info, (const int*)other);
   TTF_HIP(hipGetLastError());
   std::vector<int> hinfo(nblk);
   read_back(hinfo.data(), info, sizeof(int) * (size_t)nblk, st);
@@ -339,7 +369,7 @@ void dense_inverse(int n, double* M, hipStream_t st) {
     attr = true;
   }
   hipLaunchKernelGGL(batched_inverse_kernel, dim3(1), dim3(256), sizeof(double) * (size_t)n * (n + 1), st, n, M,
-                     (int64_t)n, (int64_t)n * n, info);
+                     (int64_t)n, (int64_t)n * n, info, nullptr);
   TTF_HIP(hipGetLastError());
   int h = 0;
   read_back(&h, info, sizeof(int), st);
