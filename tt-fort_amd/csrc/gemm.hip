@@ -82,6 +82,12 @@ static bool panel_eligible(const GemmOp<T>& op) {
 }
 
 // ---------------------------------------------------------------- autotuner
+// Set while this thread records a stream capture: untuned shapes then take
+// the heuristic configuration instead of timing runs (not recorded, so a
+// later eager call still tunes them).
+static thread_local bool t_no_tune = false;
+void gemm_no_tuning(bool on) { t_no_tune = on; }
+
 struct Cfg {
   int bm, bk, ks;   // tiles bm x bm (or bm x bn), K chunk bk, split-K factor ks
   int bn = 0;       // 0: square tiles; 80: 64 x 80 tiles, four waves stacked in rows
@@ -320,10 +326,11 @@ void gemm_launch(GemmOp<T> op, hipStream_t st) {
   }
   if (!have) {
     const bool alias = (const void*)op.C == (const void*)op.A || (const void*)op.C == (const void*)op.B;
-    const bool tuned = autotune_enabled() && op.beta == T(0) && !alias && flops >= 4e6;
+    // no timing runs while a stream capture records this thread's launches
+    const bool tuned = autotune_enabled() && !t_no_tune && op.beta == T(0) && !alias && flops >= 4e6;
     c = tuned ? tune(op, st) : heuristic(op.M, op.N, op.K, op.nbatch);
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tuned[key] = c;
+    if (!t_no_tune) g_tuned[key] = c;
     if (tuned) record_tuned(key, c);
   }
   if (c.ks > 1) op.work = splitk_work(op, c.ks);

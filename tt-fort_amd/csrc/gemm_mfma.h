@@ -501,6 +501,10 @@ void gemm_launch_ws(GemmOp<T> op, T* ws, size_t ws_cap, hipStream_t st);
 void zgemm_launch(ZGemmOp op, hipStream_t st);
 void cgemm_launch(CGemmOp op, hipStream_t st);
 
+// While on, gemm_launch on this thread runs no tuning timings (for code
+// that records a stream capture); untuned shapes use the heuristic tiles.
+void gemm_no_tuning(bool on);
+
 // Convenience constructor for a column-major GEMM with optional transposes:
 // C (M x N, leading dim ldc) = alpha * op(A) op(B) + beta * C.
 template <typename T>

@@ -482,6 +482,18 @@ static double eig_block_shift() {
   }();
   return v;
 }
+// HIP graphs for the FP64 iterations, on with TTF_EIG_GRAPHS=1.  Off by
+// default: on the Heisenberg d = 64, rmax 128 sweep recording an iteration
+// (about 20 launches) and updating the kept executable cost more than the
+// replays save over a local solve's 30-40 iterations (2.24 against 2.21 s
+// per sweep); the results are the same bits either way.
+static bool eig_graphs_enabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("TTF_EIG_GRAPHS");
+    return e && std::atoi(e) != 0;
+  }();
+  return on;
+}
 constexpr int kBlockPrecMaxOrder = 256;
 constexpr size_t kBlockPrecMaxBytes = size_t(1) << 30;
 
@@ -648,11 +660,14 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
   }
   const int res_grid = (int)std::min<int64_t>(256, ceil_div(N, 256));
   const size_t res_bytes = sizeof(double) * (size_t)res_grid * bs;
-  for (int it = 0;; ++it) {
-    double* S = Sset(cur);
-    double* X = S, *P = S + colsz * bs, *W = S + colsz * 2 * bs;
-    double* AS = S + colsz * 3 * bs;
-    double* AX = AS, *AW = AS + colsz * 2 * bs;
+  // An iteration is part A (residual block, its norms, preconditioner) and
+  // part B (projections, orthonormalisation, A W, Rayleigh-Ritz, next
+  // basis set), with the host's convergence decision between them; part B
+  // of set c is followed at once by part A of the next set.
+  auto partA = [&](int c) {
+    double* S = Sset(c);
+    double* X = S, *W = S + colsz * 2 * bs;
+    double* AX = S + colsz * 3 * bs;
     double* res_part = (double*)arena(kArenaGram, res_bytes);
     // the preconditioned residual goes to Wr; with the block preconditioner
     // the plain residual is staged in the W slot, free until the SVQB
@@ -670,36 +685,12 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
     hipLaunchKernelGGL(residual_norms, dim3((unsigned)ceil_div(16 * bs, 256)), dim3(256), 0, st, bs, res_grid,
                        res_part, nrm);
     TTF_HIP(hipGetLastError());
-    const bool at_cap = matvecs + bs > max_matvecs;
-    if (it < 3 || it % kSyncEvery == 0 || at_cap || force_sync) {
-      force_sync = false;
-      read_back(hbuf.data(), theta, sizeof(double) * (2 * kMaxM + 1), st);
-      anorm = std::max(anorm, hbuf[2 * kMaxM]);   // amax follows nrm in the buffer
-      double res2 = 0.0, rmax = 0.0;
-      for (int i = 0; i < nev; ++i) {
-        res2 += hbuf[kMaxM + i] * hbuf[kMaxM + i];
-        rmax = std::max(rmax, hbuf[kMaxM + i]);
-      }
-      stats.iters = it;
-      stats.res = std::sqrt(res2);
-      const bool done = rmax <= tol * std::max(anorm, 1e-300) || at_cap;
-      const double rswitch = kSwitch * std::max(anorm, 1e-300);
-      if (mix && !fresh) {
-        // decide on FP64 residuals: refresh at the tolerance or the cap, at
-        // the switch level, and whenever the maintained residuals fell
-        // 1000-fold since the last exact ones (the fp32 errors in A X scale
-        // with the residual at the last refresh)
-        if (done || rmax <= 1e-3 * r_ref || rmax <= rswitch) {
-          refresh();
-          continue;
-        }
-      }
-      if (done) break;
-      if (fresh) {
-        r_ref = rmax;
-        if (mix && rmax <= rswitch) fp64_finish = true;
-      }
-    }
+  };
+  auto partB = [&](int c, bool aw32) {
+    double* S = Sset(c);
+    double* X = S, *W = S + colsz * 2 * bs;
+    double* AS = S + colsz * 3 * bs;
+    double* AW = AS + colsz * 2 * bs;
     // Wr <- Wr - [X P] ([X P]^T Wr), twice (P is zero before the first update)
     for (int pass = 0; pass < 2; ++pass) {
       tsgram(N, 2 * bs, bs, X, ld, Wr, ld, Gm, 2 * bs, st);
@@ -709,22 +700,15 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
     tsgram(N, bs, bs, Wr, ld, Wr, ld, Gm, bs, st);
     orth_block(bs, Gm, Tm, maskW, st);
     mul_small(N, bs, bs, Wr, ld, Tm, bs, W, ld, 0.0, st);
-    const bool aw32 = mix && !fp64_finish;
-    if (aw32) {
-      mix->apply_block(bs, W, ld, AW, ld, st);
-      stats.matvecs32 += bs;
-      fresh = false;
-    } else {
-      op.apply(bs, W, ld, AW, ld, st);
-    }
-    matvecs += bs;
+    if (aw32) mix->apply_block(bs, W, ld, AW, ld, st);
+    else op.apply(bs, W, ld, AW, ld, st);
     // Rayleigh-Ritz on S = [X P W]
     const int m = 3 * bs;
     tsgram(N, m, m, S, ld, AS, ld, Hm, m, st);
-    hipLaunchKernelGGL(rr_kernel, dim3(1), dim3(256), 0, st, m, bs, bs, Hm, maskPb[cur & 1], maskW, theta, CZ,
-                       maskPb[(cur + 1) & 1], amax, aw32 ? 1 : 0);
+    hipLaunchKernelGGL(rr_kernel, dim3(1), dim3(256), 0, st, m, bs, bs, Hm, maskPb[c & 1], maskW, theta, CZ,
+                       maskPb[(c + 1) & 1], amax, aw32 ? 1 : 0);
     // next set: [X P] = S [C Z] and [AX AP] = AS [C Z], one batched product
-    double* Sn = Sset(1 - cur);
+    double* Sn = Sset(1 - c);
     GemmOp<double> g = colmajor<double>('N', 'N', N, 2 * bs, m, 1.0, S, ld, CZ, m, 0.0, Sn, ld);
     g.nb0 = 2;
     g.nbatch = 2;
@@ -732,8 +716,129 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
     g.sBb0 = 0;
     g.sCb0 = (int64_t)colsz * 3 * bs;   // next S -> next AS
     gemm_launch(g, st);
-    cur = 1 - cur;
+  };
+  // The FP64 path replays B(c) + A(1 - c) from two captured HIP graphs, one
+  // per basis set, after one iteration of eager launches has tuned every
+  // GEMM shape and sized every arena (a graph keeps its pointers).  The
+  // kernels and their arguments are the eager ones, so the results are the
+  // same bits.
+  bool graphs = eig_graphs_enabled() && !mix;
+  // Instantiated graphs are kept per thread between local solves: a solve
+  // of the same shape records its own iteration and updates the kept
+  // executable in place (hipGraphExecUpdate), which costs far less than a
+  // new instantiation; another shape, or an update the runtime refuses,
+  // instantiates afresh.
+  struct Kept {
+    int64_t key[4] = {-1, -1, -1, -1};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+  };
+  static thread_local Kept kept;
+  const int64_t key[4] = {N, bs, Binv ? pnb : 0, Dg ? 1 : 0};
+  if (!std::equal(key, key + 4, kept.key)) {
+    for (auto& ge : kept.exec)
+      if (ge) {
+        (void)hipGraphExecDestroy(ge);
+        ge = nullptr;
+      }
+    std::copy(key, key + 4, kept.key);
   }
+  hipGraphExec_t* gexec = kept.exec;
+  bool fresh_graph[2] = {false, false};   // updated or instantiated for this solve
+  auto release = [&]() {};
+  // Records B(c) + A(1 - c) into gexec[c]; false (nothing launched, the
+  // stream back to normal) when the recording fails, e.g. because a call in
+  // it needs a synchronisation: the solve then stays on eager launches.
+  auto capture = [&](int c) -> bool {
+    hipGraph_t gr = nullptr;
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    gemm_no_tuning(true);
+    bool ok = true;
+    try {
+      partB(c, false);
+      partA(1 - c);
+    } catch (...) {
+      ok = false;
+    }
+    gemm_no_tuning(false);
+    const hipError_t e1 = hipStreamEndCapture(st, &gr);
+    ok = ok && e1 == hipSuccess && gr;
+    if (ok && gexec[c]) {
+      hipGraphExecUpdateResult res;
+      hipGraphNode_t bad = nullptr;
+      if (hipGraphExecUpdate(gexec[c], gr, &bad, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+        (void)hipGetLastError();
+        (void)hipGraphExecDestroy(gexec[c]);
+        gexec[c] = nullptr;
+      }
+    }
+    if (ok && !gexec[c]) ok = hipGraphInstantiate(&gexec[c], gr, nullptr, nullptr, 0) == hipSuccess;
+    if (gr) (void)hipGraphDestroy(gr);
+    if (!ok) {
+      if (gexec[c]) (void)hipGraphExecDestroy(gexec[c]);
+      gexec[c] = nullptr;
+      (void)hipGetLastError();
+      ttf_clear_error();   // the failed recording is not an error of the solve
+    }
+    fresh_graph[c] = ok;
+    return ok;
+  };
+  try {
+    partA(cur);
+    for (int it = 0;; ++it) {
+      const bool at_cap = matvecs + bs > max_matvecs;
+      if (it < 3 || it % kSyncEvery == 0 || at_cap || force_sync) {
+        force_sync = false;
+        read_back(hbuf.data(), theta, sizeof(double) * (2 * kMaxM + 1), st);
+        anorm = std::max(anorm, hbuf[2 * kMaxM]);   // amax follows nrm in the buffer
+        double res2 = 0.0, rmax = 0.0;
+        for (int i = 0; i < nev; ++i) {
+          res2 += hbuf[kMaxM + i] * hbuf[kMaxM + i];
+          rmax = std::max(rmax, hbuf[kMaxM + i]);
+        }
+        stats.iters = it;
+        stats.res = std::sqrt(res2);
+        const bool done = rmax <= tol * std::max(anorm, 1e-300) || at_cap;
+        const double rswitch = kSwitch * std::max(anorm, 1e-300);
+        if (mix && !fresh) {
+          // decide on FP64 residuals: refresh at the tolerance or the cap,
+          // at the switch level, and whenever the maintained residuals fell
+          // 1000-fold since the last exact ones (the fp32 errors in A X
+          // scale with the residual at the last refresh)
+          if (done || rmax <= 1e-3 * r_ref || rmax <= rswitch) {
+            refresh();
+            partA(cur);
+            continue;
+          }
+        }
+        if (done) break;
+        if (fresh) {
+          r_ref = rmax;
+          if (mix && rmax <= rswitch) fp64_finish = true;
+        }
+      }
+      const bool aw32 = mix && !fp64_finish;
+      if (graphs && it >= 1 && !fresh_graph[cur] && !capture(cur)) graphs = false;
+      if (graphs && it >= 1) {
+        TTF_HIP(hipGraphLaunch(gexec[cur], st));
+      } else {
+        partB(cur, aw32);
+        partA(1 - cur);
+      }
+      if (aw32) {
+        stats.matvecs32 += bs;
+        fresh = false;
+      }
+      matvecs += bs;
+      cur = 1 - cur;
+    }
+  } catch (...) {
+    release();
+    throw;
+  }
+  release();
   copy_cols(Sset(cur), ld, Xout, N, N, nev, st);
   for (int i = 0; i < nev; ++i) lambda[i] = hbuf[i];
   stats.matvecs = matvecs;
