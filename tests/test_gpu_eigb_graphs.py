@@ -3,12 +3,10 @@ from captured HIP graphs (one per basis set, recorded after one iteration of
 eager launches and kept between local solves of one shape).  The graphs hold
 the same kernels with the same arguments as the eager launches, so a whole
 tt_eigb run must give the same bits with graphs and without them (the
-default), each in a child process: the
-eigenvalues, the result cores, the ranks and the matvec counts, over many
-local solves with their replays and the host synchronisations between
-them.  The run also covers the mixed-precision path, which launches eagerly,
-and a second call in the same process, whose local solves capture their own
-graphs again.
+default), each in a child process: the eigenvalues, the result cores, the
+ranks and the matvec counts, over many local solves with their replays and
+the host synchronisations between them, and over a second call in the same
+process, whose local solves update the graphs kept from the first.
 """
 import json
 import os
