@@ -24,6 +24,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <functional>
 #include <vector>
 
@@ -217,6 +218,211 @@ __global__ __launch_bounds__(256) void zqr_gs_kernel(int rows, int cols, int rne
     __syncthreads();
     orthogonalise(a, c);
     const double n1 = block_norm2(a);
+    const double inv = n1 > 0.0 ? 1.0 / sqrt(n1) : 0.0;
+    for (int r = tid; r < rows; r += nt) a[r] = make_hipDoubleComplex(a[r].x * inv, a[r].y * inv);
+    __syncthreads();
+  }
+  for (int e = tid; e < rows * rnew; e += nt) Q[e] = L[e];
+}
+
+// The same factorisation by 8-column panels: each panel is projected
+// against the basis accepted so far by block products that all 256 threads
+// share (Gram-Schmidt twice: H = Q^H P, P -= Q H), then its columns are
+// taken in order against the panel's own accepted columns.  A column is
+// dependent when its remainder is below 1e-13 of its norm before the panel's
+// projections; missing directions are completed as above.  Per column this
+// needs two barriers in place of about ten, and the long updates run on all
+// threads instead of one thread per row.
+constexpr int kQrPanelCols = 8;
+
+__global__ __launch_bounds__(256) void zqr_panel_gs_kernel(int rows, int cols, int rnew, const zc* __restrict__ A,
+                                                           zc* __restrict__ Q) {
+  extern __shared__ zc L[];   // column c at L + c * rows; the basis is compacted into the leading columns
+  __shared__ zc H[64 * kQrPanelCols];   // H(i, c) at i + 64 c
+  __shared__ double n0[kQrPanelCols];
+  __shared__ double red[4][kQrPanelCols];
+  __shared__ int nb_s;
+  const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6;
+  for (int e = tid; e < rows * cols; e += nt) L[e] = A[e];
+  if (tid == 0) nb_s = 0;
+  __syncthreads();
+  // squared norms of panel columns j0..j0+jp-1 into out (wave sums, then the
+  // four waves' partials added by each reader)
+  auto panel_norms = [&](int j0, int jp, double* out) {
+    double s[kQrPanelCols];
+#pragma unroll
+    for (int c = 0; c < kQrPanelCols; ++c) s[c] = 0.0;
+    for (int r = tid; r < rows; r += nt)
+#pragma unroll
+      for (int c = 0; c < kQrPanelCols; ++c)
+        if (c < jp) {
+          const zc v = L[(size_t)(j0 + c) * rows + r];
+          s[c] += v.x * v.x + v.y * v.y;
+        }
+#pragma unroll
+    for (int c = 0; c < kQrPanelCols; ++c) {
+      const double t = wave_sum64(s[c]);
+      if (lane == 0) red[wv][c] = t;
+    }
+    __syncthreads();
+    if (tid < kQrPanelCols) out[tid] = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
+    __syncthreads();
+  };
+  for (int j0 = 0; j0 < cols; j0 += kQrPanelCols) {
+    const int nb = nb_s;
+    if (nb >= rnew) break;
+    const int jp = min(kQrPanelCols, cols - j0);
+    panel_norms(j0, jp, n0);
+    // project the panel against the nb accepted columns, twice
+    for (int pass = 0; pass < 2 && nb > 0; ++pass) {
+      // H(i, c) = q_i^H p_c: each DPP row (16 lanes) takes one basis column
+      // i and all jp panel columns, a sixteenth of the rows per lane, and
+      // sums over the row; 16 basis columns per round
+      {
+        const int g = tid >> 4, l16 = tid & 15;
+        for (int i0 = 0; i0 < nb; i0 += nt >> 4) {
+          const int i = i0 + g;
+          const bool oki = i < nb;
+          const zc* qi = L + (size_t)(oki ? i : 0) * rows;
+          double re[kQrPanelCols], im[kQrPanelCols];
+#pragma unroll
+          for (int c = 0; c < kQrPanelCols; ++c) re[c] = im[c] = 0.0;
+          for (int r = l16; r < rows; r += 16) {
+            const zc q = qi[r];
+#pragma unroll
+            for (int c = 0; c < kQrPanelCols; ++c)
+              if (c < jp) {
+                const zc v = L[(size_t)(j0 + c) * rows + r];
+                re[c] += q.x * v.x + q.y * v.y;
+                im[c] += q.x * v.y - q.y * v.x;
+              }
+          }
+#pragma unroll
+          for (int c = 0; c < kQrPanelCols; ++c) {
+            re[c] = row_sum16(re[c]);
+            im[c] = row_sum16(im[c]);
+          }
+          if (oki && l16 < jp) {
+            // lane l16 stores column l16 (a static index through a select chain)
+            double rv = re[0], iv = im[0];
+#pragma unroll
+            for (int c = 1; c < kQrPanelCols; ++c)
+              if (l16 == c) { rv = re[c]; iv = im[c]; }
+            H[i + 64 * l16] = make_hipDoubleComplex(rv, iv);
+          }
+        }
+      }
+      __syncthreads();
+      // p_c(r) -= sum_i q_i(r) H(i, c): a thread takes one row and every
+      // (nt / rows)-th panel column, so each q_i(r) it loads serves them all
+      {
+        const int per = max(1, nt / rows);   // threads sharing a row
+        for (int e = tid; e < rows * per; e += nt) {
+          const int r = e % rows, c0 = e / rows;
+          zc acc[kQrPanelCols];
+#pragma unroll
+          for (int k = 0; k < kQrPanelCols; ++k) acc[k] = make_hipDoubleComplex(0.0, 0.0);
+          for (int i = 0; i < nb; ++i) {
+            const zc q = L[(size_t)i * rows + r];
+#pragma unroll
+            for (int k = 0; k < kQrPanelCols; ++k) {
+              const int c = c0 + per * k;
+              if (c < jp) acc[k] = zadd(acc[k], zmul(H[i + 64 * c], q));
+            }
+          }
+#pragma unroll
+          for (int k = 0; k < kQrPanelCols; ++k) {
+            const int c = c0 + per * k;
+            if (c < jp) {
+              zc* p = L + (size_t)(j0 + c) * rows + r;
+              *p = zsub(*p, acc[k]);
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+    // the panel's columns in order, against the columns this panel accepted
+    // (at most kQrPanelCols - 1): one wave per dot product
+    for (int c = 0; c < jp; ++c) {
+      const int b = nb_s, b0 = nb;   // accepted so far; of them, this panel's are b0..b-1
+      if (b >= rnew) break;
+      zc* a = L + (size_t)(j0 + c) * rows;
+      for (int pass = 0; pass < 2 && b > b0; ++pass) {
+        for (int i = b0 + wv; i < b; i += nt >> 6) {
+          const zc* qi = L + (size_t)i * rows;
+          double re = 0.0, im = 0.0;
+          for (int r = lane; r < rows; r += 64) {
+            re += qi[r].x * a[r].x + qi[r].y * a[r].y;
+            im += qi[r].x * a[r].y - qi[r].y * a[r].x;
+          }
+          re = wave_sum64(re);
+          im = wave_sum64(im);
+          if (lane == 0) H[i - b0] = make_hipDoubleComplex(re, im);
+        }
+        __syncthreads();
+        for (int r = tid; r < rows; r += nt) {
+          zc s0 = a[r];
+          for (int i = b0; i < b; ++i) s0 = zsub(s0, zmul(H[i - b0], L[(size_t)i * rows + r]));
+          a[r] = s0;
+        }
+        __syncthreads();
+      }
+      double sq = 0.0;
+      for (int r = tid; r < rows; r += nt) sq += a[r].x * a[r].x + a[r].y * a[r].y;
+      sq = wave_sum64(sq);
+      if (lane == 0) red[wv][0] = sq;
+      __syncthreads();
+      const double n1 = (red[0][0] + red[1][0]) + (red[2][0] + red[3][0]);
+      const bool accept = n1 > 1e-26 * n0[c] && n1 > 1e-300;
+      if (accept) {
+        // slot b is this column's own or an earlier, finished one
+        const double inv = 1.0 / sqrt(n1);
+        zc* dst = L + (size_t)b * rows;
+        for (int r = tid; r < rows; r += nt) dst[r] = make_hipDoubleComplex(a[r].x * inv, a[r].y * inv);
+      }
+      __syncthreads();
+      if (accept && tid == 0) nb_s = b + 1;
+      __syncthreads();
+    }
+  }
+  // completion: columns nb..rnew-1 (their LDS slots are free), each against
+  // all the columns before it, twice
+  for (int c = nb_s; c < rnew; ++c) {
+    zc* a = L + (size_t)c * rows;
+    for (int r = tid; r < rows; r += nt) {
+      unsigned hsh = (unsigned)(r * 2654435761u) ^ (unsigned)(c * 40503u + 977u);
+      hsh ^= hsh >> 13; hsh *= 0x5bd1e995u; hsh ^= hsh >> 15;
+      a[r] = make_hipDoubleComplex((double)(hsh & 0xffff) / 65536.0 - 0.5, (double)(hsh >> 16) / 65536.0 - 0.5);
+    }
+    __syncthreads();
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int i = tid >> 2; i < c; i += nt >> 2) {
+        const int part = tid & 3;
+        const zc* qi = L + (size_t)i * rows;
+        double re = 0.0, im = 0.0;
+        for (int r = part; r < rows; r += 4) {
+          re += qi[r].x * a[r].x + qi[r].y * a[r].y;
+          im += qi[r].x * a[r].y - qi[r].y * a[r].x;
+        }
+        re = quad_sum(re);
+        im = quad_sum(im);
+        if (part == 0) H[i] = make_hipDoubleComplex(re, im);
+      }
+      __syncthreads();
+      for (int r = tid; r < rows; r += nt) {
+        zc s0 = a[r];
+        for (int i = 0; i < c; ++i) s0 = zsub(s0, zmul(H[i], L[(size_t)i * rows + r]));
+        a[r] = s0;
+      }
+      __syncthreads();
+    }
+    double s = 0.0;
+    for (int r = tid; r < rows; r += nt) s += a[r].x * a[r].x + a[r].y * a[r].y;
+    s = wave_sum64(s);
+    if (lane == 0) red[wv][0] = s;
+    __syncthreads();
+    const double n1 = (red[0][0] + red[1][0]) + (red[2][0] + red[3][0]);
     const double inv = n1 > 0.0 ? 1.0 / sqrt(n1) : 0.0;
     for (int r = tid; r < rows; r += nt) a[r] = make_hipDoubleComplex(a[r].x * inv, a[r].y * inv);
     __syncthreads();
@@ -687,6 +893,16 @@ ExpInfo zexp_mv(int64_t n, int order, double tau, const zc* v, zc* w, double tol
   return info;
 }
 
+// The panel-blocked complex QR (TTF_ZQR_PANELS=0 selects the column-by-column
+// kernel).
+static bool zqr_panels() {
+  static const bool on = [] {
+    const char* e = std::getenv("TTF_ZQR_PANELS");
+    return !(e && std::atoi(e) == 0);
+  }();
+  return on;
+}
+
 void zqr(int rows, int cols, const zc* A, zc* Q, zc* R, hipStream_t st) {
   const int rnew = std::min(rows, cols);
   TTF_REQUIRE((int64_t)rows * cols <= kQrLds, "zqr: %d x %d block exceeds %d entries", rows, cols, kQrLds);
@@ -695,9 +911,16 @@ void zqr(int rows, int cols, const zc* A, zc* Q, zc* R, hipStream_t st) {
   if (!attr) {
     TTF_HIP(hipFuncSetAttribute((const void*)zqr_gs_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)(sizeof(zc) * kQrLds)));
+    TTF_HIP(hipFuncSetAttribute((const void*)zqr_panel_gs_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(sizeof(zc) * kQrLds)));
     attr = true;
   }
-  hipLaunchKernelGGL(zqr_gs_kernel, dim3(1), dim3(256), sizeof(zc) * (size_t)rows * cols, st, rows, cols, rnew, A, Q);
+  if (zqr_panels())
+    hipLaunchKernelGGL(zqr_panel_gs_kernel, dim3(1), dim3(256), sizeof(zc) * (size_t)rows * cols, st, rows, cols,
+                       rnew, A, Q);
+  else
+    hipLaunchKernelGGL(zqr_gs_kernel, dim3(1), dim3(256), sizeof(zc) * (size_t)rows * cols, st, rows, cols, rnew, A,
+                       Q);
   TTF_HIP(hipGetLastError());
   // R = Q^H A (rnew x cols)
   zgemm_cm('C', 'N', rnew, cols, rows, make_hipDoubleComplex(1.0, 0.0), Q, rows, A, rows,
