@@ -64,8 +64,11 @@ def test_gesvd_graded_relative_accuracy(ttlib):
     assert np.allclose(U.T @ U, np.eye(n), atol=1e-12)
 
 
-@pytest.mark.parametrize("n", [1, 2, 5, 24, 48, 64])
+@pytest.mark.parametrize("n", [1, 2, 5, 24, 31, 32, 33, 48, 64])
 def test_syev_small(ttlib, n):
+    """Orders up to 32 take the slot-ordered Jacobi (one 2x2 block per
+    thread), 33 and above the double-buffered circle one; odd orders are
+    padded with a zero row."""
     import ttfort
     rng = np.random.default_rng(n)
     As = []

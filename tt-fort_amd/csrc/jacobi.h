@@ -71,9 +71,9 @@ __device__ __forceinline__ bool jacobi_go(double apq, double app, double aqq, do
 // the next sweep would not change the result and is skipped.
 constexpr double kJacobiSmall = 1e-9;
 
-// Diagonalises the symmetric n x n matrix held in A (row stride LD, n <= 64)
-// and accumulates the rotations into V (V <- V J).  B is a second buffer:
-// each round of the circle schedule reads one buffer and writes the other,
+// The double-buffered form, for any n <= 64: diagonalises the symmetric
+// n x n matrix held in A (row stride LD) and accumulates the rotations into
+// V (V <- V J).  B is a second buffer: each round of the circle schedule reads one buffer and writes the other,
 // so one barrier per round suffices.  Odd n is padded with a zero row and
 // column n (in A, B and V; the buffers hold at least n + 1 rows), so every
 // pair is real and no load is conditional; rotations involving the pad are
@@ -90,8 +90,8 @@ constexpr double kJacobiSmall = 1e-9;
 // of the block call it; the returned pointer (A or B) holds the diagonalised
 // matrix.
 template <int LD>
-__device__ double (*jacobi_sym(double (*A)[LD], double (*B)[LD], double (*V)[LD], int n, int max_sweeps,
-                               double tol_abs, int* flags, double small = kJacobiSmall))[LD] {
+__device__ double (*jacobi_sym_circle(double (*A)[LD], double (*B)[LD], double (*V)[LD], int n, int max_sweeps,
+                                      double tol_abs, int* flags, double small = kJacobiSmall))[LD] {
   const int tid = threadIdx.x, nt = blockDim.x;
   const int nn = n + (n & 1);
   const int half = nn / 2;
@@ -165,6 +165,137 @@ __device__ double (*jacobi_sym(double (*A)[LD], double (*B)[LD], double (*V)[LD]
     if (!*moved) break;
   }
   return cur;
+}
+
+// The same circle schedule with the matrix kept in slot order: slots 2k and
+// 2k+1 hold the pair that round's rotation k acts on, and between rounds the
+// contents move by one fixed permutation of the slots (the circle's
+// rotation), so that after the nn - 1 rounds of a sweep every index is back
+// in its starting slot.  Slot s starts with index jacobi_slot_index(s).
+__device__ inline int jacobi_slot_index(int s, int nn) {
+  return (s & 1) == 0 ? s >> 1 : (s == 1 ? nn - 1 : nn - 1 - (s >> 1));
+}
+
+// The slot that the contents of slot s move to between two rounds (half =
+// nn / 2 pairs).
+__device__ inline int jacobi_slot_next(int s, int half) {
+  if (half == 1 || s == 1) return s;
+  if (s == 0) return 3;
+  if ((s & 1) == 0) return s - 2;
+  if (s == 2 * half - 1) return 2 * half - 2;
+  return s + 2;
+}
+
+// jacobi_sym for (n + 1) / 2 squared <= blockDim.x: thread (kp, kq) owns
+// the 2x2 block of slot pairs kp x kq for the whole solve, with every
+// address fixed once.  A round: each thread loads its block and its two
+// rows of the vectors' slot pair kq; the diagonal-block threads compute the
+// round's rotations and publish them; after a barrier every thread applies
+// its two rotations and stores the block at the slots its contents move to.
+// All loads of a round precede the barrier and all stores follow it, so the
+// matrix and the vectors are updated in place.  Working copies: the matrix
+// in slot order in B, the vectors (V with columns in slot order) in A; at
+// the end V receives the vectors in index order and A's diagonal the
+// eigenvalues, and A is returned.
+template <int LD>
+__device__ double (*jacobi_sym_slots(double (*A)[LD], double (*B)[LD], double (*V)[LD], int n, int max_sweeps,
+                                     double tol_abs, int* flags, double small))[LD] {
+  __shared__ double2 csb[kJacobiMaxN / 2];
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int nn = n + (n & 1);
+  const int half = nn / 2;
+  if (n & 1) {
+    for (int j = tid; j <= n; j += nt) {
+      A[n][j] = 0.0;
+      A[j][n] = 0.0;
+      V[n][j] = 0.0;
+      V[j][n] = 0.0;
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < nn * nn; e += nt) {
+    const int s = e % nn, t = e / nn;
+    B[s][t] = A[jacobi_slot_index(s, nn)][jacobi_slot_index(t, nn)];
+  }
+  __syncthreads();
+  for (int e = tid; e < nn * nn; e += nt) {
+    const int r = e % nn, s = e / nn;
+    A[r][s] = V[r][jacobi_slot_index(s, nn)];
+  }
+  if (tid < 3) flags[tid] = 0;
+  __syncthreads();
+  double(*M)[LD] = B;
+  double(*W)[LD] = A;
+  const bool task = tid < half * half;
+  const int kp = task ? tid % half : 0, kq = task ? tid / half : 0;
+  const int p0 = 2 * kp, p1 = p0 + 1, q0 = 2 * kq, q1 = q0 + 1;
+  const int np0 = jacobi_slot_next(p0, half), np1 = jacobi_slot_next(p1, half);
+  const int nq0 = jacobi_slot_next(q0, half), nq1 = jacobi_slot_next(q1, half);
+  const int r0 = kp, r1 = kp + half;   // the rows of W this thread rotates
+  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+    int* moved = flags + sweep % 3;
+    // flags[(sweep+1) % 3] was last read before the previous sweep's barriers
+    if (tid == 0) flags[(sweep + 1) % 3] = 0;
+    for (int r = 0; r < nn - 1; ++r) {
+      double m00 = 0.0, m01 = 0.0, m10 = 0.0, m11 = 0.0, w00 = 0.0, w01 = 0.0, w10 = 0.0, w11 = 0.0;
+      if (task) {
+        m00 = M[p0][q0]; m01 = M[p0][q1]; m10 = M[p1][q0]; m11 = M[p1][q1];
+        w00 = W[r0][q0]; w01 = W[r0][q1]; w10 = W[r1][q0]; w11 = W[r1][q1];
+        if (kp == kq) {   // a diagonal block: [[a_pp, a_pq], [a_pq, a_qq]]
+          double c, s;
+          jacobi_cs(m01, m00, m11, jacobi_go(m01, m00, m11, tol_abs), c, s);
+          csb[kp] = make_double2(c, s);
+          if (fabs(s) > small) *moved = 1;
+        }
+      }
+      __syncthreads();
+      if (task) {
+        const double2 rp = csb[kp], rq = csb[kq];
+        const double c1 = rp.x, s1 = rp.y, c2 = rq.x, s2 = rq.y;
+        // columns of pair kq, then rows of pair kp
+        const double t0 = c2 * m00 - s2 * m01, t1 = s2 * m00 + c2 * m01;
+        const double t2 = c2 * m10 - s2 * m11, t3 = s2 * m10 + c2 * m11;
+        M[np0][nq0] = c1 * t0 - s1 * t2;
+        M[np0][nq1] = c1 * t1 - s1 * t3;
+        M[np1][nq0] = s1 * t0 + c1 * t2;
+        M[np1][nq1] = s1 * t1 + c1 * t3;
+        W[r0][nq0] = c2 * w00 - s2 * w01;
+        W[r0][nq1] = s2 * w00 + c2 * w01;
+        W[r1][nq0] = c2 * w10 - s2 * w11;
+        W[r1][nq1] = s2 * w10 + c2 * w11;
+      }
+      __syncthreads();
+    }
+    if (!*moved) break;
+  }
+  // a sweep ends with every index in its starting slot
+  for (int e = tid; e < nn * nn; e += nt) {
+    const int r = e % nn, s = e / nn;
+    V[r][jacobi_slot_index(s, nn)] = W[r][s];
+  }
+  __syncthreads();
+  for (int s = tid; s < nn; s += nt) {
+    const int i = jacobi_slot_index(s, nn);
+    A[i][i] = M[s][s];
+  }
+  __syncthreads();
+  return A;
+}
+
+// Diagonalises the symmetric n x n matrix held in A (row stride LD, n <= 64)
+// and accumulates the rotations into V (V <- V J); B is a second buffer of
+// the same shape.  Returns the matrix (A or B) whose diagonal holds the
+// eigenvalues, in the order of V's columns.  Orders with (n + 1) / 2 squared
+// <= blockDim.x take the slot-ordered kernel above, larger ones the
+// double-buffered one.
+template <int LD>
+__device__ double (*jacobi_sym(double (*A)[LD], double (*B)[LD], double (*V)[LD], int n, int max_sweeps,
+                               double tol_abs, int* flags, double small = kJacobiSmall))[LD] {
+  if (n <= 1) return A;
+  const int half = (n + 1) / 2;
+  if (half * half <= (int)blockDim.x && n <= kJacobiMaxN)
+    return jacobi_sym_slots<LD>(A, B, V, n, max_sweeps, tol_abs, flags, small);
+  return jacobi_sym_circle<LD>(A, B, V, n, max_sweeps, tol_abs, flags, small);
 }
 
 // Block-wide sum of v over all threads (blockDim.x a multiple of 64, at most

@@ -189,14 +189,20 @@ void orth_block(int k, const double* G, double* T, int* mask, hipStream_t st) {
 // maskP_in and maskW.  Writes theta (bs), CZ (m x 2bs: Ritz coefficients C,
 // then P-direction coefficients Z), maskP (bs, validity of the new P) and
 // amax = max |valid Ritz value|.
+// The LDS arrays hold MR rows (m + 1 for the odd-order pad) with row stride
+// LD: rr_kernel<33, 34> serves m <= 32 (the even stride keeps each
+// thread's pair of Jacobi block entries 16-byte aligned for one load; the
+// 24x24 Jacobi measured 12% faster than with the stride 65) and
+// rr_kernel<kMaxM, kMaxM + 1> the rest (launch_rr picks).
+template <int MR, int LD>
 __global__ __launch_bounds__(256) void rr_kernel(int m, int bs, int nx, const double* __restrict__ H,
                                                  const int* __restrict__ maskP_in, const int* __restrict__ maskW,
                                                  double* __restrict__ theta, double* __restrict__ CZ,
                                                  int* __restrict__ maskP, double* __restrict__ amax,
                                                  int lower = 0) {
-  __shared__ double A[kMaxM][kMaxM + 1], V[kMaxM][kMaxM + 1], Y[kMaxM][kMaxM + 1];
-  __shared__ double Z[kMaxM][kMaxM / 2 + 1];
-  __shared__ double w[kMaxM], dg[kMaxM], red[16];
+  __shared__ double A[MR][LD], V[MR][LD], Y[MR][LD];
+  __shared__ double Z[MR][MR / 2 + 1];
+  __shared__ double w[MR], dg[MR], red[16];
   __shared__ int flags[3];
   const int tid = threadIdx.x;
   double mx = 0.0;
@@ -234,7 +240,7 @@ __global__ __launch_bounds__(256) void rr_kernel(int m, int bs, int nx, const do
   // entries of order 1e-14 ||H||: Ritz values within that of H's
   // eigenvalues, far inside the solver's tolerance, and the next iteration's
   // Rayleigh-Ritz starts again from the exact projection.
-  double(*D)[kMaxM + 1] = jacobi_sym<kMaxM + 1>(A, Y, V, m, 40, kEps * sqrt(f2), flags, 1e-7);
+  double(*D)[LD] = jacobi_sym<LD>(A, Y, V, m, 40, kEps * sqrt(f2), flags, 1e-7);
   for (int i = tid; i < m; i += blockDim.x) dg[i] = D[i][i];
   __syncthreads();
   // ascending order
@@ -264,7 +270,7 @@ __global__ __launch_bounds__(256) void rr_kernel(int m, int bs, int nx, const do
   __syncthreads();
   for (int pass = 0; pass < 2; ++pass) {
     // Z -= C (C^T Z): coefficient matrix computed per (i, c)
-    __shared__ double P[kMaxM / 2][kMaxM / 2 + 1];
+    __shared__ double P[MR / 2][MR / 2 + 1];
     for (int e = tid; e < bs * bs; e += blockDim.x) {
       int i = e % bs, c = e / bs;
       double s = 0.0;
@@ -304,6 +310,17 @@ __global__ __launch_bounds__(256) void rr_kernel(int m, int bs, int nx, const do
     int r = e % m, c = e / m;
     CZ[r + m * c] = (c < bs) ? Y[r][c] : Z[r][c - bs];
   }
+}
+
+// Rayleigh-Ritz launch on the basis order m (arguments of rr_kernel after m).
+static void launch_rr(hipStream_t st, int m, int bs, int nx, const double* H, const int* maskP_in, const int* maskW,
+                      double* theta, double* CZ, int* maskP, double* amax, int lower) {
+  if (m <= 32)
+    hipLaunchKernelGGL((rr_kernel<33, 34>), dim3(1), dim3(256), 0, st, m, bs, nx, H, maskP_in, maskW, theta, CZ,
+                       maskP, amax, lower);
+  else
+    hipLaunchKernelGGL((rr_kernel<kMaxM, kMaxM + 1>), dim3(1), dim3(256), 0, st, m, bs, nx, H, maskP_in, maskW,
+                       theta, CZ, maskP, amax, lower);
 }
 
 // W = AX - X diag(theta); each work-group takes a range of rows and stores
@@ -576,8 +593,7 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
     }
     op.apply(bs, X, ld, AX, ld, st);
     tsgram(N, bs, bs, X, ld, AX, ld, Hm, bs, st);
-    hipLaunchKernelGGL(rr_kernel, dim3(1), dim3(256), 0, st, bs, bs, bs, Hm, maskPb[0], maskW, theta, CZ,
-                       maskPb[1], amax);
+    launch_rr(st, bs, bs, bs, Hm, maskPb[0], maskW, theta, CZ, maskPb[1], amax, 0);
     double* Xn = Sset(1);
     mul_small(N, bs, bs, X, ld, CZ, bs, Xn, ld, 0.0, st);
     mul_small(N, bs, bs, AX, ld, CZ, bs, Xn + colsz * 3 * bs, ld, 0.0, st);
@@ -629,8 +645,7 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
     matvecs += 2 * bs;
     ++stats.refreshes;
     tsgram(N, bs, bs, X, ld, AX, ld, Hm, bs, st);
-    hipLaunchKernelGGL(rr_kernel, dim3(1), dim3(256), 0, st, bs, bs, bs, Hm, maskPb[cur & 1], maskW, theta, CZ,
-                       maskPb[(cur + 1) & 1], amax);
+    launch_rr(st, bs, bs, bs, Hm, maskPb[cur & 1], maskW, theta, CZ, maskPb[(cur + 1) & 1], amax, 0);
     double* Xn = Sset(1 - cur);
     mul_small(N, bs, bs, X, ld, CZ, bs, Xn, ld, 0.0, st);
     mul_small(N, bs, bs, AX, ld, CZ, bs, Xn + colsz * 3 * bs, ld, 0.0, st);
@@ -705,8 +720,7 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
     // Rayleigh-Ritz on S = [X P W]
     const int m = 3 * bs;
     tsgram(N, m, m, S, ld, AS, ld, Hm, m, st);
-    hipLaunchKernelGGL(rr_kernel, dim3(1), dim3(256), 0, st, m, bs, bs, Hm, maskPb[c & 1], maskW, theta, CZ,
-                       maskPb[(c + 1) & 1], amax, aw32 ? 1 : 0);
+    launch_rr(st, m, bs, bs, Hm, maskPb[c & 1], maskW, theta, CZ, maskPb[(c + 1) & 1], amax, aw32 ? 1 : 0);
     // next set: [X P] = S [C Z] and [AX AP] = AS [C Z], one batched product
     double* Sn = Sset(1 - c);
     GemmOp<double> g = colmajor<double>('N', 'N', N, 2 * bs, m, 1.0, S, ld, CZ, m, 0.0, Sn, ld);
