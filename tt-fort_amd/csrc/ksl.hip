@@ -235,8 +235,48 @@ __global__ __launch_bounds__(256) void zqr_gs_kernel(int rows, int cols, int rne
 // threads instead of one thread per row.
 constexpr int kQrPanelCols = 8;
 
-__global__ __launch_bounds__(256) void zqr_panel_gs_kernel(int rows, int cols, int rnew, const zc* __restrict__ A,
-                                                           zc* __restrict__ Q) {
+// Panel update P(:, c) -= sum_i q_i H(i, c) for the nb basis columns: a
+// thread takes one row and CPT panel columns (per = kQrPanelCols / CPT
+// threads share a row), so each q_i(r) it loads serves all of them.  Columns
+// at or beyond jp read column jp - 1 instead and are not stored: the loads
+// carry no condition (a conditional LDS load becomes a branch with a full
+// wait, which serialises the loop).
+template <int CPT>
+__device__ __forceinline__ void zqr_panel_update(zc* L, const zc* H, int rows, int nb, int j0, int jp) {
+  constexpr int per = kQrPanelCols / CPT;
+  for (int e = threadIdx.x; e < rows * per; e += blockDim.x) {
+    const int r = e % rows, c0 = e / rows;
+    int hc[CPT];
+    zc acc[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      hc[k] = 64 * min(c0 + per * k, jp - 1);
+      acc[k] = make_hipDoubleComplex(0.0, 0.0);
+    }
+    for (int i = 0; i < nb; ++i) {
+      const zc q = L[(size_t)i * rows + r];
+      zc h[CPT];
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) h[k] = H[i + hc[k]];
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) acc[k] = zadd(acc[k], zmul(h[k], q));
+    }
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      const int c = c0 + per * k;
+      if (c < jp) {
+        zc* p = L + (size_t)(j0 + c) * rows + r;
+        *p = zsub(*p, acc[k]);
+      }
+    }
+  }
+}
+
+// One work-group per launch, so the register budget is that of one wave per
+// SIMD (amdgpu_waves_per_eu(1, 1)): the compiler then keeps a loop's LDS
+// loads in flight together instead of reusing one register per load.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void zqr_panel_gs_kernel(
+    int rows, int cols, int rnew, const zc* __restrict__ A, zc* __restrict__ Q) {
   extern __shared__ zc L[];   // column c at L + c * rows; the basis is compacted into the leading columns
   __shared__ zc H[64 * kQrPanelCols];   // H(i, c) at i + 64 c
   __shared__ double n0[kQrPanelCols];
@@ -252,13 +292,13 @@ __global__ __launch_bounds__(256) void zqr_panel_gs_kernel(int rows, int cols, i
     double s[kQrPanelCols];
 #pragma unroll
     for (int c = 0; c < kQrPanelCols; ++c) s[c] = 0.0;
+    // columns at or beyond jp repeat column jp - 1 (unconditional loads)
     for (int r = tid; r < rows; r += nt)
 #pragma unroll
-      for (int c = 0; c < kQrPanelCols; ++c)
-        if (c < jp) {
-          const zc v = L[(size_t)(j0 + c) * rows + r];
-          s[c] += v.x * v.x + v.y * v.y;
-        }
+      for (int c = 0; c < kQrPanelCols; ++c) {
+        const zc v = L[(size_t)(j0 + min(c, jp - 1)) * rows + r];
+        s[c] += v.x * v.x + v.y * v.y;
+      }
 #pragma unroll
     for (int c = 0; c < kQrPanelCols; ++c) {
       const double t = wave_sum64(s[c]);
@@ -287,15 +327,18 @@ __global__ __launch_bounds__(256) void zqr_panel_gs_kernel(int rows, int cols, i
           double re[kQrPanelCols], im[kQrPanelCols];
 #pragma unroll
           for (int c = 0; c < kQrPanelCols; ++c) re[c] = im[c] = 0.0;
+          // columns at or beyond jp repeat column jp - 1 (unconditional
+          // loads; their sums are not stored)
           for (int r = l16; r < rows; r += 16) {
             const zc q = qi[r];
+            zc v[kQrPanelCols];
 #pragma unroll
-            for (int c = 0; c < kQrPanelCols; ++c)
-              if (c < jp) {
-                const zc v = L[(size_t)(j0 + c) * rows + r];
-                re[c] += q.x * v.x + q.y * v.y;
-                im[c] += q.x * v.y - q.y * v.x;
-              }
+            for (int c = 0; c < kQrPanelCols; ++c) v[c] = L[(size_t)(j0 + min(c, jp - 1)) * rows + r];
+#pragma unroll
+            for (int c = 0; c < kQrPanelCols; ++c) {
+              re[c] += q.x * v[c].x + q.y * v[c].y;
+              im[c] += q.x * v[c].y - q.y * v[c].x;
+            }
           }
 #pragma unroll
           for (int c = 0; c < kQrPanelCols; ++c) {
@@ -313,32 +356,14 @@ __global__ __launch_bounds__(256) void zqr_panel_gs_kernel(int rows, int cols, i
         }
       }
       __syncthreads();
-      // p_c(r) -= sum_i q_i(r) H(i, c): a thread takes one row and every
-      // (nt / rows)-th panel column, so each q_i(r) it loads serves them all
+      // p_c(r) -= sum_i q_i(r) H(i, c); nt / rows threads share a row (at
+      // most one per panel column)
       {
-        const int per = max(1, nt / rows);   // threads sharing a row
-        for (int e = tid; e < rows * per; e += nt) {
-          const int r = e % rows, c0 = e / rows;
-          zc acc[kQrPanelCols];
-#pragma unroll
-          for (int k = 0; k < kQrPanelCols; ++k) acc[k] = make_hipDoubleComplex(0.0, 0.0);
-          for (int i = 0; i < nb; ++i) {
-            const zc q = L[(size_t)i * rows + r];
-#pragma unroll
-            for (int k = 0; k < kQrPanelCols; ++k) {
-              const int c = c0 + per * k;
-              if (c < jp) acc[k] = zadd(acc[k], zmul(H[i + 64 * c], q));
-            }
-          }
-#pragma unroll
-          for (int k = 0; k < kQrPanelCols; ++k) {
-            const int c = c0 + per * k;
-            if (c < jp) {
-              zc* p = L + (size_t)(j0 + c) * rows + r;
-              *p = zsub(*p, acc[k]);
-            }
-          }
-        }
+        const int per = nt / rows;
+        if (per >= 8) zqr_panel_update<1>(L, H, rows, nb, j0, jp);
+        else if (per >= 4) zqr_panel_update<2>(L, H, rows, nb, j0, jp);
+        else if (per >= 2) zqr_panel_update<4>(L, H, rows, nb, j0, jp);
+        else zqr_panel_update<8>(L, H, rows, nb, j0, jp);
       }
       __syncthreads();
     }

@@ -349,8 +349,10 @@ __device__ inline void block_max_into(double v, unsigned long long* m) {
 // LDS columns when it exceeds tol.  The 16 lanes of one DPP row (one row
 // segment each) share a pair and reduce the dot products in registers; with
 // NR > 0 each thread keeps its NR elements per column in registers between
-// the dot products and the update (mw <= 16 NR).
-template <int NR, int LPP = 16>
+// the dot products and the update (mw <= 16 NR).  FULL (mw == LPP NR: every
+// lane's NR rows exist) drops the per-row tests, which otherwise become a
+// branch around every load and store.
+template <int NR, int LPP = 16, bool FULL = false>
 __device__ inline double hestenes_pair(double* ap, double* aq, int mw, int l16, double small2, double tol,
                                        double* c_out, double* s_out) {
   static_assert(LPP == 16 || LPP == 32, "lanes per pair: 16 or 32");
@@ -360,8 +362,8 @@ __device__ inline double hestenes_pair(double* ap, double* aq, int mw, int l16, 
 #pragma unroll
     for (int t = 0; t < NR; ++t) {
       const int i = l16 + LPP * t;
-      xr[t] = (i < mw) ? ap[i] : 0.0;
-      yr[t] = (i < mw) ? aq[i] : 0.0;
+      xr[t] = (FULL || i < mw) ? ap[i] : 0.0;
+      yr[t] = (FULL || i < mw) ? aq[i] : 0.0;
     }
 #pragma unroll
     for (int t = 0; t < NR; ++t) {
@@ -399,7 +401,7 @@ __device__ inline double hestenes_pair(double* ap, double* aq, int mw, int l16, 
 #pragma unroll
     for (int t2 = 0; t2 < NR; ++t2) {
       const int i = l16 + LPP * t2;
-      if (i < mw) {
+      if (FULL || i < mw) {
         ap[i] = c * xr[t2] - s * yr[t2];
         aq[i] = s * xr[t2] + c * yr[t2];
       }
@@ -416,7 +418,7 @@ __device__ inline double hestenes_pair(double* ap, double* aq, int mw, int l16, 
   return rel;
 }
 
-template <int NR, int LPP = 16>
+template <int NR, int LPP = 16, bool FULL = false>
 __global__ __launch_bounds__(16 * LPP) void svd_pair_kernel(int mw, int nw, int bsz, int nb, int round,
                                                        double* __restrict__ W, double* __restrict__ V,
                                                        double small2, double tol, int inner_sweeps,
@@ -486,7 +488,7 @@ __global__ __launch_bounds__(16 * LPP) void svd_pair_kernel(int mw, int nw, int 
           jacobi_pair(r, g, nn, p, q);   // circle_pair's pairs, without the divisions
           if (q < n) {
             double c, s;
-            const double rel = hestenes_pair<NR, LPP>(lds + p * ldp, lds + q * ldp, mw, l16, small2, tol, &c, &s);
+            const double rel = hestenes_pair<NR, LPP, FULL>(lds + p * ldp, lds + q * ldp, mw, l16, small2, tol, &c, &s);
             seen = fmax(seen, rel);
             if (s != 0.0) {
               if (accum) {   // the rotations are only needed to update V
@@ -925,12 +927,16 @@ JacobiResult jacobi_columns(int mw, int nw, double* X, double* V, hipStream_t st
   int threads = 256;
   if (nr <= 16 && svd_pair_lanes() == 32) {
     kern = nr <= 4 ? svd_pair_kernel<2, 32> : nr <= 8 ? svd_pair_kernel<4, 32> : svd_pair_kernel<8, 32>;
+    if (mw == 64) kern = svd_pair_kernel<2, 32, true>;
+    else if (mw == 128) kern = svd_pair_kernel<4, 32, true>;
+    else if (mw == 256) kern = svd_pair_kernel<8, 32, true>;
     threads = 512;
   }
   static std::atomic<bool> attr_set{false};   // concurrent first calls only repeat the call
   if (!attr_set) {
     for (auto f : {svd_pair_kernel<4>, svd_pair_kernel<8>, svd_pair_kernel<16>, svd_pair_kernel<32>,
-                   svd_pair_kernel<0>, svd_pair_kernel<2, 32>, svd_pair_kernel<4, 32>, svd_pair_kernel<8, 32>})
+                   svd_pair_kernel<0>, svd_pair_kernel<2, 32>, svd_pair_kernel<4, 32>, svd_pair_kernel<8, 32>,
+                   svd_pair_kernel<2, 32, true>, svd_pair_kernel<4, 32, true>, svd_pair_kernel<8, 32, true>})
       TTF_HIP(hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)(sizeof(double) * kSvdLdsDoubles)));
     attr_set = true;
