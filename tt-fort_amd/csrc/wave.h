@@ -13,11 +13,15 @@ constexpr int kDppXor2 = 0x4E;         // quad_perm [2, 3, 0, 1]
 constexpr int kDppRowMirror = 0x140;   // lane i <- lane 15 - i within the row
 constexpr int kDppRowHalfMirror = 0x141;   // lane i <- lane 7 - i within each half row
 
+// x of the lane that CTRL selects within the caller's row.  Every control
+// above reads a lane of the same row and all rows and banks are enabled, so
+// no lane falls back to the bound-control zero; the move carries no old
+// value, which spares a register copy per 32-bit half before each move.
 template <int CTRL>
 __device__ __forceinline__ double dpp_f64(double x) {
   const long long v = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp((int)(v >> 32), (int)(v >> 32), CTRL, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(v >> 32), CTRL, 0xF, 0xF, true);
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
