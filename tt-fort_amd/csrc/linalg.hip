@@ -924,6 +924,9 @@ JacobiResult jacobi_columns(int mw, int nw, double* X, double* V, hipStream_t st
   const int nr = (int)ceil_div(mw, 16);
   auto kern = nr <= 4 ? svd_pair_kernel<4> : nr <= 8 ? svd_pair_kernel<8>
             : nr <= 16 ? svd_pair_kernel<16> : nr <= 32 ? svd_pair_kernel<32> : svd_pair_kernel<0>;
+  if (mw == 64) kern = svd_pair_kernel<4, 16, true>;
+  else if (mw == 128) kern = svd_pair_kernel<8, 16, true>;
+  else if (mw == 256) kern = svd_pair_kernel<16, 16, true>;
   int threads = 256;
   if (nr <= 16 && svd_pair_lanes() == 32) {
     kern = nr <= 4 ? svd_pair_kernel<2, 32> : nr <= 8 ? svd_pair_kernel<4, 32> : svd_pair_kernel<8, 32>;
@@ -936,7 +939,8 @@ JacobiResult jacobi_columns(int mw, int nw, double* X, double* V, hipStream_t st
   if (!attr_set) {
     for (auto f : {svd_pair_kernel<4>, svd_pair_kernel<8>, svd_pair_kernel<16>, svd_pair_kernel<32>,
                    svd_pair_kernel<0>, svd_pair_kernel<2, 32>, svd_pair_kernel<4, 32>, svd_pair_kernel<8, 32>,
-                   svd_pair_kernel<2, 32, true>, svd_pair_kernel<4, 32, true>, svd_pair_kernel<8, 32, true>})
+                   svd_pair_kernel<2, 32, true>, svd_pair_kernel<4, 32, true>, svd_pair_kernel<8, 32, true>,
+                   svd_pair_kernel<4, 16, true>, svd_pair_kernel<8, 16, true>, svd_pair_kernel<16, 16, true>})
       TTF_HIP(hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)(sizeof(double) * kSvdLdsDoubles)));
     attr_set = true;
