@@ -576,14 +576,32 @@ __global__ __launch_bounds__(256) void qr_panel_kernel(int rows, int pc, double*
     }
     __syncthreads();
     if (t != 0.0) {
+      // rows four strides at a time (four partial sums), so that a lane's
+      // loads are in flight together instead of one round trip per row
       for (int c = j + 1 + g; c < pc; c += 16) {
         double* y = P + c * lp;
-        double w = (l16 == 0) ? y[j] : 0.0;
-        for (int i = j + 1 + l16; i < rows; i += 16) w += x[i] * y[i];
-        w = row_sum16(w);
+        double w0 = (l16 == 0) ? y[j] : 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0;
+        int i = j + 1 + l16;
+        for (; i + 48 < rows; i += 64) {
+          w0 += x[i] * y[i];
+          w1 += x[i + 16] * y[i + 16];
+          w2 += x[i + 32] * y[i + 32];
+          w3 += x[i + 48] * y[i + 48];
+        }
+        for (; i < rows; i += 16) w0 += x[i] * y[i];
+        double w = row_sum16((w0 + w1) + (w2 + w3));
         w *= t;
         if (l16 == 0) y[j] -= w;
-        for (int i = j + 1 + l16; i < rows; i += 16) y[i] -= w * x[i];
+        i = j + 1 + l16;
+        for (; i + 48 < rows; i += 64) {
+          const double x0 = x[i], x1 = x[i + 16], x2 = x[i + 32], x3 = x[i + 48];
+          const double y0 = y[i], y1 = y[i + 16], y2 = y[i + 32], y3 = y[i + 48];
+          y[i] = y0 - w * x0;
+          y[i + 16] = y1 - w * x1;
+          y[i + 32] = y2 - w * x2;
+          y[i + 48] = y3 - w * x3;
+        }
+        for (; i < rows; i += 16) y[i] -= w * x[i];
       }
     }
     __syncthreads();
