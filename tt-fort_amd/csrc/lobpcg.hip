@@ -289,18 +289,27 @@ __global__ __launch_bounds__(256) void rr_kernel(int m, int bs, int nx, const do
   // modified Gram-Schmidt on the columns of Z (twice) by the first wave,
   // lane r owning row r (m <= 64): dot products are wave sums in registers
   // (DPP rows, then lane reads) and each lane touches only its own row, so
-  // the wave needs no barrier.
+  // the wave needs no barrier.  Lanes past m use row m, a spare row (m < MR)
+  // that nothing else reads, and hold zeros: the loads and stores carry no
+  // per-lane condition (which would make each a branch with a full wait),
+  // and the column being orthogonalised stays in a register.
   if (tid < 64) {
+    const bool live = tid < m;
+    const int row = live ? tid : m;
     for (int pass = 0; pass < 2; ++pass) {
       for (int c = 0; c < bs; ++c) {
+        const double zc0 = Z[row][c];
+        double zc = live ? zc0 : 0.0;
         for (int j = 0; j < c; ++j) {
-          const double s = wave_sum64((tid < m) ? Z[tid][j] * Z[tid][c] : 0.0);
-          if (tid < m) Z[tid][c] -= s * Z[tid][j];
+          const double zj0 = Z[row][j];
+          const double zj = live ? zj0 : 0.0;
+          const double s = wave_sum64(zj * zc);
+          zc -= s * zj;
         }
-        const double s = wave_sum64((tid < m) ? Z[tid][c] * Z[tid][c] : 0.0);
+        const double s = wave_sum64(zc * zc);
         double nrm = sqrt(s);
         double inv = nrm > 1e-10 ? 1.0 / nrm : 0.0;
-        if (tid < m) Z[tid][c] *= inv;
+        Z[row][c] = zc * inv;
         if (tid == 0 && pass == 1) maskP[c] = nrm > 1e-10 ? 1 : 0;
       }
     }
