@@ -147,7 +147,7 @@ static void load_tune_table() {
     if (got == 9) c.bn = 0;
     const bool valid = (got == 9 || got == 10) && (c.bm == 32 || c.bm == 64) &&
                        (c.bk == 16 || c.bk == 32 || c.bk == 64) && c.ks >= 1 && c.ks <= 16 &&
-                       (c.bn == 0 || (c.bn == 80 && c.bm == 64 && c.bk <= 32));
+                       (c.bn == 0 || (c.bn == 80 && c.bm == 64 && c.bk <= 32) || (c.bn == 16 && c.bm == 32));
     if (valid) {
       c.ks = splitk_rule(k.M, k.N, k.K, k.nbatch);   // the split is never tuned
       g_tuned[k] = c;
@@ -198,7 +198,13 @@ static Cfg heuristic(int M, int N, int K, int nbatch) {
 
 template <typename T, bool CT>
 static void launch_tiles(const GemmOp<T>& op, const Cfg& c, dim3 grid, hipStream_t st) {
-  if (cfg_bn(c) == 80 && c.bk == 16)
+  if (cfg_bn(c) == 16 && c.bk == 16)
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, 32, 16, 16, 1, CT>), grid, dim3(64), 0, st, op);
+  else if (cfg_bn(c) == 16 && c.bk == 32)
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, 32, 16, 32, 1, CT>), grid, dim3(64), 0, st, op);
+  else if (cfg_bn(c) == 16)
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, 32, 16, 64, 1, CT>), grid, dim3(64), 0, st, op);
+  else if (cfg_bn(c) == 80 && c.bk == 16)
     hipLaunchKernelGGL((gemm_mfma_kernel<T, 64, 80, 16, 4, CT, 4>), grid, dim3(256), 0, st, op);
   else if (cfg_bn(c) == 80)
     hipLaunchKernelGGL((gemm_mfma_kernel<T, 64, 80, 32, 4, CT, 4>), grid, dim3(256), 0, st, op);
@@ -248,13 +254,16 @@ static T* splitk_work(const GemmOp<T>& op, int ks) {
 template <typename T>
 static Cfg tune(const GemmOp<T>& op0, hipStream_t st) {
   std::vector<Cfg> cands;
-  // square tiles, and 64 x 80 tiles (four waves of 16 x 80) when N takes
-  // them evenly: N = 640 (a bond of 128 x MPO rank 5) splits into 8 such
-  // columns, so that a batch of 8 fills the 256 CUs once
-  for (int bn : {0, 80})
+  // square tiles, 64 x 80 tiles (four waves of 16 x 80) when N takes them
+  // evenly: N = 640 (a bond of 128 x MPO rank 5) splits into 8 such columns,
+  // so that a batch of 8 fills the 256 CUs once; and one-wave 32 x 16 tiles
+  // for N <= 16 (a block of search directions), which waste half as many
+  // matrix-core products on padding columns as 32 x 32
+  for (int bn : {0, 80, 16})
     for (int bm : {64, 32})
       for (int bk : {16, 32, 64}) {
         if (bn == 80 && (bm != 64 || bk > 32 || op0.N % 80 != 0)) continue;
+        if (bn == 16 && (bm != 32 || op0.N > 16)) continue;
         Cfg c{bm, bk, splitk_rule(op0.M, op0.N, op0.K, op0.nbatch)};
         c.bn = bn;
         cands.push_back(c);
