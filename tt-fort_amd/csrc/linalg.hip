@@ -304,12 +304,25 @@ __device__ inline void lds_gram(const double* P, int rows, int ld, int n, double
 // Column block copies between global memory (column stride gld, columns
 // cols[0..n)) and LDS (column stride ld), column by column with each
 // thread's rows: no division in the address arithmetic, and each thread
-// issues the loads of 16 columns before storing any.
+// issues the accesses of 16 columns together.  When the rows fit in half the
+// block, the two halves take alternate groups of 16 columns, so that a pair
+// of 32 columns of up to blockDim / 2 rows moves in one round trip.
+__device__ inline void cols_copy_split(int rows, int& t, int& step, int& c_first, int& c_step) {
+  const int nt = blockDim.x;
+  const bool split = 2 * rows <= nt;
+  step = split ? nt / 2 : nt;
+  const bool upper = split && (int)threadIdx.x >= step;
+  t = upper ? (int)threadIdx.x - step : (int)threadIdx.x;
+  c_first = upper ? 16 : 0;
+  c_step = split ? 32 : 16;
+}
+
 __device__ inline void cols_to_lds(const double* __restrict__ src, int64_t gld, const int* cols, int rows,
                                    int n, double* dst, int ld) {
-  const int nt = blockDim.x;
-  for (int c0 = 0; c0 < n; c0 += 16) {
-    for (int r = threadIdx.x; r < rows; r += nt) {
+  int t, step, c_first, c_step;
+  cols_copy_split(rows, t, step, c_first, c_step);
+  for (int c0 = c_first; c0 < n; c0 += c_step) {
+    for (int r = t; r < rows; r += step) {
       double v[16];
 #pragma unroll
       for (int u = 0; u < 16; ++u) v[u] = (c0 + u < n) ? src[(int64_t)r + gld * cols[c0 + u]] : 0.0;
@@ -322,9 +335,10 @@ __device__ inline void cols_to_lds(const double* __restrict__ src, int64_t gld, 
 
 __device__ inline void lds_to_cols(const double* src, int ld, int rows, int n, double* __restrict__ dst,
                                    int64_t gld, const int* cols) {
-  const int nt = blockDim.x;
-  for (int c0 = 0; c0 < n; c0 += 16) {
-    for (int r = threadIdx.x; r < rows; r += nt) {
+  int t, step, c_first, c_step;
+  cols_copy_split(rows, t, step, c_first, c_step);
+  for (int c0 = c_first; c0 < n; c0 += c_step) {
+    for (int r = t; r < rows; r += step) {
       double v[16];
 #pragma unroll
       for (int u = 0; u < 16; ++u) v[u] = (c0 + u < n) ? src[(c0 + u) * ld + r] : 0.0;
