@@ -688,7 +688,9 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
   // part B (projections, orthonormalisation, A W, Rayleigh-Ritz, next
   // basis set), with the host's convergence decision between them; part B
   // of set c is followed at once by part A of the next set.
-  auto partA = [&](int c) {
+  // With norms false the residual norms are not summed: only an iteration
+  // that reads them back (a sync point) needs them.
+  auto partA = [&](int c, bool norms = true) {
     double* S = Sset(c);
     double* X = S, *W = S + colsz * 2 * bs;
     double* AX = S + colsz * 3 * bs;
@@ -706,9 +708,11 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
       g.sCb0 = pnb;
       gemm_launch(g, st);
     }
-    hipLaunchKernelGGL(residual_norms, dim3((unsigned)ceil_div(16 * bs, 256)), dim3(256), 0, st, bs, res_grid,
-                       res_part, nrm);
-    TTF_HIP(hipGetLastError());
+    if (norms) {
+      hipLaunchKernelGGL(residual_norms, dim3((unsigned)ceil_div(16 * bs, 256)), dim3(256), 0, st, bs, res_grid,
+                         res_part, nrm);
+      TTF_HIP(hipGetLastError());
+    }
   };
   auto partB = [&](int c, bool aw32) {
     double* S = Sset(c);
@@ -848,7 +852,10 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
         TTF_HIP(hipGraphLaunch(gexec[cur], st));
       } else {
         partB(cur, aw32);
-        partA(1 - cur);
+        // the residual norms of the next iteration are summed only when it
+        // is a sync point (the same test as at the top of the loop)
+        const int next = it + 1;
+        partA(1 - cur, next < 3 || next % kSyncEvery == 0 || matvecs + 2 * bs > max_matvecs);
       }
       if (aw32) {
         stats.matvecs32 += bs;
