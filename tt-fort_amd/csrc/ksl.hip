@@ -488,17 +488,35 @@ __global__ __launch_bounds__(256) void zexpm_kernel(int m, double t, const zc* _
   __syncthreads();
   const int ns = ns_s;
   const double scale = t / ldexp(1.0, ns);
-  // C = alpha * A * B (all m x m in LDS)
+  // C = alpha * A * B (all m x m in LDS); the inner products take four k
+  // at a time into two partial sums, so that their loads are in flight
+  // together
   auto matmul = [&](const zc* A, const zc* B, zc* C, double alpha) {
     for (int e = tid; e < mm; e += nt) {
       const int i = e % m, j = e / m;
-      double re = 0.0, im = 0.0;
-      for (int k = 0; k < m; ++k) {
-        const zc a = A[i + m * k], b = B[k + m * j];
-        re += a.x * b.x - a.y * b.y;
-        im += a.x * b.y + a.y * b.x;
+      double re0 = 0.0, im0 = 0.0, re1 = 0.0, im1 = 0.0;
+      int k = 0;
+      for (; k + 4 <= m; k += 4) {
+        zc a[4], b[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          a[u] = A[i + m * (k + u)];
+          b[u] = B[k + u + m * j];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u += 2) {
+          re0 += a[u].x * b[u].x - a[u].y * b[u].y;
+          im0 += a[u].x * b[u].y + a[u].y * b[u].x;
+          re1 += a[u + 1].x * b[u + 1].x - a[u + 1].y * b[u + 1].y;
+          im1 += a[u + 1].x * b[u + 1].y + a[u + 1].y * b[u + 1].x;
+        }
       }
-      C[e] = make_hipDoubleComplex(alpha * re, alpha * im);
+      for (; k < m; ++k) {
+        const zc a = A[i + m * k], b = B[k + m * j];
+        re0 += a.x * b.x - a.y * b.y;
+        im0 += a.x * b.y + a.y * b.x;
+      }
+      C[e] = make_hipDoubleComplex(alpha * (re0 + re1), alpha * (im0 + im1));
     }
     __syncthreads();
   };
@@ -528,14 +546,22 @@ __global__ __launch_bounds__(256) void zexpm_kernel(int m, double t, const zc* _
   for (int e = tid; e < mm; e += nt) q[e] = zsub(q[e], p[e]);
   __syncthreads();
   for (int k = 0; k < m; ++k) {
-    if (tid == 0) {
-      int pv = k;
-      double best = hypot(q[k + m * k].x, q[k + m * k].y);
-      for (int i = k + 1; i < m; ++i) {
-        const double v = hypot(q[i + m * k].x, q[i + m * k].y);
-        if (v > best) { best = v; pv = i; }
+    if (tid < 64) {
+      // the first row of largest |q(i, k)| (i >= k) by a wave reduction of
+      // keys |q|^2 (non-negative, so their bits order as integers) with the
+      // low six bits replaced by 63 - i (the earlier row wins ties)
+      const int i = tid;
+      const zc v = q[(i < m ? i : k) + m * k];
+      const double a2 = v.x * v.x + v.y * v.y;
+      unsigned long long key = (i >= k && i < m)
+                                   ? (((unsigned long long)__double_as_longlong(a2)) & ~63ull) | (unsigned)(63 - i)
+                                   : 0ull;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long other = __shfl_xor(key, o);
+        key = other > key ? other : key;
       }
-      piv_s = pv;
+      if (tid == 0) piv_s = 63 - (int)(key & 63ull);
     }
     __syncthreads();
     const int pv = piv_s;
@@ -570,9 +596,16 @@ __global__ __launch_bounds__(256) void zexpm_kernel(int m, double t, const zc* _
     const double dn = d.x * d.x + d.y * d.y;
     const zc dinv = make_hipDoubleComplex(d.x / dn, -d.y / dn);
     for (int j = tid; j < m; j += nt) {
-      zc s = p[k + m * j];
-      for (int l = k + 1; l < m; ++l) s = zsub(s, zmul(q[k + m * l], p[l + m * j]));
-      p[k + m * j] = zmul(s, dinv);
+      // two partial sums, two l per step each, loads together
+      zc s0 = p[k + m * j], s1 = make_hipDoubleComplex(0.0, 0.0);
+      int l = k + 1;
+      for (; l + 2 <= m; l += 2) {
+        const zc q0 = q[k + m * l], p0 = p[l + m * j], q1 = q[k + m * (l + 1)], p1 = p[l + 1 + m * j];
+        s0 = zsub(s0, zmul(q0, p0));
+        s1 = zsub(s1, zmul(q1, p1));
+      }
+      if (l < m) s0 = zsub(s0, zmul(q[k + m * l], p[l + m * j]));
+      p[k + m * j] = zmul(zadd(s0, s1), dinv);
     }
     __syncthreads();
   }
