@@ -345,30 +345,33 @@ __global__ __launch_bounds__(256) void zgemm_mfma_kernel(ZGemmOpT<R> op) {
     int idx = tid + q * 256;
     if (b_jfast) { bj[q] = idx % BN; bk[q] = idx / BN; } else { bk[q] = idx % BK; bj[q] = idx / BK; }
   }
+  // Every element is loaded from a clamped in-range address and zeroed by a
+  // select afterwards: a load under a per-lane condition whose value is used
+  // inside the condition compiles to a branch with a full wait, which would
+  // serialise the tile's loads.
   auto load = [&](int k0) {
+    C_t va[LA], vb[LB];
 #pragma unroll
     for (int q = 0; q < LA; ++q) {
-      int gi = tm * BM + ai[q], gk = k0 + ak[q];
-      if (gi < op.M && gk < kend) {
-        C_t v = A[gi * op.sAi + gk * op.sAk];
-        ra[q].x = v.x;
-        ra[q].y = sa * v.y;
-      } else {
-        ra[q].x = R(0);
-        ra[q].y = R(0);
-      }
+      const int gi = min(tm * BM + ai[q], op.M - 1), gk = min(k0 + ak[q], kend - 1);
+      va[q] = A[gi * op.sAi + gk * op.sAk];
     }
 #pragma unroll
     for (int q = 0; q < LB; ++q) {
-      int gj = tn * BN + bj[q], gk = k0 + bk[q];
-      if (gj < op.N && gk < kend) {
-        C_t v = B[gk * op.sBk + gj * op.sBj];
-        rb[q].x = v.x;
-        rb[q].y = sb * v.y;
-      } else {
-        rb[q].x = R(0);
-        rb[q].y = R(0);
-      }
+      const int gj = min(tn * BN + bj[q], op.N - 1), gk = min(k0 + bk[q], kend - 1);
+      vb[q] = B[gk * op.sBk + gj * op.sBj];
+    }
+#pragma unroll
+    for (int q = 0; q < LA; ++q) {
+      const bool ok = tm * BM + ai[q] < op.M && k0 + ak[q] < kend;
+      ra[q].x = ok ? va[q].x : R(0);
+      ra[q].y = ok ? sa * va[q].y : R(0);
+    }
+#pragma unroll
+    for (int q = 0; q < LB; ++q) {
+      const bool ok = tn * BN + bj[q] < op.N && k0 + bk[q] < kend;
+      rb[q].x = ok ? vb[q].x : R(0);
+      rb[q].y = ok ? sb * vb[q].y : R(0);
     }
   };
 
