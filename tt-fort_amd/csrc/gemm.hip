@@ -64,13 +64,20 @@ __global__ __launch_bounds__(256) void panel_kernel(GemmOp<T> op) {
 #pragma unroll
         for (int j = 0; j < NN; ++j) acc[j] += av[u] * Bs[k0 + u][j];
     }
+    // with beta != 0 every C element of the row is loaded before any is
+    // used (one load under its own condition, used inside it, becomes a
+    // branch with a full wait per column); columns past N read column 0
+    T cv[NN];
+    if (op.beta != T(0)) {
+#pragma unroll
+      for (int j = 0; j < NN; ++j) cv[j] = c[(j < N ? j : 0) * op.sCj];
+    }
 #pragma unroll
     for (int j = 0; j < NN; ++j)
       if (j < N) {
-        T* cj = c + j * op.sCj;
         T v = op.alpha * acc[j];
-        if (op.beta != T(0)) v += op.beta * (*cj);
-        *cj = v;
+        if (op.beta != T(0)) v += op.beta * cv[j];
+        c[j * op.sCj] = v;
       }
   }
 }

@@ -191,6 +191,24 @@ __global__ __launch_bounds__(64 * NW) void gemm_mfma_kernel(GemmOp<T> op) {
   // of each 16x16 block (CT: row l%16, column mfma_row<T>(l, r)).
   if (op.ksplit == 1) {
     T* __restrict__ C = op.C + b0 * op.sCb0 + b1 * op.sCb1;
+    // with beta != 0 the tile's C elements are all loaded (from clamped
+    // in-range addresses) before any is used, so the loads stay in flight
+    // together instead of one load and wait per element
+    T cv[FM][FN][4];
+    if (op.beta != T(0)) {
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int fr = CT ? (lane & 15) : mfma_row<T>(lane, r);
+            const int fc = CT ? mfma_row<T>(lane, r) : (lane & 15);
+            const int gi = min(tm * BM + wm * WM + i * 16 + fr, op.M - 1);
+            const int gj = min(tn * BN + wn * WN + j * 16 + fc, op.N - 1);
+            cv[i][j][r] = C[gi * op.sCi + gj * op.sCj];
+          }
+    }
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -202,10 +220,9 @@ __global__ __launch_bounds__(64 * NW) void gemm_mfma_kernel(GemmOp<T> op) {
           int gi = tm * BM + wm * WM + i * 16 + fr;
           int gj = tn * BN + wn * WN + j * 16 + fc;
           if (gi < op.M && gj < op.N) {
-            T* c = C + gi * op.sCi + gj * op.sCj;
             T v = op.alpha * acc[i][j][r];
-            if (op.beta != T(0)) v += op.beta * (*c);
-            *c = v;
+            if (op.beta != T(0)) v += op.beta * cv[i][j][r];
+            C[gi * op.sCi + gj * op.sCj] = v;
           }
         }
   } else {
@@ -430,6 +447,21 @@ __global__ __launch_bounds__(256) void zgemm_mfma_kernel(ZGemmOpT<R> op) {
   }
   C_t* __restrict__ C = op.C + b0 * op.sCb0 + b1 * op.sCb1;
   const bool beta0 = (op.beta.x == R(0) && op.beta.y == R(0));
+  // with beta != 0 the tile's C elements are all loaded (from clamped
+  // in-range addresses) before any is used
+  C_t cv[FM][FN][4];
+  if (!beta0) {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gi = min(tm * BM + wm * WM + i * 16 + mfma_row<R>(lane, r), op.M - 1);
+          const int gj = min(tn * BN + wn * WN + j * 16 + (lane & 15), op.N - 1);
+          cv[i][j][r] = C[gi * op.sCi + gj * op.sCj];
+        }
+  }
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -439,16 +471,15 @@ __global__ __launch_bounds__(256) void zgemm_mfma_kernel(ZGemmOpT<R> op) {
         int gi = tm * BM + wm * WM + i * 16 + mfma_row<R>(lane, r);
         int gj = tn * BN + wn * WN + j * 16 + (lane & 15);
         if (gi < op.M && gj < op.N) {
-          C_t* c = C + gi * op.sCi + gj * op.sCj;
           R xr = cr[i][j][r], xi = ci[i][j][r];
           R vr = op.alpha.x * xr - op.alpha.y * xi;
           R vi = op.alpha.x * xi + op.alpha.y * xr;
           if (!beta0) {
-            C_t o = *c;
+            const C_t o = cv[i][j][r];
             vr += op.beta.x * o.x - op.beta.y * o.y;
             vi += op.beta.x * o.y + op.beta.y * o.x;
           }
-          *c = CplxT<R>::make(vr, vi);
+          C[gi * op.sCi + gj * op.sCj] = CplxT<R>::make(vr, vi);
         }
       }
 }
@@ -463,18 +494,25 @@ __global__ void zgemm_splitk_reduce(ZGemmOpT<R> op) {
     const int b = (int)(t / mn);
     const int64_t e = t - (int64_t)b * mn;
     const int gj = (int)(e / op.M), gi = (int)(e - (int64_t)gj * op.M);
-    R xr = R(0), xi = R(0);
-    for (int z = 0; z < op.ksplit; ++z) {
-      const C_t v = op.work[((int64_t)z * op.nbatch + b) * mn + e];
-      xr += v.x;
-      xi += v.y;
-    }
+    // all slices (at most 8) and the old C loaded before the sums, added in
+    // slice order
     const int b0 = b % op.nb0, b1 = b / op.nb0;
     C_t* c = op.C + b0 * op.sCb0 + b1 * op.sCb1 + gi * op.sCi + gj * op.sCj;
+    const bool beta0 = op.beta.x == R(0) && op.beta.y == R(0);
+    C_t part[8];
+#pragma unroll
+    for (int z = 0; z < 8; ++z) part[z] = op.work[((int64_t)(z < op.ksplit ? z : 0) * op.nbatch + b) * mn + e];
+    const C_t o = beta0 ? CplxT<R>::make(R(0), R(0)) : *c;
+    R xr = R(0), xi = R(0);
+#pragma unroll
+    for (int z = 0; z < 8; ++z)
+      if (z < op.ksplit) {
+        xr += part[z].x;
+        xi += part[z].y;
+      }
     R vr = op.alpha.x * xr - op.alpha.y * xi;
     R vi = op.alpha.x * xi + op.alpha.y * xr;
-    if (op.beta.x != R(0) || op.beta.y != R(0)) {
-      const C_t o = *c;
+    if (!beta0) {
       vr += op.beta.x * o.x - op.beta.y * o.y;
       vi += op.beta.x * o.y + op.beta.y * o.x;
     }
