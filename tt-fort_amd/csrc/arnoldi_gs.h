@@ -83,23 +83,44 @@ __global__ __launch_bounds__(kThreads) void gs_pass_kernel(int64_t N, int j, int
       const int col = tid & 63, grp = tid >> 6;
       T s;
       set_zero(s);
-      if (col < j)
-        for (int b = grp; b < nparts; b += 4) s = add(s, part_in[(int64_t)b * j + col]);
+      if (col < j) {
+        // eight partials in flight per step, added in the same order
+        int b = grp;
+        for (; b + 28 < nparts; b += 32) {
+          T pv[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) pv[u] = part_in[(int64_t)(b + 4 * u) * j + col];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) s = add(s, pv[u]);
+        }
+        for (; b < nparts; b += 4) s = add(s, part_in[(int64_t)b * j + col]);
+      }
       red[grp][col] = s;
     }
     __syncthreads();
+    // coefficients j .. j8 - 1 are zero, so that the update runs in whole
+    // steps of eight columns (their loads in flight together, no remainder
+    // loop); the padded columns read column j - 1 and add exact zeros
+    const int j8 = (j + 7) & ~7;
     if (tid < j) {
       const T h = add(add(red[0][tid], red[1][tid]), add(red[2][tid], red[3][tid]));
       hsh[tid] = h;
       if (blockIdx.x == 0) h_out[tid] = h;
+    } else if (tid < j8) {
+      set_zero(hsh[tid]);
     }
     __syncthreads();
     for (int r = tid; r < rows; r += kThreads) {
       const T* v = V + r0 + r;
       T s;
       set_zero(s);
-#pragma unroll 8
-      for (int i = 0; i < j; ++i) s = mul_acc(s, v[ldv * i], hsh[i]);
+      for (int i0 = 0; i0 < j8; i0 += 8) {
+        T vv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) vv[u] = v[ldv * min(i0 + u, j - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s = mul_acc(s, vv[u], hsh[i0 + u]);
+      }
       const T x = sub(u[r0 + r], s);
       u[r0 + r] = x;
       us[r] = x;

@@ -170,21 +170,39 @@ __global__ __launch_bounds__(256) void layer_kernel(int M, int K, int N, const f
     }
     __syncthreads();
   }
+  // the epilogue's operands for the 16 outputs (bias, and with SKIP the
+  // block diagonal and the residual), loaded from clamped in-range indices
+  // before any is used: loaded under each output's own condition they
+  // become a branch with a full wait apiece
+  float eb[16], ed[16], ex[16];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
+        const int q = i * 8 + j * 4 + r;
+        const int gn = min(n0 + wn * 32 + i * 16 + mfma_row<float>(lane, r), N - 1);
+        const int gm = min(m0 + wm * 32 + j * 16 + (lane & 15), M - 1);
+        eb[q] = bias[gn];
+        if (SKIP) {
+          ed[q] = X0[gn + (int64_t)kMlpIn * gm];
+          ex[q] = X0[3 * kMlpBlock + gn + (int64_t)kMlpIn * gm];
+        }
+      }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int q = i * 8 + j * 4 + r;
         const int gn = n0 + wn * 32 + i * 16 + mfma_row<float>(lane, r);
         const int gm = m0 + wm * 32 + j * 16 + (lane & 15);
         if (gn < N && gm < M) {
-          float val = acc[i][j][r] + bias[gn];
+          float val = acc[i][j][r] + eb[q];
           if (RELU) val = fmaxf(val, 0.f);
-          if (SKIP) {
-            const float dg = X0[gn + (int64_t)kMlpIn * gm];
-            if (dg != 0.f) val += X0[3 * kMlpBlock + gn + (int64_t)kMlpIn * gm] / dg;
-          }
+          if (SKIP) val += ed[q] != 0.f ? ex[q] / ed[q] : 0.f;
           Y[gn + (int64_t)N * gm] = val;
         }
       }
@@ -257,21 +275,39 @@ __global__ __launch_bounds__(256) void layer_bf16_kernel(int M, int N, const flo
 #pragma unroll
       for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
   }
+  // the epilogue's operands for the 16 outputs (bias, and with SKIP the
+  // block diagonal and the residual), loaded from clamped in-range indices
+  // before any is used: loaded under each output's own condition they
+  // become a branch with a full wait apiece
+  float eb[16], ed[16], ex[16];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
+        const int q = i * 8 + j * 4 + r;
+        const int gn = min(n0 + i * 16 + mfma_row<float>(lane, r), N - 1);
+        const int gm = min(m0 + j * 16 + c, M - 1);
+        eb[q] = bias[gn];
+        if (SKIP) {
+          ed[q] = X0[gn + (int64_t)kMlpIn * gm];
+          ex[q] = X0[3 * kMlpBlock + gn + (int64_t)kMlpIn * gm];
+        }
+      }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int q = i * 8 + j * 4 + r;
         const int gn = n0 + i * 16 + mfma_row<float>(lane, r);
         const int gm = m0 + j * 16 + c;
         if (gn < N && gm < M) {
-          float val = acc[i][j][r] + bias[gn];
+          float val = acc[i][j][r] + eb[q];
           if (RELU) val = fmaxf(val, 0.f);
-          if (SKIP) {
-            const float dg = X0[gn + (int64_t)kMlpIn * gm];
-            if (dg != 0.f) val += X0[3 * kMlpBlock + gn + (int64_t)kMlpIn * gm] / dg;
-          }
+          if (SKIP) val += ed[q] != 0.f ? ex[q] / ed[q] : 0.f;
           Y[gn + (int64_t)N * gm] = val;
         }
       }
