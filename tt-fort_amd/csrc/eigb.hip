@@ -209,8 +209,9 @@ extern "C" int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, con
         transpose(N, B, core[k], N, tmp, B, st);
         // right singular vectors V (nc x kk) into VT's storage
         int kpos = 0;
-        svd_vectors(mr, nc, tmp, mr, false, S, VT, nc, hs, &kpos, st);
+        const int sw = svd_vectors(mr, nc, tmp, mr, false, S, VT, nc, hs, &kpos, st);
         const int nn = std::min(my_chop3(hs, kk, eps2), rmax);
+        if (verb > 1) std::fprintf(stderr, "svd: %d x %d sweeps %d rank %d\n", mr, nc, sw, nn);
         complete_columns(nc, kpos, nn, VT, nc, st);
         // core k = V(:, 0:nn)^T; US = M V(:, 0:nn) (mr x nn)
         transpose(nc, nn, VT, nc, core[k], nn, st);
@@ -242,8 +243,9 @@ extern "C" int ttf_tt_eigb(int d, const int* n, const int* m, const int* ra, con
         const int kk = std::min(mr, nc);
         // left singular vectors U (mr x kk)
         int kpos = 0;
-        svd_vectors(mr, nc, core[k], mr, true, S, U, mr, hs, &kpos, st);
+        const int sw = svd_vectors(mr, nc, core[k], mr, true, S, U, mr, hs, &kpos, st);
         const int nn = std::min(my_chop3(hs, kk, eps2), rmax);
+        if (verb > 1) std::fprintf(stderr, "svd: %d x %d sweeps %d rank %d\n", mr, nc, sw, nn);
         complete_columns(mr, kpos, nn, U, mr, st);
         // SVT = U(:, 0:nn)^T M (nn x nc), layout (j, r_{k+1}, b)
         {

@@ -36,9 +36,10 @@ int gesvd(int m, int n, const double* A, int64_t lda, double* U, int64_t ldu, do
 // want_left, else right (n x k), as the columns of O (ldo).  The other
 // factor is the caller's product with A (O^T A or A O).  When the wanted
 // side is the shorter dimension the Jacobi iteration runs on R^T from a
-// Householder QR of the tall orientation (the preconditioning of
-// Drmac-Veselic), which keeps the sweep count low for graded spectra and
-// needs no accumulated rotations.  Vectors of zero singular values are zero
+// Householder QR of the tall orientation with its columns in decreasing norm
+// order (the preconditioning of Drmac-Veselic, the pivoting reduced to one
+// sort), which keeps the sweep count low for graded spectra and needs no
+// accumulated rotations.  Vectors of zero singular values are zero
 // columns; *kpos counts the nonzero ones.  Returns the Jacobi sweeps used.
 int svd_vectors(int m, int n, const double* A, int64_t lda, bool want_left, double* S, double* O,
                 int64_t ldo, std::vector<double>& hs, int* kpos, hipStream_t st);
