@@ -671,8 +671,13 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
   std::vector<double> hbuf(2 * kMaxM + 1);
   // the residual norms are read back (a host synchronisation) at the first
   // iterations and then every kSyncEvery: in between, iterations are queued
-  // without waiting, and at most kSyncEvery - 1 run past convergence
+  // without waiting.  When the residual's decrease since the last read-back
+  // predicts convergence sooner, the next read-back comes at the predicted
+  // iteration, so fewer iterations run past convergence (at most
+  // kSyncEvery - 1 either way).
   constexpr int kSyncEvery = 4;
+  int next_sync = 0, it_prev = -1;
+  double r_prev = 0.0;
   // the residual's partial sums share the Gram arena, which tsgram may grow
   // (and so move) between iterations: the pointer is fetched at each launch
   // the optional diagonal preconditioner (TTF_EIG_PREC = clamp delta > 0)
@@ -816,7 +821,7 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
     partA(cur);
     for (int it = 0;; ++it) {
       const bool at_cap = matvecs + bs > max_matvecs;
-      if (it < 3 || it % kSyncEvery == 0 || at_cap || force_sync) {
+      if (it < 3 || it >= next_sync || at_cap || force_sync) {
         force_sync = false;
         read_back(hbuf.data(), theta, sizeof(double) * (2 * kMaxM + 1), st);
         anorm = std::max(anorm, hbuf[2 * kMaxM]);   // amax follows nrm in the buffer
@@ -827,7 +832,8 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
         }
         stats.iters = it;
         stats.res = std::sqrt(res2);
-        const bool done = rmax <= tol * std::max(anorm, 1e-300) || at_cap;
+        const double target = tol * std::max(anorm, 1e-300);
+        const bool done = rmax <= target || at_cap;
         const double rswitch = kSwitch * std::max(anorm, 1e-300);
         if (mix && !fresh) {
           // decide on FP64 residuals: refresh at the tolerance or the cap,
@@ -841,6 +847,17 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
           }
         }
         if (done) break;
+        // next read-back: kSyncEvery iterations on, or where the linear
+        // rate since the last one reaches the tolerance, if sooner
+        int step = kSyncEvery;
+        if (it_prev >= 0 && it > it_prev && rmax < r_prev && rmax > 0.0) {
+          const double rate = std::log(rmax / r_prev) / (it - it_prev);   // < 0
+          const double need = std::log(target / rmax) / rate;                // > 0
+          if (need < kSyncEvery) step = std::max(1, (int)std::ceil(need));
+        }
+        r_prev = rmax;
+        it_prev = it;
+        next_sync = it + step;
         if (fresh) {
           r_ref = rmax;
           if (mix && rmax <= rswitch) fp64_finish = true;
@@ -855,7 +872,7 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
         // the residual norms of the next iteration are summed only when it
         // is a sync point (the same test as at the top of the loop)
         const int next = it + 1;
-        partA(1 - cur, next < 3 || next % kSyncEvery == 0 || matvecs + 2 * bs > max_matvecs);
+        partA(1 - cur, next < 3 || next >= next_sync || matvecs + 2 * bs > max_matvecs);
       }
       if (aw32) {
         stats.matvecs32 += bs;
