@@ -13,6 +13,9 @@ namespace ttf {
 // p, q <= 64.  Deterministic two-pass reduction over row chunks.
 void tsgram(int N, int p, int q, const double* X, int64_t ldx, const double* Y, int64_t ldy,
             double* G, int ldg, hipStream_t st);
+// The same for X = [X1 X2] (p1 + p2 columns from two matrices).
+void tsgram2(int N, int p1, const double* X1, int64_t ldx1, int p2, const double* X2, int64_t ldx2, int q,
+             const double* Y, int64_t ldy, double* G, int ldg, hipStream_t st);
 
 // Eigen-decomposition of `count` symmetric n x n matrices (n <= 64), stored
 // consecutively with leading dimension n at A (overwritten).  Eigenvalues go

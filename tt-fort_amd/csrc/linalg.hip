@@ -24,9 +24,11 @@ constexpr int kTsgRows = 64;   // rows per chunk: one per lane of a wave
 // block, and tsgram_reduce adds the partials in a fixed order (so the
 // result is reproducible): the 16 lanes of a DPP row take interleaved
 // partials of one output, one batch of loads each, and a row sum combines
-// them.
+// them.  X's columns may come from two matrices: columns [0, p1) from X,
+// the rest from X2 (ld ldx2).
 __global__ __launch_bounds__(256) void tsgram_partial(int N, int p, int q, const double* __restrict__ X,
-                                                      int64_t ldx, const double* __restrict__ Y,
+                                                      int64_t ldx, int p1, const double* __restrict__ X2,
+                                                      int64_t ldx2, const double* __restrict__ Y,
                                                       int64_t ldy, double* __restrict__ part) {
   // chunk rows staged in LDS (columns past p / q as zeros), partial products
   // on the matrix cores: the output's 16x16 tiles (at most 16) go to the
@@ -50,7 +52,8 @@ __global__ __launch_bounds__(256) void tsgram_partial(int N, int p, int q, const
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
       const int i = i0 + 4 * u;
-      xv[u] = (ok && i < p) ? X[(int64_t)(r0 + r) + ldx * i] : 0.0;
+      const double* xc = i < p1 ? X + ldx * i : X2 + ldx2 * (i - p1);
+      xv[u] = (ok && i < p) ? xc[r0 + r] : 0.0;
       yv[u] = (ok && i < q) ? Y[(int64_t)(r0 + r) + ldy * i] : 0.0;
     }
     __syncthreads();
@@ -107,17 +110,25 @@ __global__ __launch_bounds__(256) void tsgram_reduce(int p, int q, int nparts, c
   if (k == 0 && e < pq) G[(e % p) + (int64_t)ldg * (e / p)] = s;
 }
 
-void tsgram(int N, int p, int q, const double* X, int64_t ldx, const double* Y, int64_t ldy,
-            double* G, int ldg, hipStream_t st) {
-  TTF_REQUIRE(p >= 1 && q >= 1 && p <= 64 && q <= 64, "tsgram: p=%d q=%d out of range", p, q);
+void tsgram2(int N, int p1, const double* X1, int64_t ldx1, int p2, const double* X2, int64_t ldx2, int q,
+             const double* Y, int64_t ldy, double* G, int ldg, hipStream_t st) {
+  const int p = p1 + p2;
+  TTF_REQUIRE(p1 >= 0 && p2 >= 0 && p >= 1 && q >= 1 && p <= 64 && q <= 64, "tsgram: p=%d+%d q=%d out of range",
+              p1, p2, q);
   const int nchunks = (int)ceil_div(std::max(N, 1), kTsgRows);
   const int nparts = std::min(nchunks, 256);   // tsgram_reduce takes at most 256 partials
   double* part = (double*)arena(kArenaGram, sizeof(double) * (size_t)nparts * p * q);
-  hipLaunchKernelGGL(tsgram_partial, dim3(nparts), dim3(256), 0, st, N, p, q, X, ldx, Y, ldy, part);
+  hipLaunchKernelGGL(tsgram_partial, dim3(nparts), dim3(256), 0, st, N, p, q, X1, ldx1, p1, p2 > 0 ? X2 : X1,
+                     p2 > 0 ? ldx2 : ldx1, Y, ldy, part);
   TTF_HIP(hipGetLastError());
   hipLaunchKernelGGL(tsgram_reduce, dim3((unsigned)ceil_div(p * q, 256 / kTsgSub)), dim3(256), 0, st, p, q,
                      nparts, part, G, ldg);
   TTF_HIP(hipGetLastError());
+}
+
+void tsgram(int N, int p, int q, const double* X, int64_t ldx, const double* Y, int64_t ldy,
+            double* G, int ldg, hipStream_t st) {
+  tsgram2(N, p, X, ldx, 0, X, ldx, q, Y, ldy, G, ldg, st);
 }
 
 // ------------------------------------------------------------------ small eig

@@ -81,27 +81,61 @@ __global__ __launch_bounds__(256) void svqb_kernel(int k, const double* __restri
 // W spans, to rounding, and the Rayleigh-Ritz step depends on the span
 // only; the factorisation takes k short steps where the eigen-decomposition
 // takes several Jacobi sweeps.
+// With q > 0, G (ld ldg) is [Q Wr]^T Wr for Q with q orthonormal (or zero)
+// columns, and the factorised Gram is that of Wr - Q (Q^T Wr), which is
+// Wr^T Wr - (Q^T Wr)^T (Q^T Wr): the projection's second pass and the
+// orthonormalisation then share one pass over the vectors.  A direction
+// whose squared norm falls below 1e-8 of Wr's own in that subtraction has
+// lost its relative accuracy to cancellation and is dropped.
 constexpr int kMaxB = 32;   // block columns handled by cholqb_kernel
-__global__ __launch_bounds__(64) void cholqb_kernel(int k, const double* __restrict__ G, double* __restrict__ T,
-                                                    int* __restrict__ mask) {
+__global__ __launch_bounds__(64) void cholqb_kernel(int k, const double* __restrict__ G, int ldg, int q,
+                                                    double* __restrict__ T, int* __restrict__ mask) {
   __shared__ double A[kMaxB][kMaxB + 1];   // Schur complements (permuted order)
   __shared__ double L[kMaxB][kMaxB + 1];   // Cholesky factor (permuted order)
+  __shared__ double Gs[3 * kMaxB][kMaxB + 1];   // G (q + k rows)
   __shared__ double sc[kMaxB], lc[kMaxB];
   __shared__ int perm[kMaxB];
   __shared__ int piv, rank_;
   const int lane = threadIdx.x;
+  // G into LDS, four loads in flight per lane (clamped in-range addresses)
+  {
+    const int rows = q + k, tot = rows * k;
+    for (int e0 = 0; e0 < tot; e0 += 4 * 64) {
+      double v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int e = min(e0 + lane + 64 * u, tot - 1);
+        v[u] = G[(e % rows) + (int64_t)ldg * (e / rows)];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int e = e0 + lane + 64 * u;
+        if (e < tot) Gs[e % rows][e / rows] = v[u];
+      }
+    }
+  }
+  __syncthreads();
+  // the block's Gram into L (scratch until the factorisation)
+  for (int e = lane; e < k * k; e += 64) {
+    const int i = e % k, j = e / k;
+    double g = Gs[q + i][j];
+    for (int l = 0; l < q; ++l) g -= Gs[l][i] * Gs[l][j];
+    L[i][j] = g;
+  }
+  __syncthreads();
   for (int i = lane; i < k; i += 64) {
-    const double g = G[i + k * i];
-    sc[i] = g > 0.0 ? 1.0 / sqrt(g) : 0.0;
+    const double g = L[i][i];
+    sc[i] = (g > 0.0 && g >= 1e-8 * Gs[q + i][i]) ? 1.0 / sqrt(g) : 0.0;
     perm[i] = i;
   }
   if (lane == 0) rank_ = k;
   __syncthreads();
   for (int e = lane; e < k * k; e += 64) {
     const int i = e % k, j = e / k;
-    A[i][j] = 0.5 * (G[i + k * j] + G[j + k * i]) * sc[i] * sc[j];
-    L[i][j] = 0.0;
+    A[i][j] = 0.5 * (L[i][j] + L[j][i]) * sc[i] * sc[j];
   }
+  __syncthreads();
+  for (int e = lane; e < k * k; e += 64) L[e % k][e / k] = 0.0;
   __syncthreads();
   for (int j = 0; j < k; ++j) {
     if (lane == 0) {
@@ -173,12 +207,15 @@ static bool eig_orth_svqb() {
   return v;
 }
 
-void orth_block(int k, const double* G, double* T, int* mask, hipStream_t st) {
+// G: the k x k Gram (ld k), or with q > 0 (the Cholesky form only) the
+// (q + k) x k product [Q Wr]^T Wr (ld ldg) of cholqb_kernel.
+void orth_block(int k, const double* G, double* T, int* mask, hipStream_t st, int ldg = 0, int q = 0) {
   if (eig_orth_svqb()) {
+    TTF_REQUIRE(q == 0, "block_eig: the SVQB orthonormalisation takes a plain Gram");
     hipLaunchKernelGGL(svqb_kernel, dim3(1), dim3(256), 0, st, k, G, T, mask);
   } else {
-    TTF_REQUIRE(k <= kMaxB, "block_eig: block of %d columns exceeds %d", k, kMaxB);
-    hipLaunchKernelGGL(cholqb_kernel, dim3(1), dim3(64), 0, st, k, G, T, mask);
+    TTF_REQUIRE(k <= kMaxB && q <= 2 * kMaxB, "block_eig: block of %d columns exceeds %d", k, kMaxB);
+    hipLaunchKernelGGL(cholqb_kernel, dim3(1), dim3(64), 0, st, k, G, q > 0 ? ldg : k, q, T, mask);
   }
   TTF_HIP(hipGetLastError());
 }
@@ -724,14 +761,23 @@ EigStats block_eig(const LocalOp& op, int nev, int guard, double* Xout, double* 
     double* X = S, *W = S + colsz * 2 * bs;
     double* AS = S + colsz * 3 * bs;
     double* AW = AS + colsz * 2 * bs;
-    // Wr <- Wr - [X P] ([X P]^T Wr), twice (P is zero before the first update)
-    for (int pass = 0; pass < 2; ++pass) {
+    // Wr <- Wr - [X P] ([X P]^T Wr), twice (P is zero before the first
+    // update).  With the Cholesky orthonormalisation the second pass's Gram
+    // is [X P Wr]^T Wr, one pass over the vectors, from which the
+    // orthonormalisation kernel forms the projected block's own Gram
+    tsgram(N, 2 * bs, bs, X, ld, Wr, ld, Gm, 2 * bs, st);
+    gemm_launch(colmajor<double>('N', 'N', N, bs, 2 * bs, -1.0, X, ld, Gm, 2 * bs, 1.0, Wr, ld), st);
+    if (eig_orth_svqb()) {
       tsgram(N, 2 * bs, bs, X, ld, Wr, ld, Gm, 2 * bs, st);
       gemm_launch(colmajor<double>('N', 'N', N, bs, 2 * bs, -1.0, X, ld, Gm, 2 * bs, 1.0, Wr, ld), st);
+      tsgram(N, bs, bs, Wr, ld, Wr, ld, Gm, bs, st);
+      orth_block(bs, Gm, Tm, maskW, st);
+    } else {
+      tsgram2(N, 2 * bs, X, ld, bs, Wr, ld, bs, Wr, ld, Gm, 3 * bs, st);
+      gemm_launch(colmajor<double>('N', 'N', N, bs, 2 * bs, -1.0, X, ld, Gm, 3 * bs, 1.0, Wr, ld), st);
+      orth_block(bs, Gm, Tm, maskW, st, 3 * bs, 2 * bs);
     }
-    // SVQB: W = Wr T, then A W
-    tsgram(N, bs, bs, Wr, ld, Wr, ld, Gm, bs, st);
-    orth_block(bs, Gm, Tm, maskW, st);
+    // W = Wr T, then A W
     mul_small(N, bs, bs, Wr, ld, Tm, bs, W, ld, 0.0, st);
     if (aw32) mix->apply_block(bs, W, ld, AW, ld, st);
     else op.apply(bs, W, ld, AW, ld, st);
