@@ -272,7 +272,10 @@ typedef struct {
  * operator cores (ra_k, n_k, m_k, ra_k+1), crY0 the initial block TT with
  * ranks ry(d+1) (ry(d+1) == B, in/out), eps the tolerance, rmax the rank
  * cap, lambda(B) out.  kickrank is accepted and ignored, as in the
- * reference; nswp <= 0 means 20, max_full_size <= 0 means 2000.  The
+ * reference; nswp <= 0 means 20, max_full_size <= 0 means 2000.  verb > 0
+ * prints a line per local solve and per sweep on stderr (the reference's
+ * progress lines); verb > 1 adds a line per truncation with the Jacobi
+ * sweeps of its SVD and the kept rank.  The
  * result cores (what the reference leaves in tt_block_eig::result_core)
  * are written to `result` (capacity result_cap doubles; pass NULL to query
  * the size, returned in *result_size).  Host pointers; device-resident run. */
